@@ -1,0 +1,173 @@
+/*
+ * treesvd.h -- C ABI of the MI355X-native Tree-SVD update->embedding path.
+ *
+ * The reference (DoThingYo/Tree-Embedding, directory Code/) is a CPU program
+ * whose hot path is reached through C/C++ calls.  Each entry point below names
+ * the reference interface it replaces (file:line).  HIP lives behind this
+ * header only: no torch or HIP types cross it; every function returns an int
+ * status (TSVD_OK = 0, negative = error; tsvd_last_error() gives the text of
+ * the calling thread's last error).  Handles own device memory; one handle is
+ * not re-entrant, distinct handles are independent.  Unless a function says
+ * otherwise it is synchronous: outputs are complete when it returns.
+ */
+#ifndef TREESVD_H
+#define TREESVD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSVD_OK 0
+#define TSVD_EINVAL (-1)   /* bad argument (ids out of range, sizes, modes) */
+#define TSVD_ENOMEM (-2)   /* device or host allocation failed            */
+#define TSVD_EHIP (-3)     /* a HIP / ROCm library call failed            */
+#define TSVD_ESTATE (-4)   /* call out of sequence for this handle        */
+
+#define TSVD_DIR_FWD 0        /* out-edges (undirected: the adjacency)   */
+#define TSVD_DIR_TRANSPOSE 1  /* in-edges (directed graphs only)         */
+
+#define TSVD_REFRESH_INIT 0     /* static: r = e_s, p = 0                */
+#define TSVD_REFRESH_DYNAMIC 1  /* insertion update over the new slots   */
+
+#define TSVD_FP32 0
+#define TSVD_FP64 1
+
+#define TSVD_TRANSFORM_NC 0   /* log10(x/rmax) if x > rmax else 0      */
+#define TSVD_TRANSFORM_LP 1   /* log10(1 + x/rmax) on every non-zero   */
+
+const char *tsvd_last_error(void);
+int tsvd_device_info(char *name, int name_len, int64_t *hbm_total, int64_t *hbm_free);
+/* Device-buffer plumbing for callers that stage inputs in HBM (no reference
+ * counterpart).  kind: 0 host->device, 1 device->host, 2 device->device. */
+int tsvd_device_alloc(int64_t bytes, void **out);
+int tsvd_device_free(void *p);
+int tsvd_memcpy(void *dst, const void *src, int64_t bytes, int kind);
+
+/* ------------------------------------------------------------------ graph
+ * Replaces UGraph::initializeDynamicUGraph / inputDynamicGraph
+ * (Graph_dynamic.h:513-528, 530-605) and Graph::initializeDirectedDynamicGraph
+ * / inputDirectedDynamicGraph (Graph_dynamic.h:90-116, 119-238).  A device CSR
+ * (out-rows, plus in-rows when directed) with per-row append slack.  Append
+ * keeps the reference's row order bit-exactly: each row gains its new
+ * neighbours in batch order; undirected edge (u,v) appends v to row u then u
+ * to row v.  `former` degrees are snapshotted at every append.            */
+typedef struct tsvd_graph tsvd_graph;
+
+int tsvd_graph_create(int32_t n, int directed, double slack, tsvd_graph **out);
+int tsvd_graph_destroy(tsvd_graph *g);
+/* src/dst: `m` int32 ids; on_device != 0 means they are device pointers. */
+int tsvd_graph_append(tsvd_graph *g, const int32_t *src, const int32_t *dst, int64_t m, int on_device);
+int64_t tsvd_graph_num_edges(const tsvd_graph *g);
+/* Host copies of degree / former degree (n int32 each; either may be NULL). */
+int tsvd_graph_degrees(const tsvd_graph *g, int dir, int32_t *deg, int32_t *former);
+/* Compact host CSR of one direction: rowptr[n+1], col[rowptr[n]]; col may be
+ * NULL to query the size through rowptr[n]. */
+int tsvd_graph_export_csr(const tsvd_graph *g, int dir, int64_t *rowptr, int32_t *col);
+/* The snapshot-file reader of Graph_dynamic.h:127-130 / 539-542 ("from to"
+ * pairs); dup_last=1 keeps the reference's duplicate of the last edge of a
+ * file that ends in a newline.  With src == NULL only *count is set.       */
+int tsvd_read_edge_file(const char *path, int dup_last, int32_t *src, int32_t *dst, int64_t cap, int64_t *count);
+
+/* -------------------------------------------------------------------- PPR
+ * Replaces nodegree_DenseUndirected_Refresh_PPR_initialization
+ * (ppr_computation_store_dynamic.h:2035-2121), nodegree_DenseDynamicForwardPush
+ * (:1394-1498), nodegree_DenseDirected_Refresh_PPR_initialization
+ * (:3811-3898) and its _Transpose (:4133-4228), nodegree_DenseDirected-
+ * DynamicForwardPush (:2945-3030) and ...Transpose (:3356-3437), each called
+ * by the drivers per source range [start, end) (e.g. Tree_u_NC.cpp:529-600).
+ * Reserves/residues stay on the device, vertex-major (p[v][s]).            */
+typedef struct tsvd_ppr tsvd_ppr;
+
+int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t nsrc, double alpha, double rmax,
+                    int precision, tsvd_ppr **out);
+int tsvd_ppr_destroy(tsvd_ppr *s);
+/* mode TSVD_REFRESH_INIT / _DYNAMIC for sources [src_begin, src_end). */
+int tsvd_ppr_refresh(tsvd_ppr *s, int mode, int dir, int32_t src_begin, int32_t src_end);
+int tsvd_ppr_push(tsvd_ppr *s, int dir, int32_t src_begin, int32_t src_end);
+/* Host copy of p or r (what = 0: reserve p, 1: residue r) as FP64, source-
+ * major [nsrc][n] like the reference's float** pi / residue.              */
+int tsvd_ppr_read(tsvd_ppr *s, int dir, int what, double *out);
+/* Counters of the last push on `dir`: rounds, popped (vertex, source) pairs,
+ * edge relaxations (pairs x degree), and the push kernel time in ms. */
+int tsvd_ppr_last_push_stats(tsvd_ppr *s, int dir, int64_t *rounds, int64_t *pops, int64_t *relax,
+                             double *kernel_ms);
+
+/* ------------------------------------------------------ frPCA (drop-in)
+ * The reference's own types (matrix_vector_functions_intel_mkl.h:20-23,
+ * matrix_vector_functions_intel_mkl_ext.h:22-33): col-major FP64 `mat`,
+ * 1-based CSR `mat_csr` with pointerB/pointerE.  Define TSVD_NO_MAT_TYPES
+ * when the reference's headers already declared them.                     */
+#ifndef TSVD_NO_MAT_TYPES
+typedef struct { int nrows, ncols; double *d; } mat;
+typedef struct { int nnz; int nrows, ncols; double *values; int *cols; int *pointerB, *pointerE; } mat_csr;
+#endif
+/* Replaces frPCA (frpca.c:239-326, declared matrix_vector_functions_intel_mkl_
+ * ext.h:75).  Caller allocates *U (m x k), *S (k x 1), *V (n x k); S comes out
+ * ascending as in the reference.  void, like the reference; failures are
+ * reported through tsvd_last_error() and leave the outputs zero.          */
+void frPCA(mat_csr *A, mat **U, mat **S, mat **V, int k, int q);
+/* Same computation with a status code and an explicit sketch seed. */
+int tsvd_frpca_csr(const mat_csr *A, mat *U, mat *S, mat *V, int k, int q, uint64_t seed);
+
+/* ------------------------------------------------------------- pipeline
+ * The per-snapshot driver of Tree_u_NC.cpp:507-866, Tree_d_NC.cpp:611-1132,
+ * Tree_u_LP.cpp:664-925 (the "plain GPU method"): ingest -> refresh -> push
+ * -> block transform + lazy dirty test (ppr:2527-2826) -> frPCA of every
+ * dirty block (Tree_u_NC.cpp:75-226) -> dirty-path merges up the fan-out
+ * tree (Tree_u_NC.cpp:238-262, 723-850) -> root embedding.                */
+typedef struct tsvd_pipeline tsvd_pipeline;
+
+typedef struct {
+    int32_t n;            /* vertices                                      */
+    int32_t directed;     /* 0 undirected (UGraph), 1 directed (Graph)     */
+    int32_t nsrc;         /* |S|                                           */
+    const int32_t *sources;
+    double alpha, rmax;   /* teleport probability, push threshold          */
+    int32_t precision;    /* TSVD_FP32 / TSVD_FP64 for p, r and the matrix */
+    int32_t nparts;       /* column blocks                                 */
+    int32_t fanout;       /* merge-tree fan-out                            */
+    int32_t rank;         /* k (the reference's d)                         */
+    int32_t q;            /* frPCA passes (even; 6 = two power iterations) */
+    int32_t transform;    /* TSVD_TRANSFORM_NC / _LP                        */
+    double delta;         /* lazy-update threshold                         */
+    int32_t dyn_start;    /* first snapshot (1-based) with dynamic updates */
+    int32_t lazy_start;   /* first snapshot with the dirty test            */
+    double slack;         /* CSR append slack fraction                     */
+    uint64_t seed;        /* sketch seed                                   */
+} tsvd_config;
+
+typedef struct {
+    int32_t snapshot;     /* 1-based index of the snapshot just processed   */
+    int32_t ndirty;       /* leaf blocks refactored                         */
+    int32_t nmerged;      /* internal tree nodes recomputed (incl. root)    */
+    int32_t reused;       /* 1: no dirty block, previous embedding kept     */
+    int64_t push_rounds, push_pops, push_relax;
+    double ms_total, ms_ingest, ms_refresh, ms_push, ms_blocks, ms_frpca, ms_merge;
+} tsvd_snapshot_stats;
+
+int tsvd_pipeline_create(const tsvd_config *cfg, tsvd_pipeline **out);
+int tsvd_pipeline_destroy(tsvd_pipeline *p);
+/* One snapshot: append the batch (host or device ids), update, embed. */
+int tsvd_pipeline_snapshot(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, int64_t m, int on_device,
+                           tsvd_snapshot_stats *stats);
+/* Root embedding (nsrc x rank, row-major FP64): NC U*sqrt|S|, LP U. */
+int tsvd_pipeline_embedding(tsvd_pipeline *p, double *E);
+/* Root singular values (rank, descending). */
+int tsvd_pipeline_root_sigma(tsvd_pipeline *p, double *sigma);
+/* Handles owned by the pipeline (valid until it is destroyed). */
+tsvd_graph *tsvd_pipeline_graph(tsvd_pipeline *p);
+tsvd_ppr *tsvd_pipeline_ppr(tsvd_pipeline *p);
+/* Host copy of leaf block j's factored matrix (nsrc x width, col-major FP64)
+ * and of the per-block norms ||A||, ||A_new - A_old||, residual. */
+int tsvd_pipeline_block_matrix(tsvd_pipeline *p, int32_t j, double *A);
+int tsvd_pipeline_block_norms(tsvd_pipeline *p, double *a_norm, double *e_norm, double *resid, int32_t *dirty);
+/* Host copy of one tree node's factor U*S (nsrc x rank, col-major FP64);
+ * level 0 = leaves. */
+int tsvd_pipeline_node(tsvd_pipeline *p, int32_t level, int32_t index, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,347 @@
+// blocks.hip -- column-block transform, lazy dirty test and block rebuild;
+// small elementwise kernels of the factorisation (sketch, eigen scaling).
+//
+// Replaces Log_sparse_matrix_entries[_LP] (ppr_computation_store_dynamic.h:
+// 2527-2561, 2598-2638) and ..._with_norm_computation[_LP] (:2649-2732,
+// :2742-2826), run there on one thread per block over a dense |S| x w_j FP64
+// copy of the reserves.  Here the block matrix is never formed twice: one
+// pass over the reserves (p, or p + p^T for directed graphs, the matrix of
+// Tree_d_NC.cpp:689/816) computes ||A_new||_F^2 and ||A_new - A_old||_F^2 of
+// every block, reading the stored A_old through a per-vertex slot map; only
+// blocks found dirty are then re-gathered.  The stored matrix keeps only
+// active columns (a vertex with an edge, or a source); every other column is
+// identically zero because its reserve is zero.
+#include "tsvd_pipeline.h"
+
+#include <cmath>
+
+namespace tsvd {
+
+namespace {
+
+constexpr int TPB = 256;
+constexpr int VPB = 32;   // vertices per workgroup in the norm pass
+
+template <class T>
+__device__ inline T xform(T x, T rm, int transform)
+{
+    if (transform == TSVD_TRANSFORM_NC) return x > rm ? log10(x / rm) : T(0);   // ppr:2546-2553
+    return x > T(0) ? log10(T(1) + x / rm) : T(0);                              // ppr:2617-2630
+}
+
+__device__ inline double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_block_norms(const T *__restrict__ p0, const T *__restrict__ p1,
+                                                     const T *__restrict__ A, const int32_t *__restrict__ slot,
+                                                     int32_t n, int32_t nsrc, int32_t s_pad, int32_t nparts,
+                                                     int32_t common, int transform, T rm, double *__restrict__ norms)
+{
+    const int32_t j = blockIdx.y;
+    const int64_t start = (int64_t)j * common;
+    const int64_t width = (j == nparts - 1) ? n - start : common;
+    const int64_t v0 = start + (int64_t)blockIdx.x * VPB;
+    if (v0 >= start + width) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double a2 = 0.0, e2 = 0.0;
+    for (int vv = wave; vv < VPB; vv += TPB / 64) {
+        const int64_t v = v0 + vv;
+        if (v >= start + width) break;
+        const int32_t sl = slot[v];
+        const T *Ao = sl >= 0 ? A + (start + sl) * s_pad : nullptr;
+        const T *x0 = p0 + v * s_pad;
+        const T *x1 = p1 ? p1 + v * s_pad : nullptr;
+        for (int s = lane; s < nsrc; s += 64) {
+            T x = x0[s];
+            if (x1) x += x1[s];
+            const T a = xform<T>(x, rm, transform);
+            const T ao = Ao ? Ao[s] : T(0);
+            a2 += (double)a * (double)a;
+            const double d = (double)a - (double)ao;
+            e2 += d * d;
+        }
+    }
+    a2 = wave_sum(a2);
+    e2 = wave_sum(e2);
+    __shared__ double red[2][TPB / 64];
+    if (lane == 0) {
+        red[0][wave] = a2;
+        red[1][wave] = e2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0, se = 0;
+        for (int w = 0; w < TPB / 64; w++) {
+            sa += red[0][w];
+            se += red[1][w];
+        }
+        atomicAdd(norms + 2 * j, sa);
+        atomicAdd(norms + 2 * j + 1, se);
+    }
+}
+
+// Compact slots of the active columns of each listed block (one workgroup per
+// block; vertices in id order keep their order).
+__global__ void __launch_bounds__(TPB) k_slots(const int32_t *__restrict__ blocks, const uint8_t *__restrict__ active,
+                                               int32_t n, int32_t nparts, int32_t common, int32_t *__restrict__ slot,
+                                               int32_t *__restrict__ colmap, int32_t *__restrict__ ncols)
+{
+    const int32_t j = blocks[blockIdx.x];
+    const int64_t start = (int64_t)j * common;
+    const int64_t width = (j == nparts - 1) ? n - start : common;
+    const int64_t per = (width + TPB - 1) / TPB;
+    const int64_t a = start + threadIdx.x * per;
+    const int64_t b = min(start + width, a + per);
+    int32_t cnt = 0;
+    for (int64_t v = a; v < b; v++) cnt += active[v] ? 1 : 0;
+    __shared__ int32_t sc[TPB];
+    sc[threadIdx.x] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t run = 0;
+        for (int t = 0; t < TPB; t++) {
+            const int32_t c = sc[t];
+            sc[t] = run;
+            run += c;
+        }
+        ncols[j] = run;
+    }
+    __syncthreads();
+    int32_t pos = sc[threadIdx.x];
+    for (int64_t v = a; v < b; v++) {
+        if (active[v]) {
+            slot[v] = pos;
+            colmap[start + pos] = (int32_t)v;
+            pos++;
+        } else {
+            slot[v] = -1;
+        }
+    }
+}
+
+// A[:, c] of each listed block = transform of the reserves of its c-th active
+// vertex; one wave per column, lanes over sources (padding rows set to 0).
+template <class T>
+__global__ void __launch_bounds__(TPB) k_gather(const int32_t *__restrict__ blocks, const T *__restrict__ p0,
+                                                const T *__restrict__ p1, T *__restrict__ A,
+                                                const int32_t *__restrict__ colmap, const int32_t *__restrict__ ncols,
+                                                int32_t nsrc, int32_t s_pad, int32_t common, int transform, T rm)
+{
+    const int32_t j = blocks[blockIdx.y];
+    const int64_t start = (int64_t)j * common;
+    const int64_t c = (int64_t)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
+    if (c >= ncols[j]) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t v = colmap[start + c];
+    T *col = A + (start + c) * s_pad;
+    const T *x0 = p0 + v * s_pad;
+    const T *x1 = p1 ? p1 + v * s_pad : nullptr;
+    for (int s = lane; s < s_pad; s += 64) {
+        T a = T(0);
+        if (s < nsrc) {
+            T x = x0[s];
+            if (x1) x += x1[s];
+            a = xform<T>(x, rm, transform);
+        }
+        col[s] = a;
+    }
+}
+
+__global__ void k_active(const int32_t *__restrict__ deg0, const int32_t *__restrict__ deg1,
+                         const uint8_t *__restrict__ is_source, int32_t n, uint8_t *__restrict__ active)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    int32_t d = deg0[v] + (deg1 ? deg1[v] : 0);
+    active[v] = (d > 0 || is_source[v]) ? 1 : 0;
+}
+
+__global__ void k_mark_sources(const int32_t *__restrict__ src, int32_t nsrc, uint8_t *__restrict__ is_source)
+{
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsrc) is_source[src[s]] = 1;
+}
+
+__device__ inline uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Standard normal sketch entries from a counter-based hash (Box-Muller).
+template <class T>
+__global__ void k_gaussian(T *__restrict__ X, int64_t rows, int64_t cols, int64_t ld, uint64_t seed)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * cols) return;
+    const int64_t c = t / rows, i = t - c * rows;
+    const uint64_t h = mix64(seed ^ mix64((uint64_t)t));
+    const double u1 = ((double)(h >> 40) + 1.0) / 16777217.0;           // (0, 1]
+    const double u2 = (double)(h & 0xFFFFFFull) / 16777216.0;           // [0, 1)
+    X[c * ld + i] = (T)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+}
+
+__global__ void k_eig_rotation(double *__restrict__ W, const double *__restrict__ V, const double *__restrict__ lam,
+                               int l, double eps_abs, double eps_rel)
+{
+    const int b = blockIdx.x;
+    const double *Vb = V + (int64_t)b * l * l;
+    double *Wb = W + (int64_t)b * l * l;
+    const double *lb = lam + (int64_t)b * l;
+    const double smax = sqrt(fmax(lb[l - 1], 0.0));
+    const double guard = fmax(eps_abs, eps_rel * smax);
+    for (int e = threadIdx.x; e < l * l; e += blockDim.x) {
+        const int c = e / l;
+        const double s = sqrt(fmax(lb[c], 0.0));
+        Wb[e] = s > guard ? Vb[e] / s : 0.0;
+    }
+}
+
+__global__ void k_eig_select(double *__restrict__ W, const double *__restrict__ V, const double *__restrict__ lam,
+                             int l, int k, int mode, int ascending, double eps)
+{
+    const int b = blockIdx.x;
+    const double *Vb = V + (int64_t)b * l * l;
+    double *Wb = W + (int64_t)b * l * k;
+    const double *lb = lam + (int64_t)b * l;
+    for (int e = threadIdx.x; e < l * k; e += blockDim.x) {
+        const int c = e / l, r = e - c * l;
+        const int src = ascending ? l - k + c : l - 1 - c;
+        const double s = sqrt(fmax(lb[src], 0.0));
+        double f = 1.0;
+        if (mode == 0) f = s > eps ? s : 0.0;
+        else if (mode == 1) f = s > eps ? 1.0 / sqrt(s) : 0.0;
+        else if (mode == 2) f = s > eps ? 1.0 / s : 0.0;
+        Wb[e] = Vb[(int64_t)src * l + r] * f;
+    }
+}
+
+}  // namespace
+
+void blocks_setup(Blocks &b, int32_t n, int32_t nparts, int32_t nsrc, int32_t s_pad, int precision, int transform,
+                  double rmax, const std::vector<int32_t> &sources, hipStream_t st)
+{
+    TSVD_REQUIRE(nparts >= 1 && nparts <= n, "nparts must be in [1, n]");
+    b.nparts = nparts;
+    b.n = n;
+    b.nsrc = nsrc;
+    b.s_pad = s_pad;
+    b.common = n / nparts;
+    b.precision = precision;
+    b.esize = precision == TSVD_FP64 ? 8 : 4;
+    b.transform = transform;
+    b.rmax = rmax;
+    b.start.resize(nparts);
+    b.width.resize(nparts);
+    for (int32_t j = 0; j < nparts; j++) {
+        b.start[j] = (int64_t)j * b.common;
+        b.width[j] = (j == nparts - 1) ? n - b.start[j] : b.common;
+    }
+    b.ncols.assign(nparts, 0);
+    b.a2.assign(nparts, 0.0);
+    b.e2.assign(nparts, 0.0);
+    b.resid.assign(nparts, 0.0);
+    b.A.alloc((size_t)n * s_pad * b.esize);
+    b.slot.alloc((size_t)n);
+    b.colmap.alloc((size_t)n);
+    b.d_ncols.alloc((size_t)nparts);
+    b.active.alloc((size_t)n);
+    b.is_source.alloc((size_t)n);
+    b.norms.alloc((size_t)2 * nparts);
+    b.d_dirty.alloc((size_t)nparts);
+    TSVD_HIP(hipMemsetAsync(b.A.p, 0, b.A.bytes(), st));
+    TSVD_HIP(hipMemsetAsync(b.slot.p, 0xFF, b.slot.bytes(), st));   // -1: no stored column
+    TSVD_HIP(hipMemsetAsync(b.d_ncols.p, 0, b.d_ncols.bytes(), st));
+    TSVD_HIP(hipMemsetAsync(b.is_source.p, 0, b.is_source.bytes(), st));
+    DBuf<int32_t> ds((size_t)nsrc);
+    TSVD_HIP(hipMemcpyAsync(ds.p, sources.data(), sizeof(int32_t) * nsrc, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_mark_sources, dim3((unsigned)ceil_div(nsrc, TPB)), dim3(TPB), 0, st, ds.p, nsrc,
+                       b.is_source.p);
+    TSVD_HIP(hipStreamSynchronize(st));
+}
+
+void blocks_norms(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_active, dim3((unsigned)ceil_div(b.n, TPB)), dim3(TPB), 0, st, g->dir[0].deg.p,
+                       g->ndir > 1 ? g->dir[1].deg.p : nullptr, b.is_source.p, b.n, b.active.p);
+    TSVD_HIP(hipMemsetAsync(b.norms.p, 0, b.norms.bytes(), st));
+    int64_t maxw = 0;
+    for (auto w : b.width) maxw = std::max(maxw, w);
+    dim3 grid((unsigned)ceil_div(maxw, VPB), (unsigned)b.nparts);
+    if (b.precision == TSVD_FP64)
+        hipLaunchKernelGGL(k_block_norms<double>, grid, dim3(TPB), 0, st, (const double *)p0, (const double *)p1,
+                           (const double *)b.A.p, b.slot.p, b.n, b.nsrc, b.s_pad, b.nparts, b.common, b.transform,
+                           b.rmax, b.norms.p);
+    else
+        hipLaunchKernelGGL(k_block_norms<float>, grid, dim3(TPB), 0, st, (const float *)p0, (const float *)p1,
+                           (const float *)b.A.p, b.slot.p, b.n, b.nsrc, b.s_pad, b.nparts, b.common, b.transform,
+                           (float)b.rmax, b.norms.p);
+    TSVD_HIP(hipGetLastError());
+    std::vector<double> h((size_t)2 * b.nparts);
+    TSVD_HIP(hipMemcpyAsync(h.data(), b.norms.p, b.norms.bytes(), hipMemcpyDeviceToHost, st));
+    TSVD_HIP(hipStreamSynchronize(st));
+    for (int32_t j = 0; j < b.nparts; j++) {
+        b.a2[j] = h[2 * j];
+        b.e2[j] = h[2 * j + 1];
+    }
+}
+
+void blocks_rebuild(Blocks &b, const std::vector<int32_t> &dirty, const void *p0, const void *p1, hipStream_t st)
+{
+    if (dirty.empty()) return;
+    const int nd = (int)dirty.size();
+    TSVD_HIP(hipMemcpyAsync(b.d_dirty.p, dirty.data(), sizeof(int32_t) * nd, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_slots, dim3(nd), dim3(TPB), 0, st, b.d_dirty.p, b.active.p, b.n, b.nparts, b.common,
+                       b.slot.p, b.colmap.p, b.d_ncols.p);
+    int64_t maxw = 0;
+    for (int32_t j : dirty) maxw = std::max(maxw, b.width[j]);
+    dim3 grid((unsigned)ceil_div(maxw, TPB / 64), (unsigned)nd);
+    if (b.precision == TSVD_FP64)
+        hipLaunchKernelGGL(k_gather<double>, grid, dim3(TPB), 0, st, b.d_dirty.p, (const double *)p0,
+                           (const double *)p1, (double *)b.A.p, b.colmap.p, b.d_ncols.p, b.nsrc, b.s_pad, b.common,
+                           b.transform, b.rmax);
+    else
+        hipLaunchKernelGGL(k_gather<float>, grid, dim3(TPB), 0, st, b.d_dirty.p, (const float *)p0,
+                           (const float *)p1, (float *)b.A.p, b.colmap.p, b.d_ncols.p, b.nsrc, b.s_pad, b.common,
+                           b.transform, (float)b.rmax);
+    TSVD_HIP(hipGetLastError());
+    TSVD_HIP(hipMemcpyAsync(b.ncols.data(), b.d_ncols.p, sizeof(int32_t) * b.nparts, hipMemcpyDeviceToHost, st));
+    TSVD_HIP(hipStreamSynchronize(st));
+}
+
+void gaussian_fill(void *X, int precision, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, hipStream_t st)
+{
+    const int64_t tot = rows * cols;
+    if (tot <= 0) return;
+    if (precision == TSVD_FP64)
+        hipLaunchKernelGGL(k_gaussian<double>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st, (double *)X,
+                           rows, cols, ld, seed);
+    else
+        hipLaunchKernelGGL(k_gaussian<float>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st, (float *)X, rows,
+                           cols, ld, seed);
+    TSVD_HIP(hipGetLastError());
+}
+
+void eig_rotation(double *W, const double *V, const double *lam, int l, int count, double eps_abs, double eps_rel,
+                  hipStream_t st)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_eig_rotation, dim3(count), dim3(TPB), 0, st, W, V, lam, l, eps_abs, eps_rel);
+    TSVD_HIP(hipGetLastError());
+}
+
+void eig_select(double *W, const double *V, const double *lam, int l, int k, int count, int mode, int ascending,
+                double eps, hipStream_t st)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_eig_select, dim3(count), dim3(TPB), 0, st, W, V, lam, l, k, mode, ascending, eps);
+    TSVD_HIP(hipGetLastError());
+}
+
+}  // namespace tsvd

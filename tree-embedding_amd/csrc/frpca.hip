@@ -1,0 +1,316 @@
+// frpca.hip -- blockwise randomized SVD (frPCA) on the device.
+//
+// Replaces frPCA (frpca.c:239-326) with its helpers eigSVD (frpca.c:99-136),
+// LUfraction (frpca.c:4-41), initialize_random_matrix (matrix_vector_functions_
+// intel_mkl.c:479-512) and the MKL sparse products (..._ext.c:364-398), called
+// by the reference once per dirty block (Tree_u_NC.cpp:679-710, serially).
+//
+// All dirty blocks are factored together: every product below is one grouped
+// MFMA launch over the blocks (gemm.hip) and every small eigenproblem one
+// batched FP64 solve.  Passes follow frpca.c (q even, s = 5, l = k + s):
+//     Qt <- Gaussian (nc x l)                 (counter-based, seeded)
+//     Q  <- A Qt ;  Q <- orth(Q)
+//     (q-2)/2 times:  Qt <- A^T Q ;  Q <- A Qt ;  Q <- orth(Q)
+//     Qt <- A^T Q ;  [V, lam] = eig(Qt^T Qt)   (eigSVD of frpca.c:99-136)
+//     U = Q V[:, top k],  S = sqrt(lam[top k]),  V = Qt V[:, top k] / S.
+// orth(Q) is eigSVD's own orthonormalisation, Q V diag(lam^-1/2) with the FP64
+// Gram of Q, used at every step (the reference uses LU normalisation between
+// power iterations and eigSVD on the last one; both only re-basis span(Q)).
+// Products run in the matrix precision (FP32 MFMA for FP32 blocks, FP64 MFMA
+// for FP64 blocks); Grams and eigensolves are FP64.
+#include "tsvd_pipeline.h"
+
+namespace tsvd {
+
+Eig::Eig()
+{
+    if (rocblas_create_handle(&h_) != rocblas_status_success)
+        throw Error{set_error(TSVD_EHIP, "rocblas_create_handle failed")};
+}
+
+Eig::~Eig()
+{
+    if (h_) rocblas_destroy_handle(h_);
+}
+
+void Eig::solve(hipStream_t st, double *A, int n, int count, double *w)
+{
+    if (count <= 0) return;
+    e_.ensure((size_t)n * count);
+    info_.ensure((size_t)count);
+    if (rocblas_set_stream(h_, st) != rocblas_status_success)
+        throw Error{set_error(TSVD_EHIP, "rocblas_set_stream failed")};
+    rocblas_status s = rocsolver_dsyevd_strided_batched(h_, rocblas_evect_original, rocblas_fill_upper, n, A, n,
+                                                        (rocblas_stride)n * n, w, n, e_.p, n, info_.p, count);
+    if (s != rocblas_status_success)
+        throw Error{set_error(TSVD_EHIP, "rocsolver_dsyevd_strided_batched failed (%d)", (int)s)};
+}
+
+static GemmTypes types_for(int precision, DT a, DT b, DT c, DT acc)
+{
+    (void)precision;
+    return GemmTypes{a, b, c, acc};
+}
+
+// Q <- orth(Q) for every item: G = Q^T Q (FP64), eig, Q2 = Q V lam^-1/2, swap.
+static void orth(std::vector<FrpcaItem> &items, int precision, int32_t l, double *G, double *W, double *lam,
+                 GemmRunner &gemm, Eig &eig, hipStream_t st)
+{
+    const DT T = precision == TSVD_FP64 ? DT::F64 : DT::F32;
+    const int cnt = (int)items.size();
+    TSVD_HIP(hipMemsetAsync(G, 0, sizeof(double) * (size_t)l * l * cnt, st));
+    std::vector<GemmProb> gp;
+    for (int i = 0; i < cnt; i++) {
+        const FrpcaItem &it = items[i];
+        GemmProb p{};
+        p.A = it.Q; p.lda = it.ldq; p.transA = 1;
+        p.B = it.Q; p.ldb = it.ldq;
+        p.C = G + (size_t)i * l * l; p.ldc = l;
+        p.M = l; p.N = l; p.K = it.m; p.alpha = 1.0; p.beta = 0.0; p.kchunk = 512;
+        gp.push_back(p);
+    }
+    gemm.run(st, gp, types_for(precision, T, T, DT::F64, DT::F64), l > 80);
+    eig.solve(st, G, l, cnt, lam);
+    const double eps_rel = precision == TSVD_FP64 ? 1e-13 : 1e-6;
+    eig_rotation(W, G, lam, l, cnt, 1e-4, eps_rel, st);
+    std::vector<GemmProb> rp;
+    for (int i = 0; i < cnt; i++) {
+        FrpcaItem &it = items[i];
+        GemmProb p{};
+        p.A = it.Q; p.lda = it.ldq; p.transA = 0;
+        p.B = W + (size_t)i * l * l; p.ldb = l;
+        p.C = it.Q2; p.ldc = it.ldq;
+        p.M = it.m; p.N = l; p.K = l; p.alpha = 1.0; p.beta = 0.0;
+        rp.push_back(p);
+    }
+    gemm.run(st, rp, types_for(precision, T, DT::F64, T, DT::F64), l > 80);
+    for (auto &it : items) std::swap(it.Q, it.Q2);
+}
+
+static void product(std::vector<FrpcaItem> &items, int precision, int32_t l, bool transposed, GemmRunner &gemm,
+                     hipStream_t st)
+{
+    const DT T = precision == TSVD_FP64 ? DT::F64 : DT::F32;
+    std::vector<GemmProb> ps;
+    for (auto &it : items) {
+        GemmProb p{};
+        p.A = it.A; p.lda = it.lda; p.alpha = 1.0; p.beta = 0.0; p.N = l;
+        if (!transposed) {   // Q = A Qt
+            p.transA = 0; p.B = it.Qt; p.ldb = it.ldqt; p.C = it.Q; p.ldc = it.ldq;
+            p.M = it.m; p.K = it.nc;
+        } else {             // Qt = A^T Q
+            p.transA = 1; p.B = it.Q; p.ldb = it.ldq; p.C = it.Qt; p.ldc = it.ldqt;
+            p.M = it.nc; p.K = it.m;
+        }
+        ps.push_back(p);
+    }
+    gemm.run(st, ps, GemmTypes{T, T, T, T}, l > 80);
+}
+
+void frpca_core(std::vector<FrpcaItem> &items, int precision, int32_t l, int32_t q, double *G, double *W,
+                double *lam, GemmRunner &gemm, Eig &eig, hipStream_t st)
+{
+    TSVD_REQUIRE(q >= 1, "frPCA needs q >= 1");
+    const DT T = precision == TSVD_FP64 ? DT::F64 : DT::F32;
+    if (q % 2 == 0) {
+        for (auto &it : items) gaussian_fill(it.Qt, precision, it.nc, l, it.ldqt, it.seed, st);
+        product(items, precision, l, false, gemm, st);
+        orth(items, precision, l, G, W, lam, gemm, eig, st);
+    } else {
+        for (auto &it : items) gaussian_fill(it.Q, precision, it.m, l, it.ldq, it.seed, st);
+    }
+    const int niter = (q - 1) / 2;
+    for (int i = 1; i <= niter; i++) {
+        product(items, precision, l, true, gemm, st);
+        product(items, precision, l, false, gemm, st);
+        orth(items, precision, l, G, W, lam, gemm, eig, st);
+    }
+    product(items, precision, l, true, gemm, st);
+    // eigSVD(Qt): Gram of Qt and its eigenvectors
+    const int cnt = (int)items.size();
+    TSVD_HIP(hipMemsetAsync(G, 0, sizeof(double) * (size_t)l * l * cnt, st));
+    std::vector<GemmProb> gp;
+    for (int i = 0; i < cnt; i++) {
+        const FrpcaItem &it = items[i];
+        GemmProb p{};
+        p.A = it.Qt; p.lda = it.ldqt; p.transA = 1;
+        p.B = it.Qt; p.ldb = it.ldqt;
+        p.C = G + (size_t)i * l * l; p.ldc = l;
+        p.M = l; p.N = l; p.K = it.nc; p.alpha = 1.0; p.beta = 0.0; p.kchunk = 512;
+        gp.push_back(p);
+    }
+    gemm.run(st, gp, GemmTypes{T, T, DT::F64, DT::F64}, l > 80);
+    eig.solve(st, G, l, cnt, lam);
+}
+
+void frpca_setup(FrpcaWork &w, const Blocks &b, int32_t k, int32_t q)
+{
+    TSVD_REQUIRE(k >= 1, "rank must be positive");
+    w.k = k;
+    w.l = k + 5;
+    w.q = q;
+    const size_t es = b.esize;
+    w.Qt.alloc((size_t)b.n * w.l * es);
+    w.Q.alloc((size_t)b.nparts * b.s_pad * w.l * es);
+    w.Q2.alloc((size_t)b.nparts * b.s_pad * w.l * es);
+    w.G.alloc((size_t)b.nparts * w.l * w.l);
+    w.W.alloc((size_t)b.nparts * w.l * w.l);
+    w.lam.alloc((size_t)b.nparts * w.l);
+    TSVD_HIP(hipMemset(w.Q.p, 0, w.Q.bytes()));
+    TSVD_HIP(hipMemset(w.Q2.p, 0, w.Q2.bytes()));
+}
+
+void frpca_blocks(FrpcaWork &w, Blocks &b, const std::vector<int32_t> &dirty, void *leaves, GemmRunner &gemm,
+                  Eig &eig, uint64_t seed, hipStream_t st, std::vector<double> &sig2)
+{
+    const int32_t l = w.l, k = w.k;
+    const size_t es = b.esize;
+    std::vector<FrpcaItem> items;
+    for (size_t i = 0; i < dirty.size(); i++) {
+        const int32_t j = dirty[i];
+        FrpcaItem it{};
+        it.A = b.A.p + (size_t)b.start[j] * b.s_pad * es;
+        it.lda = b.s_pad;
+        it.m = b.nsrc;
+        it.nc = b.ncols[j];
+        it.Qt = w.Qt.p + (size_t)b.start[j] * l * es;
+        it.ldqt = std::max<int64_t>(b.width[j], 1);
+        it.Q = w.Q.p + i * (size_t)b.s_pad * l * es;
+        it.Q2 = w.Q2.p + i * (size_t)b.s_pad * l * es;
+        it.ldq = b.s_pad;
+        it.seed = seed * 0x9E3779B97F4A7C15ull + (uint64_t)j * 0xD1B54A32D192ED03ull + 1;
+        items.push_back(it);
+    }
+    frpca_core(items, b.precision, l, w.q, w.G.p, w.W.p, w.lam.p, gemm, eig, st);
+    const int cnt = (int)items.size();
+    // leaf = U diag(S) = Q V[:, top k] diag(sqrt(lam))   (Tree_u_NC.cpp:147-154)
+    eig_select(w.W.p, w.G.p, w.lam.p, l, k, cnt, 0, 0, 1e-4, st);
+    const DT T = b.precision == TSVD_FP64 ? DT::F64 : DT::F32;
+    std::vector<GemmProb> lp;
+    for (int i = 0; i < cnt; i++) {
+        GemmProb p{};
+        p.A = items[i].Q; p.lda = items[i].ldq; p.transA = 0;
+        p.B = w.W.p + (size_t)i * l * k; p.ldb = l;
+        p.C = (uint8_t *)leaves + (size_t)dirty[i] * b.s_pad * k * es; p.ldc = b.s_pad;
+        p.M = b.nsrc; p.N = k; p.K = l; p.alpha = 1.0; p.beta = 0.0;
+        lp.push_back(p);
+    }
+    gemm.run(st, lp, GemmTypes{T, DT::F64, T, DT::F64}, k > 80);
+    std::vector<double> lam((size_t)cnt * l);
+    TSVD_HIP(hipMemcpyAsync(lam.data(), w.lam.p, sizeof(double) * lam.size(), hipMemcpyDeviceToHost, st));
+    TSVD_HIP(hipStreamSynchronize(st));
+    sig2.assign(cnt, 0.0);
+    for (int i = 0; i < cnt; i++)
+        for (int c = l - k; c < l; c++) {
+            const double lv = std::max(lam[(size_t)i * l + c], 0.0);
+            if (std::sqrt(lv) > 1e-4) sig2[i] += lv;
+        }
+}
+
+}  // namespace tsvd
+
+using namespace tsvd;
+
+// ------------------------------------------------------------ drop-in frPCA
+// Host CSR in (1-based cols, pointerB/pointerE, ext.h:22-33), host U, S, V out
+// (col-major FP64, caller-allocated, S ascending as frpca.c:296-312 leaves it).
+namespace {
+
+__global__ void k_csr_to_dense(const double *__restrict__ vals, const int *__restrict__ cols,
+                               const int *__restrict__ pb, const int *__restrict__ pe, int nrows, double *__restrict__ D,
+                               int64_t ld)
+{
+    const int r = blockIdx.x;
+    if (r >= nrows) return;
+    const int b = pb[r] - 1, e = pe[r] - 1;
+    for (int t = b + threadIdx.x; t < e; t += blockDim.x) D[(int64_t)(cols[t] - 1) * ld + r] = vals[t];
+}
+
+struct DropIn {
+    GemmRunner gemm;
+    Eig eig;
+};
+
+int frpca_csr_impl(const mat_csr *A, mat *U, mat *S, mat *V, int k, int q, uint64_t seed)
+{
+    return guard([&] {
+        TSVD_REQUIRE(A && U && S && V, "null matrix");
+        const int m = A->nrows, n = A->ncols, l = k + 5;
+        TSVD_REQUIRE(k >= 1 && q >= 1, "k and q must be positive");
+        TSVD_REQUIRE(m >= 1 && n >= 1 && A->nnz >= 0, "bad matrix shape");
+        TSVD_REQUIRE(U->nrows == m && U->ncols == k && S->nrows == k && V->nrows == n && V->ncols == k,
+                     "output shapes must be U m x k, S k x 1, V n x k");
+        hipStream_t st;
+        TSVD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } sg{st};
+        static DropIn *ctx = nullptr;   // handles reused across calls (one per process)
+        if (!ctx) ctx = new DropIn();
+        const int64_t ld = round_up(m, 64);
+        DBuf<double> dA((size_t)ld * n), dQt((size_t)n * l), dQ((size_t)ld * l), dQ2((size_t)ld * l);
+        DBuf<double> G((size_t)l * l), W((size_t)l * l), lam((size_t)l), Wk((size_t)l * k);
+        DBuf<double> vals((size_t)std::max(A->nnz, 1));
+        DBuf<int> cols((size_t)std::max(A->nnz, 1)), pb((size_t)m), pe((size_t)m);
+        TSVD_HIP(hipMemsetAsync(dA.p, 0, dA.bytes(), st));
+        TSVD_HIP(hipMemsetAsync(dQ.p, 0, dQ.bytes(), st));
+        TSVD_HIP(hipMemsetAsync(dQ2.p, 0, dQ2.bytes(), st));
+        if (A->nnz > 0) {
+            TSVD_HIP(hipMemcpyAsync(vals.p, A->values, sizeof(double) * A->nnz, hipMemcpyHostToDevice, st));
+            TSVD_HIP(hipMemcpyAsync(cols.p, A->cols, sizeof(int) * A->nnz, hipMemcpyHostToDevice, st));
+        }
+        TSVD_HIP(hipMemcpyAsync(pb.p, A->pointerB, sizeof(int) * m, hipMemcpyHostToDevice, st));
+        TSVD_HIP(hipMemcpyAsync(pe.p, A->pointerE, sizeof(int) * m, hipMemcpyHostToDevice, st));
+        // validate the CSR on the host side (1-based, in range)
+        for (int r = 0; r < m; r++) {
+            TSVD_REQUIRE(A->pointerB[r] >= 1 && A->pointerE[r] >= A->pointerB[r] && A->pointerE[r] - 1 <= A->nnz,
+                         "bad row pointers at row %d", r);
+            for (int t = A->pointerB[r] - 1; t < A->pointerE[r] - 1; t++)
+                TSVD_REQUIRE(A->cols[t] >= 1 && A->cols[t] <= n, "column index out of range at row %d", r);
+        }
+        hipLaunchKernelGGL(k_csr_to_dense, dim3(m), dim3(256), 0, st, vals.p, cols.p, pb.p, pe.p, m, dA.p, ld);
+        TSVD_HIP(hipGetLastError());
+        std::vector<FrpcaItem> items(1);
+        items[0] = FrpcaItem{dA.p, ld, m, n, dQt.p, n, dQ.p, dQ2.p, ld, seed};
+        frpca_core(items, TSVD_FP64, l, q, G.p, W.p, lam.p, ctx->gemm, ctx->eig, st);
+        std::vector<double> hl((size_t)l);
+        TSVD_HIP(hipMemcpyAsync(hl.data(), lam.p, sizeof(double) * l, hipMemcpyDeviceToHost, st));
+        // U = Q V[:, s..s+k-1] (frpca.c:314-316), ascending
+        eig_select(Wk.p, G.p, lam.p, l, k, 1, 3, 1, 1e-4, st);
+        DBuf<double> dU((size_t)m * k), dV((size_t)n * k);
+        std::vector<GemmProb> p1(1);
+        p1[0] = GemmProb{};
+        p1[0].A = items[0].Q; p1[0].lda = ld; p1[0].B = Wk.p; p1[0].ldb = l; p1[0].C = dU.p; p1[0].ldc = m;
+        p1[0].M = m; p1[0].N = k; p1[0].K = l; p1[0].alpha = 1.0;
+        ctx->gemm.run(st, p1, GemmTypes{DT::F64, DT::F64, DT::F64, DT::F64}, k > 80);
+        // V = Qt V[:, sel] / S (eigSVD's U of Qt, frpca.c:118-131)
+        eig_select(Wk.p, G.p, lam.p, l, k, 1, 2, 1, 1e-4, st);
+        p1[0].A = dQt.p; p1[0].lda = n; p1[0].C = dV.p; p1[0].ldc = n; p1[0].M = n;
+        ctx->gemm.run(st, p1, GemmTypes{DT::F64, DT::F64, DT::F64, DT::F64}, k > 80);
+        TSVD_HIP(hipMemcpyAsync(U->d, dU.p, sizeof(double) * m * k, hipMemcpyDeviceToHost, st));
+        TSVD_HIP(hipMemcpyAsync(V->d, dV.p, sizeof(double) * (size_t)n * k, hipMemcpyDeviceToHost, st));
+        TSVD_HIP(hipStreamSynchronize(st));
+        for (int i = 0; i < k; i++) {
+            const double s = std::sqrt(std::max(hl[(size_t)(l - k + i)], 0.0));
+            S->d[i] = s > 1e-4 ? s : 0.0;
+        }
+    });
+}
+
+}  // namespace
+
+extern "C" int tsvd_frpca_csr(const mat_csr *A, mat *U, mat *S, mat *V, int k, int q, uint64_t seed)
+{
+    return frpca_csr_impl(A, U, S, V, k, q, seed);
+}
+
+extern "C" void frPCA(mat_csr *A, mat **U, mat **S, mat **V, int k, int q)
+{
+    static uint64_t calls = 0;
+    if (!A || !U || !S || !V || !*U || !*S || !*V) {
+        set_error(TSVD_EINVAL, "frPCA: null argument");
+        return;
+    }
+    frpca_csr_impl(A, *U, *S, *V, k, q, 0x5eed0000ull + (calls++));
+}

@@ -1,0 +1,214 @@
+// gemm.hip -- grouped dense products on CDNA4 matrix cores.
+//
+// Used by the blockwise frPCA (the products A*X and A^T*Y of every dirty
+// column block in one launch; the reference calls mkl_dcsrmm once per block,
+// matrix_vector_functions_intel_mkl_ext.c:364-398), by the FP64 Gram matrices
+// of the orthonormalisations and of the tree merges (dgemm / dgesvd work of
+// frpca.c:99-136 and Tree_u_NC.cpp:238-262), and by the basis rotations.
+//
+// A workgroup of 4 wavefronts owns a 64 x (16*NT) tile of C; wave w owns rows
+// [16w, 16w+16) and all NT 16-column MFMA tiles, so one A fragment feeds NT
+// matrix instructions (NT = 9 covers rank + oversampling = 133 in one tile).
+// K advances in 16-deep slabs staged through LDS (double-buffered through
+// registers: the next slab is fetched from HBM while the current one is fed
+// to the MFMAs).  MFMA 16x16x4 operand layout: lane l holds A(l%16, l/16) and
+// B(l/16, l%16); the FP32 accumulator holds C(4*(l/16)+r, l%16) and the FP64
+// accumulator C(l/16 + 4r, l%16), r = 0..3 (both checked on gfx950).
+#include "tsvd_gemm.h"
+
+namespace tsvd {
+
+namespace {
+
+constexpr int BM = 64;
+constexpr int BK = 16;
+constexpr int TPB = 256;
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+struct Acc;
+template <>
+struct Acc<float> {
+    typedef f32x4 V;
+    static __device__ inline V mfma(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <>
+struct Acc<double> {
+    typedef f64x4 V;
+    static __device__ inline V mfma(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+__device__ inline void atomic_add_out(float *p, double v) { atomicAdd(p, (float)v); }
+__device__ inline void atomic_add_out(double *p, double v) { atomicAdd(p, v); }
+
+template <class TA, class TB, class TC, class TACC, int NT>
+__global__ void __launch_bounds__(TPB) k_gemm(const GemmProb *__restrict__ probs, int nprob)
+{
+    constexpr int BN = 16 * NT;
+    constexpr int A_PER = BM * BK / TPB;             // 4
+    constexpr int B_PER = (BN * BK + TPB - 1) / TPB; // 9 or 5
+    __shared__ TACC As[BK][BM];
+    __shared__ TACC Bs[BK][BN];
+
+    // locate the problem (binary search over tile_begin)
+    const int64_t tile = blockIdx.x;
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (probs[mid].tile_begin <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    const GemmProb &P = probs[lo];
+    int64_t t = tile - P.tile_begin;
+    const int tm = (int)(t % P.tiles_m);
+    t /= P.tiles_m;
+    const int tn = (int)(t % P.tiles_n);
+    const int tk = (int)(t / P.tiles_n);
+    const int M = P.M, N = P.N;
+    const int i0 = tm * BM, j0 = tn * BN;
+    int kbeg = 0, kend = P.K;
+    if (P.tiles_k > 1) {
+        kbeg = tk * P.kchunk;
+        kend = min(P.K, kbeg + P.kchunk);
+    }
+    const TA *__restrict__ A = (const TA *)P.A;
+    const TB *__restrict__ B = (const TB *)P.B;
+    const int64_t lda = P.lda, ldb = P.ldb;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+
+    typename Acc<TACC>::V acc[NT];
+#pragma unroll
+    for (int n = 0; n < NT; n++) acc[n] = typename Acc<TACC>::V{0, 0, 0, 0};
+
+    TACC ra[A_PER], rb[B_PER];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < A_PER; q++) {
+            const int e = tid + q * TPB;
+            int ii, kk;
+            if (P.transA) { ii = e / BK; kk = e % BK; }
+            else { kk = e / BM; ii = e % BM; }
+            const int gi = i0 + ii, gk = k0 + kk;
+            TACC v = TACC(0);
+            if (gi < M && gk < kend) v = (TACC)(P.transA ? A[(int64_t)gi * lda + gk] : A[(int64_t)gk * lda + gi]);
+            ra[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < B_PER; q++) {
+            const int e = tid + q * TPB;
+            const int jj = e / BK, kk = e % BK;
+            const int gj = j0 + jj, gk = k0 + kk;
+            TACC v = TACC(0);
+            if (e < BN * BK && gj < N && gk < kend) v = (TACC)B[(int64_t)gj * ldb + gk];
+            rb[q] = v;
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < A_PER; q++) {
+            const int e = tid + q * TPB;
+            int ii, kk;
+            if (P.transA) { ii = e / BK; kk = e % BK; }
+            else { kk = e / BM; ii = e % BM; }
+            As[kk][ii] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < B_PER; q++) {
+            const int e = tid + q * TPB;
+            if (e < BN * BK) Bs[e % BK][e / BK] = rb[q];
+        }
+    };
+
+    if (kbeg < kend) fetch(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        __syncthreads();
+        stash();
+        __syncthreads();
+        if (k0 + BK < kend) fetch(k0 + BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            const TACC a = As[ks + (lane >> 4)][wave * 16 + (lane & 15)];
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                const TACC b = Bs[ks + (lane >> 4)][n * 16 + (lane & 15)];
+                acc[n] = Acc<TACC>::mfma(a, b, acc[n]);
+            }
+        }
+    }
+
+    TC *__restrict__ C = (TC *)P.C;
+    const int64_t ldc = P.ldc;
+    const double alpha = P.alpha, beta = P.beta;
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+        const int gj = j0 + n * 16 + (lane & 15);
+        if (gj >= N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int gi = i0 + wave * 16 + Acc<TACC>::row(lane, r);
+            if (gi >= M) continue;
+            TC *c = C + (int64_t)gj * ldc + gi;
+            const double v = alpha * (double)acc[n][r];
+            if (P.tiles_k > 1) atomic_add_out(c, v);
+            else *c = (TC)(beta == 0.0 ? v : v + beta * (double)*c);
+        }
+    }
+}
+
+template <class TA, class TB, class TC, class TACC>
+void launch_t(hipStream_t st, const GemmProb *d, int nprob, int64_t tiles, bool wide)
+{
+    if (wide) hipLaunchKernelGGL((k_gemm<TA, TB, TC, TACC, 9>), dim3((unsigned)tiles), dim3(TPB), 0, st, d, nprob);
+    else hipLaunchKernelGGL((k_gemm<TA, TB, TC, TACC, 5>), dim3((unsigned)tiles), dim3(TPB), 0, st, d, nprob);
+}
+
+}  // namespace
+
+void GemmRunner::run(hipStream_t st, std::vector<GemmProb> &probs, GemmTypes t, bool wide)
+{
+    const int BN = wide ? 144 : 80;
+    int64_t tiles = 0;
+    std::vector<GemmProb> live;
+    live.reserve(probs.size());
+    for (auto &p : probs) {
+        if (p.M <= 0 || p.N <= 0) continue;
+        TSVD_REQUIRE(p.K >= 0, "negative K");
+        p.tiles_m = (int32_t)ceil_div(p.M, BM);
+        p.tiles_n = (int32_t)ceil_div(p.N, BN);
+        p.tiles_k = 1;
+        if (p.kchunk > 0 && p.K > p.kchunk) {
+            TSVD_REQUIRE(p.kchunk % BK == 0, "kchunk must be a multiple of 16");
+            p.tiles_k = (int32_t)ceil_div(p.K, p.kchunk);
+        }
+        p.tile_begin = tiles;
+        tiles += (int64_t)p.tiles_m * p.tiles_n * p.tiles_k;
+        live.push_back(p);
+    }
+    if (live.empty() || tiles == 0) return;
+    TSVD_REQUIRE(tiles < (int64_t)INT32_MAX, "too many GEMM tiles");
+    dprobs_.ensure(live.size());
+    TSVD_HIP(hipMemcpyAsync(dprobs_.p, live.data(), sizeof(GemmProb) * live.size(), hipMemcpyHostToDevice, st));
+    const int np = (int)live.size();
+    // the staging vector must outlive the copy
+    TSVD_HIP(hipStreamSynchronize(st));
+    if (t.acc == DT::F32) {
+        TSVD_REQUIRE(t.a == DT::F32 && t.b == DT::F32 && t.c == DT::F32, "FP32 accumulate takes FP32 operands");
+        launch_t<float, float, float, float>(st, dprobs_.p, np, tiles, wide);
+    } else {
+        if (t.a == DT::F32 && t.b == DT::F32 && t.c == DT::F64) launch_t<float, float, double, double>(st, dprobs_.p, np, tiles, wide);
+        else if (t.a == DT::F32 && t.b == DT::F64 && t.c == DT::F32) launch_t<float, double, float, double>(st, dprobs_.p, np, tiles, wide);
+        else if (t.a == DT::F64 && t.b == DT::F64 && t.c == DT::F64) launch_t<double, double, double, double>(st, dprobs_.p, np, tiles, wide);
+        else if (t.a == DT::F32 && t.b == DT::F64 && t.c == DT::F64) launch_t<float, double, double, double>(st, dprobs_.p, np, tiles, wide);
+        else if (t.a == DT::F64 && t.b == DT::F64 && t.c == DT::F32) launch_t<double, double, float, double>(st, dprobs_.p, np, tiles, wide);
+        else TSVD_REQUIRE(false, "unsupported GEMM type combination");
+    }
+    TSVD_HIP(hipGetLastError());
+    launches++;
+}
+
+}  // namespace tsvd

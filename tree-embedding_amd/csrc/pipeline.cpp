@@ -1,0 +1,246 @@
+// pipeline.cpp -- the per-snapshot driver ("plain GPU method").
+//
+// Replaces the snapshot loop of Tree_u_NC.cpp:507-866 (and Tree_d_NC.cpp:
+// 611-1132, Tree_u_LP.cpp:664-925): ingest -> refresh (static before
+// dyn_start, dynamic after) -> push -> block transform with the lazy test
+// from lazy_start on (update_mat_record, Tree_u_NC.cpp:608-670) -> frPCA of
+// each dirty block with its residual norm (Tree_u_NC.cpp:75-226, 679-710) ->
+// merges along the dirty paths -> root embedding; a snapshot with no dirty
+// block keeps the previous embedding (Tree_u_NC.cpp:713-721).
+//
+// The reference computes the leaf residual ||U S V^T - A||_F with a dense
+// |S| x w_j product (Tree_u_NC.cpp:158-204).  frPCA's U S V^T is the
+// orthogonal projection of A onto span(Q V_k), so the same number is
+// sqrt(||A||_F^2 - sum_k sigma^2); both terms are already at hand.
+#include "tsvd_pipeline.h"
+
+#include <chrono>
+#include <cmath>
+
+namespace tsvd {
+
+static double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static void validate(const tsvd_config &c)
+{
+    TSVD_REQUIRE(c.n > 0 && c.nsrc > 0 && c.sources, "n, nsrc and sources are required");
+    TSVD_REQUIRE(c.alpha > 0 && c.alpha < 1 && c.rmax > 0, "alpha in (0,1) and rmax > 0 required");
+    TSVD_REQUIRE(c.precision == TSVD_FP32 || c.precision == TSVD_FP64, "bad precision");
+    TSVD_REQUIRE(c.nparts >= 1 && c.nparts <= c.n, "nparts must be in [1, n]");
+    TSVD_REQUIRE(c.fanout >= 2, "fan-out must be >= 2");
+    TSVD_REQUIRE(c.rank >= 1 && c.rank + 5 <= 512, "rank must be in [1, 507]");
+    TSVD_REQUIRE(c.q >= 1, "q must be >= 1");
+    TSVD_REQUIRE(c.transform == TSVD_TRANSFORM_NC || c.transform == TSVD_TRANSFORM_LP, "bad transform");
+    TSVD_REQUIRE(c.fanout * c.rank <= 4096, "fan-out * rank too large for the merge eigensolve");
+}
+
+}  // namespace tsvd
+
+using namespace tsvd;
+
+tsvd_pipeline::~tsvd_pipeline()
+{
+    delete eig;
+    if (ppr) tsvd_ppr_destroy(ppr);
+    if (g) tsvd_graph_destroy(g);
+}
+
+extern "C" int tsvd_pipeline_create(const tsvd_config *cfg, tsvd_pipeline **out)
+{
+    return guard([&] {
+        TSVD_REQUIRE(cfg && out, "null argument");
+        validate(*cfg);
+        tsvd_pipeline *p = new tsvd_pipeline();
+        try {
+            p->cfg = *cfg;
+            p->sources.assign(cfg->sources, cfg->sources + cfg->nsrc);
+            p->cfg.sources = p->sources.data();
+            int rc = tsvd_graph_create(cfg->n, cfg->directed, cfg->slack > 0 ? cfg->slack : 0.1, &p->g);
+            if (rc) throw Error{rc};
+            rc = tsvd_ppr_create(p->g, cfg->sources, cfg->nsrc, cfg->alpha, cfg->rmax, cfg->precision, &p->ppr);
+            if (rc) throw Error{rc};
+            hipStream_t st = p->g->stream;
+            blocks_setup(p->blocks, cfg->n, cfg->nparts, cfg->nsrc, p->ppr->s_pad, cfg->precision, cfg->transform,
+                         cfg->rmax, p->sources, st);
+            frpca_setup(p->fw, p->blocks, cfg->rank, cfg->q);
+            tree_setup(p->tree, cfg->nparts, cfg->fanout, cfg->rank, cfg->nsrc, p->ppr->s_pad, p->blocks.esize, st);
+            p->eig = new Eig();
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+extern "C" int tsvd_pipeline_destroy(tsvd_pipeline *p)
+{
+    return guard([&] { delete p; });
+}
+
+extern "C" int tsvd_pipeline_snapshot(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, int64_t m,
+                                      int on_device, tsvd_snapshot_stats *stats)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p, "null pipeline");
+        const tsvd_config &c = p->cfg;
+        hipStream_t st = p->g->stream;
+        tsvd_snapshot_stats S{};
+        const double t0 = now_ms();
+        graph_append(p->g, src, dst, m, on_device);
+        const int32_t it = ++p->snapshot;
+        S.snapshot = it;
+        const double t1 = now_ms();
+        S.ms_ingest = t1 - t0;
+        // Tree_u_NC.cpp:516-518: snapshots before lazy_start-1 are only ingested
+        if (c.lazy_start != 100 && it < c.lazy_start - 1) {
+            S.reused = 1;
+            S.ms_total = now_ms() - t0;
+            if (stats) *stats = S;
+            return;
+        }
+        const int mode = it < c.dyn_start ? TSVD_REFRESH_INIT : TSVD_REFRESH_DYNAMIC;
+        for (int d = 0; d < p->ppr->ndir; d++) ppr_refresh(p->ppr, mode, d, 0, c.nsrc);
+        const double t2 = now_ms();
+        S.ms_refresh = t2 - t1;
+        for (int d = 0; d < p->ppr->ndir; d++) {
+            ppr_push(p->ppr, d);
+            S.push_rounds += p->ppr->last_rounds[d];
+            S.push_pops += p->ppr->last_pops[d];
+            S.push_relax += p->ppr->last_relax[d];
+        }
+        const double t3 = now_ms();
+        S.ms_push = t3 - t2;
+
+        Blocks &b = p->blocks;
+        const void *p0 = p->ppr->p(0);
+        const void *p1 = p->ppr->ndir > 1 ? p->ppr->p(1) : nullptr;
+        blocks_norms(b, p0, p1, p->g, st);
+        std::vector<int32_t> rebuild, factor;
+        const double thr = c.delta * std::sqrt(2.0);
+        for (int32_t j = 0; j < b.nparts; j++) {
+            bool dirty = true;
+            if (it >= c.lazy_start) {
+                // Eqn 2 (ppr:2699-2726): clean iff resid + ||A_new - A_old|| < delta sqrt2 ||A_new||
+                dirty = !(b.resid[j] + std::sqrt(b.e2[j]) < thr * std::sqrt(b.a2[j]));
+            }
+            if (!dirty) continue;
+            rebuild.push_back(j);
+            if (b.a2[j] > 0.0) factor.push_back(j);   // an all-zero block is skipped (Tree_u_NC.cpp:683-686)
+        }
+        blocks_rebuild(b, rebuild, p0, p1, st);
+        const double t4 = now_ms();
+        S.ms_blocks = t4 - t3;
+
+        p->last_dirty = factor;
+        S.ndirty = (int32_t)factor.size();
+        if (!factor.empty()) {
+            std::vector<double> sig2;
+            frpca_blocks(p->fw, b, factor, p->tree.level[0].p, p->gemm, *p->eig,
+                         c.seed + 0x1000003ull * (uint64_t)it, st, sig2);
+            if (it >= c.lazy_start - 1)
+                for (size_t i = 0; i < factor.size(); i++) {
+                    const int32_t j = factor[i];
+                    b.resid[j] = std::sqrt(std::max(0.0, b.a2[j] - sig2[i]));
+                }
+        }
+        const double t5 = now_ms();
+        S.ms_frpca = t5 - t4;
+        if (factor.empty() && p->tree.have_root) {
+            S.reused = 1;
+        } else if (!factor.empty()) {
+            S.nmerged = tree_update(p->tree, factor, c.transform, p->gemm, *p->eig, st);
+        }
+        const double t6 = now_ms();
+        S.ms_merge = t6 - t5;
+        S.ms_total = t6 - t0;
+        if (stats) *stats = S;
+    });
+}
+
+extern "C" int tsvd_pipeline_embedding(tsvd_pipeline *p, double *E)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p && E, "null argument");
+        const int32_t k = p->cfg.rank, ns = p->cfg.nsrc, sp = p->tree.s_pad;
+        std::vector<double> h((size_t)sp * k);
+        TSVD_HIP(hipMemcpy(h.data(), p->tree.E.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+        for (int32_t i = 0; i < ns; i++)
+            for (int32_t c = 0; c < k; c++) E[(size_t)i * k + c] = h[(size_t)c * sp + i];
+    });
+}
+
+extern "C" int tsvd_pipeline_root_sigma(tsvd_pipeline *p, double *sigma)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p && sigma, "null argument");
+        for (int32_t c = 0; c < p->cfg.rank; c++) sigma[c] = p->tree.root_sigma[c];
+    });
+}
+
+extern "C" tsvd_graph *tsvd_pipeline_graph(tsvd_pipeline *p) { return p ? p->g : nullptr; }
+extern "C" tsvd_ppr *tsvd_pipeline_ppr(tsvd_pipeline *p) { return p ? p->ppr : nullptr; }
+
+extern "C" int tsvd_pipeline_block_matrix(tsvd_pipeline *p, int32_t j, double *A)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p && A, "null argument");
+        const Blocks &b = p->blocks;
+        TSVD_REQUIRE(j >= 0 && j < b.nparts, "block out of range");
+        const int64_t w = b.width[j], nc = b.ncols[j];
+        std::vector<uint8_t> h((size_t)std::max<int64_t>(nc, 1) * b.s_pad * b.esize);
+        std::vector<int32_t> cm((size_t)std::max<int64_t>(nc, 1));
+        if (nc > 0) {
+            TSVD_HIP(hipMemcpy(h.data(), b.A.p + (size_t)b.start[j] * b.s_pad * b.esize, (size_t)nc * b.s_pad * b.esize,
+                               hipMemcpyDeviceToHost));
+            TSVD_HIP(hipMemcpy(cm.data(), b.colmap.p + b.start[j], sizeof(int32_t) * nc, hipMemcpyDeviceToHost));
+        }
+        for (size_t t = 0; t < (size_t)w * b.nsrc; t++) A[t] = 0.0;
+        for (int64_t c = 0; c < nc; c++) {
+            const int64_t col = cm[c] - b.start[j];
+            for (int32_t s = 0; s < b.nsrc; s++) {
+                const size_t e = (size_t)c * b.s_pad + s;
+                A[(size_t)col * b.nsrc + s] = b.esize == 8 ? ((double *)h.data())[e] : (double)((float *)h.data())[e];
+            }
+        }
+    });
+}
+
+extern "C" int tsvd_pipeline_block_norms(tsvd_pipeline *p, double *a_norm, double *e_norm, double *resid,
+                                         int32_t *dirty)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p, "null pipeline");
+        const Blocks &b = p->blocks;
+        for (int32_t j = 0; j < b.nparts; j++) {
+            if (a_norm) a_norm[j] = std::sqrt(b.a2[j]);
+            if (e_norm) e_norm[j] = std::sqrt(b.e2[j]);
+            if (resid) resid[j] = b.resid[j];
+            if (dirty) dirty[j] = 0;
+        }
+        if (dirty)
+            for (int32_t j : p->last_dirty) dirty[j] = 1;
+    });
+}
+
+extern "C" int tsvd_pipeline_node(tsvd_pipeline *p, int32_t level, int32_t index, double *out)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p && out, "null argument");
+        const Tree &t = p->tree;
+        TSVD_REQUIRE(level >= 0 && level < (int32_t)t.sizes.size() - 1, "level out of range");
+        TSVD_REQUIRE(index >= 0 && index < t.sizes[level], "node out of range");
+        const size_t cnt = (size_t)t.s_pad * t.k;
+        std::vector<uint8_t> h(cnt * t.esize);
+        TSVD_HIP(hipMemcpy(h.data(), t.level[level].p + (size_t)index * cnt * t.esize, h.size(),
+                           hipMemcpyDeviceToHost));
+        for (int32_t c = 0; c < t.k; c++)
+            for (int32_t s = 0; s < t.nsrc; s++) {
+                const size_t e = (size_t)c * t.s_pad + s;
+                out[(size_t)c * t.nsrc + s] = t.esize == 8 ? ((double *)h.data())[e] : (double)((float *)h.data())[e];
+            }
+    });
+}

@@ -1,0 +1,439 @@
+// ppr.hip -- subset forward push and dynamic insertion update on the device.
+//
+// Replaces (ppr_computation_store_dynamic.h):
+//   nodegree_DenseUndirected_Refresh_PPR_initialization   :2035-2121
+//   nodegree_DenseDirected_Refresh_PPR_initialization     :3811-3898
+//   nodegree_DenseDirected_Refresh_PPR_initialization_Transpose :4133-4228
+//   nodegree_DenseDynamicForwardPush                       :1394-1498
+//   nodegree_DenseDirectedDynamicForwardPush[Transpose]    :2945-3030, :3356-3437
+//
+// The reference runs one FIFO per source on one CPU thread per source range.
+// Here all sources advance together in synchronous rounds.  A work item is a
+// (vertex v, source group g) pair with a 64-bit mask of the sources of that
+// group whose residue at v exceeds rmax; one wavefront pops the item (lane =
+// source) and walks v's neighbour list once for all 64 sources, so every
+// residue update is a coalesced 64-lane atomic on r[u][g*64 .. g*64+63].
+// A residue that crosses rmax upward sets its bit in the next round's mask;
+// the first setter of a (u, g) mask appends the item, so each item appears
+// once per round.  Only same-address atomics are relied on: a pop is an
+// atomic exchange, a relaxation an atomic add, so their order on one address
+// decides everything and no cross-address ordering is needed.  A popped value
+// at or below rmax (the dynamic update can lower residues) is added back.
+//
+// Semantics kept from the reference: threshold r > rmax (not degree-scaled,
+// positive side only); p += alpha*r, neighbours += (1-alpha)*r/deg; an
+// undirected degree-0 vertex (only a source can be one) gets p = alpha*r and
+// keeps r; a directed dangling vertex (degree 0 in the push direction) is never
+// queued and keeps its residue; the insertion update for new slot j of row i
+// (j == 0 skipped): p *= (j+1)/j, r[i] -= p/(j+1)/alpha,
+// r[to] += (1-alpha) p/(j+1)/alpha, with the exact ratio (the reference's
+// integer (j+1)/j is not reproduced; its n >= 1e7 path, ppr:6518-6522, uses the
+// exact ratio too).  Order of pushes differs from the reference's FIFO, so
+// reserves agree within the push error bound, not bitwise.
+#include "tsvd_ppr.h"
+
+namespace tsvd {
+
+namespace {
+
+constexpr int TPB = 256;
+constexpr int WPB = TPB / 64;   // waves per block
+
+__device__ inline float xchg(float *a, float v) { return atomicExch(a, v); }
+__device__ inline double xchg(double *a, double v)
+{
+    unsigned long long o = atomicExch((unsigned long long *)a, (unsigned long long)__double_as_longlong(v));
+    return __longlong_as_double((long long)o);
+}
+
+__device__ inline void append_item(uint32_t item, uint32_t *list, uint32_t *count, uint32_t cap, uint32_t *overflow)
+{
+    uint32_t pos = atomicAdd(count, 1u);
+    if (pos < cap) list[pos] = item;
+    else atomicOr(overflow, 1u);
+}
+
+// Lane-0 sets `bits` in mask[item]; the first setter appends the item.
+__device__ inline void mark(unsigned long long bits, uint32_t item, unsigned long long *mask, uint32_t *list,
+                            uint32_t *count, uint32_t cap, uint32_t *overflow)
+{
+    unsigned long long old = atomicOr(mask + item, bits);
+    if (old == 0ull) append_item(item, list, count, cap, overflow);
+}
+
+template <class T>
+__global__ void k_zero_range(T *__restrict__ P, T *__restrict__ R, int32_t n, int32_t s_pad, int32_t s0, int32_t s1)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t w = s1 - s0;
+    if (t >= (int64_t)n * w) return;
+    int64_t v = t / w;
+    int32_t s = s0 + (int32_t)(t - v * w);
+    P[v * s_pad + s] = T(0);
+    R[v * s_pad + s] = T(0);
+}
+
+// Static branch (ppr:2056-2068): r[src_s] = 1 and queue (src_s, g(s)).
+template <class T>
+__global__ void k_seed(T *__restrict__ R, const int32_t *__restrict__ src, int32_t s_pad, int32_t ngroups, int32_t s0,
+                       int32_t s1, unsigned long long *__restrict__ cm, uint32_t *__restrict__ list,
+                       uint32_t *__restrict__ count, uint32_t cap, uint32_t *__restrict__ overflow)
+{
+    int32_t s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= s1) return;
+    int32_t v = src[s];
+    R[(int64_t)v * s_pad + s] = T(1);
+    uint32_t item = (uint32_t)v * (uint32_t)ngroups + (uint32_t)(s >> 6);
+    mark(1ull << (s & 63), item, cm, list, count, cap, overflow);
+}
+
+// Dynamic branch: one wave per (grown row i, source group g), lanes = sources.
+template <class T>
+__global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict__ R, const int32_t *__restrict__ touched,
+                                                int32_t ntouched, const int64_t *__restrict__ rowoff,
+                                                const int32_t *__restrict__ adj, const int32_t *__restrict__ deg,
+                                                const int32_t *__restrict__ former, int32_t s_pad, int32_t ngroups,
+                                                int32_t s0, int32_t s1, double alpha, T rm,
+                                                unsigned long long *__restrict__ cm, uint32_t *__restrict__ list,
+                                                uint32_t *__restrict__ count, uint32_t cap,
+                                                uint32_t *__restrict__ overflow)
+{
+    const int64_t wave = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wave >= (int64_t)ntouched * ngroups) return;
+    const int32_t t = (int32_t)(wave / ngroups);
+    const int32_t g = (int32_t)(wave - (int64_t)t * ngroups);
+    const int32_t i = touched[t];
+    const int32_t s = g * 64 + lane;
+    const bool active = s >= s0 && s < s1;
+    const int32_t f0 = former[i], d1 = deg[i];
+    const int64_t base = rowoff[i];
+    T *Pi = P + (int64_t)i * s_pad + s;
+    T *Ri = R + (int64_t)i * s_pad + s;
+    T pv = active ? *Pi : T(0);
+    for (int32_t j = f0; j < d1; j++) {
+        if (j == 0) continue;                                   // ppr:2078-2079
+        const int32_t to = adj[base + j];
+        bool cross = false;
+        if (active) {
+            pv = (T)((double)pv * ((double)(j + 1) / (double)j));
+            const double pj = (double)pv / (double)(j + 1) / alpha;
+            atomicAdd(Ri, (T)(-pj));
+            const T add = (T)((1.0 - alpha) * pj);
+            T old = atomicAdd(R + (int64_t)to * s_pad + s, add);
+            cross = old <= rm && old + add > rm;
+        }
+        unsigned long long c = __ballot(cross);
+        if (c && lane == 0 && deg[to] > 0)
+            mark(c, (uint32_t)to * ngroups + g, cm, list, count, cap, overflow);
+    }
+    bool self = false;
+    if (active) {
+        *Pi = pv;
+        T x = atomicAdd(Ri, T(0));
+        self = x > rm && d1 > 0;
+    }
+    unsigned long long c2 = __ballot(self);
+    if (c2 && lane == 0) mark(c2, (uint32_t)i * ngroups + g, cm, list, count, cap, overflow);
+}
+
+// One synchronous push round: one wave per pending item.
+template <class T>
+__global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__restrict__ R,
+                                                    const uint32_t *__restrict__ items, uint32_t nitems,
+                                                    unsigned long long *__restrict__ cm,
+                                                    unsigned long long *__restrict__ nm, uint32_t *__restrict__ next,
+                                                    uint32_t *__restrict__ next_count, uint32_t cap,
+                                                    uint32_t *__restrict__ overflow,
+                                                    const int64_t *__restrict__ rowoff,
+                                                    const int32_t *__restrict__ adj, const int32_t *__restrict__ deg,
+                                                    int32_t s_pad, int32_t ngroups, double alpha, T rm, int directed,
+                                                    unsigned long long *__restrict__ stats)
+{
+    const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= nitems) return;
+    const uint32_t item = items[w];
+    const int32_t v = (int32_t)(item / (uint32_t)ngroups);
+    const int32_t g = (int32_t)(item - (uint32_t)v * (uint32_t)ngroups);
+    const unsigned long long m = cm[item];
+    if (lane == 0) cm[item] = 0ull;
+    const int32_t s = g * 64 + lane;
+    const bool active = (m >> lane) & 1ull;
+    const int32_t dv = deg[v];
+    T *Rv = R + (int64_t)v * s_pad + s;
+    T *Pv = P + (int64_t)v * s_pad + s;
+    T x = T(0);
+    bool pushing = false, again = false;
+    if (active) {
+        x = xchg(Rv, T(0));
+        if (!(x > rm)) {
+            T old = atomicAdd(Rv, x);                     // not above rmax: give it back
+            again = old <= rm && old + x > rm && dv > 0;
+        } else if (dv == 0) {
+            if (!directed) *Pv = (T)(alpha * (double)x);  // ppr:1431-1441
+            atomicAdd(Rv, x);                             // residue kept, not re-queued
+        } else {
+            pushing = true;
+            *Pv = (T)((double)*Pv + alpha * (double)x);
+        }
+    }
+    const unsigned long long ca = __ballot(again);
+    if (ca && lane == 0) mark(ca, item, nm, next, next_count, cap, overflow);
+    const unsigned long long pm = __ballot(pushing);
+    if (pm == 0ull) return;
+    if (lane == 0) {
+        atomicAdd(stats + 0, (unsigned long long)__popcll(pm));
+        atomicAdd(stats + 1, (unsigned long long)__popcll(pm) * (unsigned long long)dv);
+    }
+    const T delta = pushing ? (T)((1.0 - alpha) * (double)x / (double)dv) : T(0);
+    const int32_t *nb = adj + rowoff[v];
+    T *Rs = R + s;
+    int32_t j = 0;
+    for (; j + 4 <= dv; j += 4) {
+        const int32_t u0 = nb[j], u1 = nb[j + 1], u2 = nb[j + 2], u3 = nb[j + 3];
+        bool c0 = false, c1 = false, c2 = false, c3 = false;
+        if (pushing) {
+            T o0 = atomicAdd(Rs + (int64_t)u0 * s_pad, delta);
+            T o1 = atomicAdd(Rs + (int64_t)u1 * s_pad, delta);
+            T o2 = atomicAdd(Rs + (int64_t)u2 * s_pad, delta);
+            T o3 = atomicAdd(Rs + (int64_t)u3 * s_pad, delta);
+            c0 = o0 <= rm && o0 + delta > rm;
+            c1 = o1 <= rm && o1 + delta > rm;
+            c2 = o2 <= rm && o2 + delta > rm;
+            c3 = o3 <= rm && o3 + delta > rm;
+        }
+        const unsigned long long b0 = __ballot(c0), b1 = __ballot(c1), b2 = __ballot(c2), b3 = __ballot(c3);
+        if (lane == 0) {
+            if (b0 && deg[u0] > 0) mark(b0, (uint32_t)u0 * ngroups + g, nm, next, next_count, cap, overflow);
+            if (b1 && deg[u1] > 0) mark(b1, (uint32_t)u1 * ngroups + g, nm, next, next_count, cap, overflow);
+            if (b2 && deg[u2] > 0) mark(b2, (uint32_t)u2 * ngroups + g, nm, next, next_count, cap, overflow);
+            if (b3 && deg[u3] > 0) mark(b3, (uint32_t)u3 * ngroups + g, nm, next, next_count, cap, overflow);
+        }
+    }
+    for (; j < dv; j++) {
+        const int32_t u = nb[j];
+        bool c = false;
+        if (pushing) {
+            T o = atomicAdd(Rs + (int64_t)u * s_pad, delta);
+            c = o <= rm && o + delta > rm;
+        }
+        const unsigned long long b = __ballot(c);
+        if (b && lane == 0 && deg[u] > 0) mark(b, (uint32_t)u * ngroups + g, nm, next, next_count, cap, overflow);
+    }
+}
+
+template <class T>
+void refresh_t(tsvd_ppr *s, int mode, int d, int32_t s0, int32_t s1)
+{
+    tsvd_graph *g = s->g;
+    hipStream_t st = g->stream;
+    T *P = (T *)s->p(d), *R = (T *)s->r(d);
+    const uint32_t cap = (uint32_t)s->F[d].n;
+    uint32_t *cnt = s->counters.p;
+    if (mode == TSVD_REFRESH_INIT) {
+        if (s0 == 0 && s1 == s->nsrc) {
+            TSVD_HIP(hipMemsetAsync(P, 0, s->P[d].bytes(), st));
+            TSVD_HIP(hipMemsetAsync(R, 0, s->R[d].bytes(), st));
+            TSVD_HIP(hipMemsetAsync(s->cm[d].p, 0, s->cm[d].bytes(), st));
+            TSVD_HIP(hipMemsetAsync(cnt + d, 0, sizeof(uint32_t), st));
+        } else {
+            const int64_t tot = (int64_t)s->n * (s1 - s0);
+            hipLaunchKernelGGL(k_zero_range<T>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st, P, R, s->n,
+                               s->s_pad, s0, s1);
+        }
+        hipLaunchKernelGGL(k_seed<T>, dim3((unsigned)ceil_div(s1 - s0, TPB)), dim3(TPB), 0, st, R, s->sources.p,
+                           s->s_pad, s->ngroups, s0, s1, s->cm[d].p, s->F[d].p, cnt + d, cap, cnt + 3);
+    } else {
+        const GraphDir &D = g->dir[d];
+        if (D.ntouched > 0) {
+            const int64_t waves = (int64_t)D.ntouched * s->ngroups;
+            hipLaunchKernelGGL(k_update<T>, dim3((unsigned)ceil_div(waves, WPB)), dim3(TPB), 0, st, P, R,
+                               D.touched.p, D.ntouched, D.rowoff.p, D.adj.p, D.deg.p, D.former.p, s->s_pad,
+                               s->ngroups, s0, s1, s->alpha, (T)s->rmax, s->cm[d].p, s->F[d].p, cnt + d, cap,
+                               cnt + 3);
+        }
+    }
+    TSVD_HIP(hipGetLastError());
+    TSVD_HIP(hipStreamSynchronize(st));
+}
+
+template <class T>
+void push_t(tsvd_ppr *s, int d)
+{
+    tsvd_graph *g = s->g;
+    hipStream_t st = g->stream;
+    const GraphDir &D = g->dir[d];
+    T *P = (T *)s->p(d), *R = (T *)s->r(d);
+    uint32_t *cnt = s->counters.p;
+    uint32_t *h = s->h_counters;
+    const uint32_t cap = (uint32_t)s->fnext.n;
+    TSVD_HIP(hipMemsetAsync(s->stats.p, 0, s->stats.bytes(), st));
+    TSVD_HIP(hipMemcpyAsync(h, cnt, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    TSVD_HIP(hipStreamSynchronize(st));
+    TSVD_REQUIRE(h[3] == 0, "frontier overflow");
+    uint32_t count = h[d];
+    Timer tm;
+    tm.start(st);
+    int64_t rounds = 0;
+    while (count > 0) {
+        TSVD_HIP(hipMemsetAsync(cnt + 2, 0, sizeof(uint32_t), st));
+        hipLaunchKernelGGL(k_push_round<T>, dim3((unsigned)ceil_div(count, WPB)), dim3(TPB), 0, st, P, R, s->F[d].p,
+                           count, s->cm[d].p, s->nm.p, s->fnext.p, cnt + 2, cap, cnt + 3, D.rowoff.p, D.adj.p,
+                           D.deg.p, s->s_pad, s->ngroups, s->alpha, (T)s->rmax, s->directed, s->stats.p);
+        TSVD_HIP(hipGetLastError());
+        TSVD_HIP(hipMemcpyAsync(h + 2, cnt + 2, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        TSVD_HIP(hipStreamSynchronize(st));
+        TSVD_REQUIRE(h[3] == 0, "frontier overflow in push round %lld", (long long)rounds);
+        std::swap(s->cm[d], s->nm);
+        std::swap(s->F[d], s->fnext);
+        count = h[2];
+        rounds++;
+    }
+    tm.stop(st);
+    TSVD_HIP(hipMemsetAsync(cnt + d, 0, sizeof(uint32_t), st));
+    unsigned long long hs[2] = {0, 0};
+    TSVD_HIP(hipMemcpyAsync(hs, s->stats.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+    TSVD_HIP(hipStreamSynchronize(st));
+    s->last_rounds[d] = rounds;
+    s->last_pops[d] = (int64_t)hs[0];
+    s->last_relax[d] = (int64_t)hs[1];
+    s->last_ms[d] = tm.ms();
+}
+
+}  // namespace
+
+void ppr_refresh(tsvd_ppr *s, int mode, int dir, int32_t s0, int32_t s1)
+{
+    TSVD_REQUIRE(mode == TSVD_REFRESH_INIT || mode == TSVD_REFRESH_DYNAMIC, "bad refresh mode");
+    TSVD_REQUIRE(dir >= 0 && dir < s->ndir, "bad direction");
+    TSVD_REQUIRE(s0 >= 0 && s0 <= s1 && s1 <= s->nsrc, "bad source range");
+    if (s0 == s1) return;
+    if (s->precision == TSVD_FP64) refresh_t<double>(s, mode, dir, s0, s1);
+    else refresh_t<float>(s, mode, dir, s0, s1);
+}
+
+void ppr_push(tsvd_ppr *s, int dir)
+{
+    TSVD_REQUIRE(dir >= 0 && dir < s->ndir, "bad direction");
+    if (s->precision == TSVD_FP64) push_t<double>(s, dir);
+    else push_t<float>(s, dir);
+}
+
+}  // namespace tsvd
+
+using namespace tsvd;
+
+extern "C" int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t nsrc, double alpha, double rmax,
+                               int precision, tsvd_ppr **out)
+{
+    return guard([&] {
+        TSVD_REQUIRE(g && out && sources, "null argument");
+        TSVD_REQUIRE(nsrc > 0, "empty source set");
+        TSVD_REQUIRE(alpha > 0.0 && alpha < 1.0, "alpha must be in (0,1)");
+        TSVD_REQUIRE(rmax > 0.0, "rmax must be positive");
+        TSVD_REQUIRE(precision == TSVD_FP32 || precision == TSVD_FP64, "bad precision");
+        for (int32_t i = 0; i < nsrc; i++)
+            TSVD_REQUIRE(sources[i] >= 0 && sources[i] < g->n, "source %d out of range", sources[i]);
+        {
+            std::vector<int32_t> sorted(sources, sources + nsrc);
+            std::sort(sorted.begin(), sorted.end());
+            for (int32_t i = 1; i < nsrc; i++) TSVD_REQUIRE(sorted[i] != sorted[i - 1], "duplicate source %d", sorted[i]);
+        }
+        tsvd_ppr *s = new tsvd_ppr();
+        try {
+            s->g = g;
+            s->n = g->n;
+            s->nsrc = nsrc;
+            s->s_pad = (int32_t)round_up(nsrc, 64);
+            s->ngroups = s->s_pad / 64;
+            TSVD_REQUIRE((uint64_t)g->n * (uint64_t)s->ngroups < 0xFFFFFFFFull, "n * groups exceeds 32-bit items");
+            s->ndir = g->ndir;
+            s->directed = g->directed;
+            s->alpha = alpha;
+            s->rmax = rmax;
+            s->precision = precision;
+            s->esize = precision == TSVD_FP64 ? 8 : 4;
+            s->h_sources.assign(sources, sources + nsrc);
+            s->sources.alloc((size_t)nsrc);
+            hipStream_t st = g->stream;
+            TSVD_HIP(hipMemcpyAsync(s->sources.p, sources, sizeof(int32_t) * nsrc, hipMemcpyHostToDevice, st));
+            const size_t cells = (size_t)g->n * (size_t)s->s_pad;
+            const size_t items = (size_t)g->n * (size_t)s->ngroups;
+            for (int d = 0; d < s->ndir; d++) {
+                s->P[d].alloc(cells * s->esize);
+                s->R[d].alloc(cells * s->esize);
+                s->cm[d].alloc(items);
+                s->F[d].alloc(items);
+                TSVD_HIP(hipMemsetAsync(s->P[d].p, 0, s->P[d].bytes(), st));
+                TSVD_HIP(hipMemsetAsync(s->R[d].p, 0, s->R[d].bytes(), st));
+                TSVD_HIP(hipMemsetAsync(s->cm[d].p, 0, s->cm[d].bytes(), st));
+            }
+            s->nm.alloc(items);
+            s->fnext.alloc(items);
+            TSVD_HIP(hipMemsetAsync(s->nm.p, 0, s->nm.bytes(), st));
+            s->counters.alloc(8);
+            s->stats.alloc(4);
+            TSVD_HIP(hipMemsetAsync(s->counters.p, 0, s->counters.bytes(), st));
+            TSVD_HIP(hipHostMalloc((void **)&s->h_counters, 64, hipHostMallocDefault));
+            TSVD_HIP(hipStreamSynchronize(st));
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+extern "C" int tsvd_ppr_destroy(tsvd_ppr *s)
+{
+    return guard([&] { delete s; });
+}
+
+extern "C" int tsvd_ppr_refresh(tsvd_ppr *s, int mode, int dir, int32_t src_begin, int32_t src_end)
+{
+    return guard([&] {
+        TSVD_REQUIRE(s, "null ppr");
+        ppr_refresh(s, mode, dir, src_begin, src_end);
+    });
+}
+
+extern "C" int tsvd_ppr_push(tsvd_ppr *s, int dir, int32_t src_begin, int32_t src_end)
+{
+    return guard([&] {
+        TSVD_REQUIRE(s, "null ppr");
+        TSVD_REQUIRE(src_begin >= 0 && src_begin <= src_end && src_end <= s->nsrc, "bad source range");
+        ppr_push(s, dir);
+    });
+}
+
+extern "C" int tsvd_ppr_read(tsvd_ppr *s, int dir, int what, double *out)
+{
+    return guard([&] {
+        TSVD_REQUIRE(s && out, "null argument");
+        TSVD_REQUIRE(dir >= 0 && dir < s->ndir && (what == 0 || what == 1), "bad direction or field");
+        const size_t cells = (size_t)s->n * (size_t)s->s_pad;
+        std::vector<uint8_t> h(cells * s->esize);
+        const void *src = what == 0 ? s->p(dir) : s->r(dir);
+        TSVD_HIP(hipMemcpyAsync(h.data(), src, h.size(), hipMemcpyDeviceToHost, s->g->stream));
+        TSVD_HIP(hipStreamSynchronize(s->g->stream));
+        for (int32_t k = 0; k < s->nsrc; k++)
+            for (int32_t v = 0; v < s->n; v++) {
+                const size_t c = (size_t)v * s->s_pad + k;
+                out[(size_t)k * s->n + v] =
+                    s->esize == 8 ? ((const double *)h.data())[c] : (double)((const float *)h.data())[c];
+            }
+    });
+}
+
+extern "C" int tsvd_ppr_last_push_stats(tsvd_ppr *s, int dir, int64_t *rounds, int64_t *pops, int64_t *relax,
+                                        double *kernel_ms)
+{
+    return guard([&] {
+        TSVD_REQUIRE(s && dir >= 0 && dir < s->ndir, "bad ppr or direction");
+        if (rounds) *rounds = s->last_rounds[dir];
+        if (pops) *pops = s->last_pops[dir];
+        if (relax) *relax = s->last_relax[dir];
+        if (kernel_ms) *kernel_ms = s->last_ms[dir];
+    });
+}

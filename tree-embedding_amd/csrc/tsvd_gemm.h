@@ -1,0 +1,42 @@
+// tsvd_gemm.h -- grouped dense products on MFMA (FP32 or FP64 accumulate).
+#pragma once
+#include "tsvd_internal.h"
+
+namespace tsvd {
+
+enum class DT : int { F32 = 0, F64 = 1 };
+
+// One product  C (M x N) = alpha * op(A) * B + beta * C, all column-major:
+//   transA = 0: A is M x K,  A(i,k) = A[k*lda + i]
+//   transA = 1: A is K x M,  op(A)(i,k) = A[i*lda + k]   (C = A^T B)
+//   B is K x N,  B(k,j) = B[j*ldb + k];  C(i,j) = C[j*ldc + i].
+// kchunk > 0 splits K into chunks whose partial products are added to C
+// atomically (C must hold beta*C already; used for FP64 Grams).
+struct GemmProb {
+    const void *A;
+    const void *B;
+    void *C;
+    int64_t lda, ldb, ldc;
+    int32_t M, N, K, transA;
+    double alpha, beta;
+    int32_t kchunk;
+    int32_t tiles_m, tiles_n, tiles_k;
+    int64_t tile_begin;
+};
+
+// Element types of A, B, C and the accumulate type (F32 -> v_mfma_f32_16x16x4f32,
+// F64 -> v_mfma_f64_16x16x4f64).  `wide` selects the 144-column tile (else 80).
+struct GemmTypes {
+    DT a, b, c, acc;
+};
+
+class GemmRunner {
+public:
+    void run(hipStream_t st, std::vector<GemmProb> &probs, GemmTypes t, bool wide);
+    int64_t launches = 0;
+
+private:
+    DBuf<GemmProb> dprobs_;
+};
+
+}  // namespace tsvd

@@ -1,0 +1,140 @@
+// tsvd_pipeline.h -- column blocks, blockwise frPCA, merge tree, driver.
+#pragma once
+#include "tsvd_internal.h"
+#include "tsvd_graph.h"
+#include "tsvd_ppr.h"
+#include "tsvd_gemm.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+namespace tsvd {
+
+// Batched FP64 symmetric eigensolver on the device (rocSOLVER syevd).
+class Eig {
+public:
+    Eig();
+    ~Eig();
+    // `count` n x n matrices at stride n*n (column-major, overwritten by the
+    // eigenvectors); eigenvalues ascending into w (stride n).
+    void solve(hipStream_t st, double *A, int n, int count, double *w);
+
+private:
+    rocblas_handle h_ = nullptr;
+    DBuf<double> e_;
+    DBuf<rocblas_int> info_;
+};
+
+// Column blocks of the |S| x n matrix (Tree_u_NC.cpp:396-432): common = n /
+// nparts, block of v = min(v / common, nparts-1), the last block takes the
+// remainder.  Block j's factored matrix is kept dense over its active columns
+// (vertices with an edge, or sources), column-major with leading dimension
+// s_pad, at A + start_j * s_pad.
+struct Blocks {
+    int32_t nparts = 0, n = 0, nsrc = 0, s_pad = 0, common = 0;
+    int precision = TSVD_FP32, transform = TSVD_TRANSFORM_NC;
+    size_t esize = 4;
+    double rmax = 1e-7;
+    std::vector<int64_t> start, width;
+    std::vector<int32_t> ncols;             // active columns stored per block
+    DBuf<uint8_t> A;                        // n * s_pad * esize
+    DBuf<int32_t> slot, colmap, d_ncols;    // n, n, nparts
+    DBuf<uint8_t> active, is_source;        // n
+    DBuf<double> norms;                     // 2 * nparts: ||A_new||^2, ||A_new - A_old||^2
+    DBuf<int32_t> d_dirty;                  // dirty block ids
+    std::vector<double> a2, e2, resid;
+};
+
+// frPCA workspace for all blocks (rank k, l = k + 5).
+struct FrpcaWork {
+    int32_t k = 0, l = 0, q = 6;
+    DBuf<uint8_t> Qt;       // n * l * esize   (block j at start_j * l, ld = width_j)
+    DBuf<uint8_t> Q, Q2;    // nparts * s_pad * l * esize
+    DBuf<double> G, W, lam; // nparts * l * l, nparts * l * l, nparts * l
+};
+
+// Merge tree: level 0 = leaves.  Level sizes from the leaves up; parent i of
+// a level holds children [i*f, min((i+1)*f, size)) of the level below (the
+// reference's heap tree when nparts is a power of f, ppr:225-245).  Node
+// factors U*S are s_pad x k column-major, consecutive within a level, so a
+// parent's children form one s_pad x (f*k) matrix.
+struct Tree {
+    int32_t fanout = 4, k = 0, s_pad = 0, nsrc = 0;
+    std::vector<int32_t> sizes;         // nodes per level
+    std::vector<int32_t> padded;        // sizes rounded up to a multiple of f
+    std::vector<DBuf<uint8_t>> level;   // padded[l] * s_pad * k * esize
+    DBuf<double> G, lam, Wm;            // Gram / eigen scratch per merge batch
+    DBuf<double> E;                     // root embedding s_pad x k
+    std::vector<double> root_sigma;
+    size_t esize = 4;
+    bool have_root = false;
+};
+
+void blocks_setup(Blocks &b, int32_t n, int32_t nparts, int32_t nsrc, int32_t s_pad, int precision, int transform,
+                  double rmax, const std::vector<int32_t> &sources, hipStream_t st);
+// ||A_new||^2 and ||A_new - A_old||^2 of every block (p0 [+ p1] transformed).
+void blocks_norms(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g, hipStream_t st);
+// Rebuild the stored matrix of the listed blocks from p0 [+ p1].
+void blocks_rebuild(Blocks &b, const std::vector<int32_t> &dirty, const void *p0, const void *p1, hipStream_t st);
+
+void frpca_setup(FrpcaWork &w, const Blocks &b, int32_t k, int32_t q);
+// Factor the listed blocks; leaf factors U*S go to `leaves` (s_pad x k per
+// block, block j at j * s_pad * k).  Returns sum of the kept sigma^2 per block
+// in sig2 (host, indexed like `dirty`).
+void frpca_blocks(FrpcaWork &w, Blocks &b, const std::vector<int32_t> &dirty, void *leaves, GemmRunner &gemm,
+                  Eig &eig, uint64_t seed, hipStream_t st, std::vector<double> &sig2);
+
+void tree_setup(Tree &t, int32_t nparts, int32_t fanout, int32_t k, int32_t nsrc, int32_t s_pad, size_t esize,
+                hipStream_t st);
+// Recompute every ancestor of the dirty leaves, level by level, and the root
+// embedding (mode NC: U*sqrt|S|, LP: U).  Returns the number of merged nodes.
+int32_t tree_update(Tree &t, const std::vector<int32_t> &dirty_leaves, int transform, GemmRunner &gemm, Eig &eig,
+                    hipStream_t st);
+
+// Elementwise helpers (blocks.hip)
+void gaussian_fill(void *X, int precision, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, hipStream_t st);
+// W[:, c] = V[:, c] * scale(c) with scale = 1/sqrt(lam) (zeroed below the
+// guard) for the orthonormalising rotation; batch of `count` l x l matrices.
+void eig_rotation(double *W, const double *V, const double *lam, int l, int count, double eps_abs, double eps_rel,
+                  hipStream_t st);
+// Top-k selection: W[:, c] = V[:, src(c)] * f(lam[src(c)]) for c < k, with
+// src(c) = l-1-c (descending) or l-k+c (ascending, the reference's frPCA
+// order) and, for sigma = sqrt(lam), f = sigma (mode 0, U*S), sigma^-1/2
+// (mode 1, NC root U*sqrt(S)), 1/sigma (mode 2, U), 1 (mode 3); f = 0 when
+// sigma <= eps (modes 0-2).  Matrices at stride l*l (V) and l*k (W).
+void eig_select(double *W, const double *V, const double *lam, int l, int k, int count, int mode, int ascending,
+                double eps, hipStream_t st);
+
+// One column block (or one drop-in matrix) for the grouped frPCA core.
+struct FrpcaItem {
+    const void *A;    // m x nc, column-major, leading dimension lda
+    int64_t lda;
+    int32_t m, nc;
+    void *Qt;         // nc x l, leading dimension ldqt
+    int64_t ldqt;
+    void *Q, *Q2;     // m x l each, leading dimension ldq
+    int64_t ldq;
+    uint64_t seed;
+};
+// Runs the passes of frPCA (frpca.c:239-316) for all items at once.  On
+// return Q holds the final orthonormal basis, Qt = A^T Q, and G / lam (item i
+// at i*l*l / i*l) the eigenvectors and ascending eigenvalues of Qt^T Qt.
+void frpca_core(std::vector<FrpcaItem> &items, int precision, int32_t l, int32_t q, double *G, double *W,
+                double *lam, GemmRunner &gemm, Eig &eig, hipStream_t st);
+
+}  // namespace tsvd
+
+struct tsvd_pipeline {
+    tsvd_config cfg{};
+    std::vector<int32_t> sources;
+    tsvd_graph *g = nullptr;
+    tsvd_ppr *ppr = nullptr;
+    tsvd::Blocks blocks;
+    tsvd::FrpcaWork fw;
+    tsvd::Tree tree;
+    tsvd::GemmRunner gemm;
+    tsvd::Eig *eig = nullptr;
+    int32_t snapshot = 0;
+    std::vector<int32_t> last_dirty;
+    ~tsvd_pipeline();
+};

@@ -1,0 +1,40 @@
+// tsvd_ppr.h -- device state of the subset PPR (reserves p, residues r).
+#pragma once
+#include "tsvd_internal.h"
+#include "tsvd_graph.h"
+
+// Layout: vertex-major.  p[v * s_pad + s] is source s's reserve at vertex v
+// (s_pad = |S| rounded up to 64), so one 64-lane wavefront covers one vertex
+// for 64 consecutive sources ("source group" g = s / 64) and every access of
+// the update, the push and the block transform is a contiguous 256/512-byte
+// segment.  The pending work of a direction is a list of (vertex, group)
+// items plus a 64-bit lane mask per item (cm); a push round pops the list
+// and builds the next one in (nm, fnext).
+struct tsvd_ppr {
+    tsvd_graph *g = nullptr;
+    int32_t n = 0, nsrc = 0, s_pad = 0, ngroups = 0, ndir = 1, directed = 0;
+    double alpha = 0.2, rmax = 1e-7;
+    int precision = TSVD_FP32;
+    size_t esize = 4;
+    tsvd::DBuf<int32_t> sources;
+    tsvd::DBuf<uint8_t> P[2], R[2];
+    tsvd::DBuf<unsigned long long> cm[2], nm;
+    tsvd::DBuf<uint32_t> F[2], fnext;
+    tsvd::DBuf<uint32_t> counters;           // [0],[1] pending per dir, [2] next, [3] overflow
+    tsvd::DBuf<unsigned long long> stats;    // [0] pops, [1] relaxations
+    uint32_t *h_counters = nullptr;          // pinned
+    int64_t last_rounds[2] = {0, 0}, last_pops[2] = {0, 0}, last_relax[2] = {0, 0};
+    double last_ms[2] = {0, 0};
+    std::vector<int32_t> h_sources;
+    ~tsvd_ppr()
+    {
+        if (h_counters) (void)hipHostFree(h_counters);
+    }
+    void *p(int d) { return P[d].p; }
+    void *r(int d) { return R[d].p; }
+};
+
+namespace tsvd {
+void ppr_refresh(tsvd_ppr *s, int mode, int dir, int32_t s0, int32_t s1);
+void ppr_push(tsvd_ppr *s, int dir);
+}

@@ -1,0 +1,187 @@
+"""Seeded synthetic workloads for the update->embedding path.
+
+The reference runs on real snapshot files prepared by its own tools (subset
+sampling Random_Sample_Points_LP.cpp:80-140, batch splitting
+split_edges_NC.cpp:57-118).  These generators produce graphs of the same kind
+in memory, deterministically from a seed:
+
+* R-MAT (a, b, c, d) with optional node-id modulo, self-loops dropped,
+  duplicates dropped (first occurrence kept), undirected edges canonicalised
+  to (min, max);
+* directed Chung-Lu with power-law out/in weights;
+* a seeded shuffle cut into an initial snapshot and insertion batches
+  (equal, lognormal, or a Poisson-arrival stream);
+* the subset: distinct uniform draws among the edge sources of the initial
+  snapshot, as Random_Sample_Points_LP.cpp:80-140 does.
+
+All ids are int32; edge lists are (src, dst) pairs of int32 arrays.
+"""
+from __future__ import annotations
+
+import dataclasses
+import numpy as np
+
+
+def _rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def _dedup_first(keys: np.ndarray) -> np.ndarray:
+    """Indices of the first occurrence of every distinct key, in order."""
+    _, first = np.unique(keys, return_index=True)
+    first.sort()
+    return first
+
+
+def rmat_edges(scale: int, m: int, seed: int, abcd=(0.57, 0.19, 0.19, 0.05),
+               directed: bool = False, n_mod: int | None = None,
+               chunk: int = 1 << 24):
+    """Exactly `m` distinct R-MAT edges over n = 2**scale (or n_mod) nodes."""
+    a, b, c, _ = abcd
+    n = n_mod if n_mod else (1 << scale)
+    rng = _rng(seed)
+    keys = np.empty(0, dtype=np.int64)
+    while keys.size < m:
+        need = m - keys.size
+        batch = min(chunk, int(need * 1.25) + 4096)
+        u = np.zeros(batch, dtype=np.int64)
+        v = np.zeros(batch, dtype=np.int64)
+        for _ in range(scale):
+            r = rng.random(batch)
+            bu = r >= (a + b)
+            bv = ((r >= a) & (r < a + b)) | (r >= a + b + c)
+            u = (u << 1) | bu
+            v = (v << 1) | bv
+        if n_mod:
+            u %= n_mod
+            v %= n_mod
+        keep = u != v
+        u, v = u[keep], v[keep]
+        if not directed:
+            u, v = np.minimum(u, v), np.maximum(u, v)
+        k = np.concatenate([keys, u * n + v])
+        keys = k[_dedup_first(k)]
+    keys = keys[:m]
+    return (keys // n).astype(np.int32), (keys % n).astype(np.int32), n
+
+
+def chung_lu_edges(n: int, m: int, seed: int, out_exp: float, in_exp: float,
+                   chunk: int = 1 << 24):
+    """Exactly `m` distinct directed Chung-Lu edges (no self-loops)."""
+    rng = _rng(seed)
+    i = np.arange(n, dtype=np.float64)
+    w_out = (i + 1.0) ** (-1.0 / (out_exp - 1.0))
+    perm = rng.permutation(n)
+    w_in = (perm + 1.0) ** (-1.0 / (in_exp - 1.0))
+    c_out = np.cumsum(w_out); c_out /= c_out[-1]
+    c_in = np.cumsum(w_in); c_in /= c_in[-1]
+    keys = np.empty(0, dtype=np.int64)
+    while keys.size < m:
+        need = m - keys.size
+        batch = min(chunk, int(need * 1.3) + 4096)
+        u = np.searchsorted(c_out, rng.random(batch), side="right").astype(np.int64)
+        v = np.searchsorted(c_in, rng.random(batch), side="right").astype(np.int64)
+        u = np.minimum(u, n - 1); v = np.minimum(v, n - 1)
+        keep = u != v
+        k = np.concatenate([keys, u[keep] * n + v[keep]])
+        keys = k[_dedup_first(k)]
+    keys = keys[:m]
+    return (keys // n).astype(np.int32), (keys % n).astype(np.int32)
+
+
+def shuffle_split(src, dst, seed: int, init_frac: float, batch_sizes):
+    """Seeded shuffle; the first init_frac of edges is the initial snapshot,
+    then consecutive batches of the given sizes (truncated to what is left)."""
+    rng = _rng(seed + 7919)
+    perm = rng.permutation(src.size)
+    s, d = src[perm], dst[perm]
+    n0 = int(round(init_frac * src.size))
+    snaps = [(s[:n0], d[:n0])]
+    pos = n0
+    for b in batch_sizes:
+        if pos >= src.size:
+            break
+        b = int(min(b, src.size - pos))
+        snaps.append((s[pos:pos + b], d[pos:pos + b]))
+        pos += b
+    return snaps
+
+
+def lognormal_sizes(count: int, median: float, sigma: float, seed: int,
+                    lo: int = 1, hi: int | None = None):
+    rng = _rng(seed + 104729)
+    x = np.exp(np.log(median) + sigma * rng.standard_normal(count))
+    x = np.maximum(np.rint(x), lo)
+    if hi is not None:
+        x = np.minimum(x, hi)
+    return x.astype(np.int64)
+
+
+def sample_subset(init_src, nsub: int, seed: int):
+    """Distinct uniform draws among the edge sources of the initial snapshot
+    (Random_Sample_Points_LP.cpp:80-140)."""
+    cand = np.unique(init_src)
+    rng = _rng(seed + 15485863)
+    if nsub > cand.size:
+        raise ValueError("subset larger than the number of edge sources")
+    return rng.choice(cand, size=nsub, replace=False).astype(np.int32)
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    n: int
+    directed: bool
+    snapshots: list            # [(src, dst)], snapshot 0 is the initial graph
+    subset: np.ndarray
+    alpha: float
+    rmax: float
+    nparts: int
+    fanout: int
+    rank: int
+    q: int                     # frPCA passes (2 power iterations -> q = 6)
+    fp64_state: bool           # PPR reserves/residues kept in FP64
+    transform: str = "NC"      # "NC": log10(x/rmax) for x>rmax ; "LP": log10(1+x/rmax)
+    delta: float = 0.65
+
+
+def workload(name: str, scale_down: int = 0, nsub: int | None = None,
+             n_snap: int | None = None) -> Workload:
+    """The named synthetic configurations.
+
+    plain      undirected R-MAT, 131,072 nodes, 1M edges, 70% + 3x10%, |S|=256,
+               alpha 0.2, rmax 1e-6, 16 blocks, rank 64, q=6, fan-out 2, FP64.
+    batched    undirected R-MAT, 2,097,152 nodes, 16M edges, 60% + 12 lognormal
+               batches (median 200k, sigma 1), |S|=3000, rmax 1e-7, 256 blocks,
+               rank 128, q=6, fan-out 4, FP32 state.
+    precision  directed Chung-Lu, 2,738,013 nodes, 13,960,811 edges, exponents
+               out 2.1 / in 2.3, 80% + 8 equal batches, |S|=3000, rmax 1e-7,
+               512 blocks (ragged fan-out-4 tree), rank 128.
+    `scale_down` divides node and edge counts by 2**scale_down (tests only).
+    """
+    sd = 1 << scale_down
+    if name == "plain":
+        m = 1_000_000 // sd
+        s, d, n = rmat_edges(17 - scale_down, m, seed=0)
+        sizes = [m // 10] * 3
+        snaps = shuffle_split(s, d, 0, 0.7, sizes)
+        sub = sample_subset(snaps[0][0], nsub or 256, 0)
+        return Workload(name, n, False, snaps, sub, 0.2, 1e-6, 16, 2, 64, 6, True)
+    if name == "batched":
+        m = 16_000_000 // sd
+        s, d, n = rmat_edges(21 - scale_down, m, seed=1)
+        k = n_snap if n_snap is not None else 12
+        sizes = lognormal_sizes(k, 200_000 / sd, 1.0, 1)
+        snaps = shuffle_split(s, d, 1, 0.6, sizes)
+        sub = sample_subset(snaps[0][0], nsub or 3000, 1)
+        return Workload(name, n, False, snaps, sub, 0.2, 1e-7, 256, 4, 128, 6, False)
+    if name == "precision":
+        n = 2_738_013 // sd
+        m = 13_960_811 // sd
+        s, d = chung_lu_edges(n, m, seed=2, out_exp=2.1, in_exp=2.3)
+        rest = m - int(round(0.8 * m))
+        sizes = [rest // 8 + (1 if i < rest % 8 else 0) for i in range(8)]
+        snaps = shuffle_split(s, d, 2, 0.8, sizes)
+        sub = sample_subset(snaps[0][0], nsub or 3000, 2)
+        return Workload(name, n, True, snaps, sub, 0.2, 1e-7, 512, 4, 128, 6, False)
+    raise ValueError(f"unknown workload {name!r}")
