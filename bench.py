@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Update->embedding throughput of the Tree-SVD path on one MI355X.
+
+A step is one snapshot of the plain per-snapshot method (tsvd_pipeline_snapshot):
+an edge-insertion batch, already resident in HBM, goes through ingest ->
+dynamic PPR update -> push -> block transform + lazy dirty test -> blockwise
+frPCA of the dirty blocks -> dirty-path merges -> root embedding.  The initial
+snapshot (static push over the first 60% of the edges) is setup and is timed
+separately.  Output: one JSON line (snapshots/s, p50/p99 ms per update, the
+dominant kernel's roofline, and the CPU restatement timed on this host).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batched]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tree-embedding_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP32_MFMA_TFLOPS = 157.3       # MI355X dense FP32 matrix peak
+FP64_MFMA_TFLOPS = 78.6
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def pipeline_for(capi, w, seed=7):
+    return capi.Pipeline(w.n, w.directed, w.subset, w.alpha, w.rmax, w.nparts, w.fanout, w.rank, w.q,
+                         precision=capi.TSVD_FP64 if w.fp64_state else capi.TSVD_FP32,
+                         transform=capi.TSVD_TRANSFORM_NC if w.transform == "NC" else capi.TSVD_TRANSFORM_LP,
+                         delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed)
+
+
+def roofline_of(prof, precision_fp64):
+    """Dominant kernel by total time; HBM-bound kernels report GB/s against
+    8 TB/s, the MFMA products TFLOP/s against the dense matrix peak."""
+    name, (launches, ms, nbytes, flops) = max(prof.items(), key=lambda kv: kv[1][1])
+    if name.startswith("gemm"):
+        peak = FP64_MFMA_TFLOPS if precision_fp64 else FP32_MFMA_TFLOPS
+        achieved = flops / (ms * 1e-3) / 1e12
+        return {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                "launches": launches, "avg_launch_ms": round(ms / launches, 4),
+                "algorithmic_per_launch": flops / launches}
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "launches": launches, "avg_launch_ms": round(ms / launches, 4),
+            "algorithmic_per_launch": nbytes / launches}
+
+
+def cpu_baseline(w, gpu_ndirty, gpu_nmerged, n_steps_hint, budget_s=25.0):
+    """The oracle (the CPU restatement of the reference path: C FIFO push on
+    all host threads, numpy FP64 frPCA and merges) on a bounded sample of the
+    same workload: the push/update of a sample of sources over the initial
+    snapshot and one insertion batch, scaled by |S|/sample (sources are
+    independent), plus one full-size frPCA and one merge, scaled by the
+    device's dirty-block and merged-node counts per step."""
+    from oracle import oracle as orc
+    nthreads = os.cpu_count() or 1
+    nsamp = min(w.subset.size, max(8, 2 * nthreads))
+    sub = w.subset[:nsamp]
+    g = orc.Graph(w.n, w.directed)
+    s0, d0 = w.snapshots[0]
+    g.append(s0, d0)
+    ppr = orc.PPR(w.n, sub, w.directed, w.alpha, w.rmax, fp32=not w.fp64_state)
+    ppr.init_static()
+    ppr.push(g, nthreads)
+    s1, d1 = w.snapshots[1]
+    t0 = time.perf_counter()
+    g.append(s1, d1)
+    ppr.refresh_dynamic(g, nthreads)
+    ppr.push(g, nthreads)
+    t_push = (time.perf_counter() - t0) * (w.subset.size / nsamp)
+    starts, widths = orc.block_map(w.n, w.nparts)
+    rng = np.random.Generator(np.random.PCG64(1))
+    A = np.abs(rng.standard_normal((w.subset.size, int(widths[0])))) * 3.0
+    t1 = time.perf_counter()
+    orc.frpca(A, w.rank, w.q, omega=orc.sketch(3, A.shape[1], w.rank + 5))
+    t_frpca = time.perf_counter() - t1
+    C = rng.standard_normal((w.subset.size, w.fanout * w.rank))
+    t2 = time.perf_counter()
+    orc.merge([C], w.rank)
+    t_merge = time.perf_counter() - t2
+    per_snap = t_push + gpu_ndirty * t_frpca + gpu_nmerged * t_merge
+    return {"value": round(1.0 / per_snap, 6), "unit": "snapshots/s", "cores": nthreads, "kind": "port",
+            "seconds_per_snapshot": round(per_snap, 3),
+            "sample": (f"push+update of {nsamp}/{w.subset.size} sources over one insertion batch "
+                       f"({s1.size} edges) scaled x{w.subset.size / nsamp:.1f}; one {A.shape[0]}x{A.shape[1]} "
+                       f"FP64 frPCA (k={w.rank}, q={w.q}) x {gpu_ndirty:.1f} dirty blocks; one "
+                       f"{C.shape[0]}x{C.shape[1]} merge SVD x {gpu_nmerged:.1f} merges"),
+            "phase_s": {"push_update": round(t_push, 3), "frpca_one_block": round(t_frpca, 3),
+                        "merge_one": round(t_merge, 3)}}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="batched", choices=["plain", "batched", "precision"])
+    ap.add_argument("--scale-down", type=int, default=0, help="divide nodes/edges by 2**k (smoke runs)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is not launched by this build)"}))
+        sys.exit(2)
+
+    from treesvd_amd import capi, gen
+    t = time.time()
+    w = gen.workload(args.workload, scale_down=args.scale_down)
+    log(f"workload {args.workload}: n={w.n} snapshots={len(w.snapshots)} |S|={w.subset.size} "
+        f"gen {time.time() - t:.1f}s")
+    need = args.warmup + args.steps
+    if need > len(w.snapshots) - 1:
+        raise SystemExit(f"workload has {len(w.snapshots) - 1} insertion batches; warmup+steps={need}")
+    name, hbm_total, hbm_free = capi.device_info()
+    log(f"device {name}, HBM {hbm_total / 2**30:.0f} GiB ({hbm_free / 2**30:.0f} free)")
+
+    pipe = pipeline_for(capi, w)
+    batches = [(capi.DeviceArray(s.astype(np.int32)), capi.DeviceArray(d.astype(np.int32)))
+               for s, d in w.snapshots[:1 + need]]
+    t = time.perf_counter()
+    st0 = pipe.snapshot(*batches[0])
+    init_ms = (time.perf_counter() - t) * 1e3
+    log(f"initial snapshot {init_ms:.0f} ms (push {st0['ms_push']:.0f} ms, {st0['push_rounds']} rounds, "
+        f"frpca {st0['ms_frpca']:.0f} ms, {st0['ndirty']} blocks)")
+    for i in range(args.warmup):
+        st = pipe.snapshot(*batches[1 + i])
+        log(f"warmup {i}: {st['ms_total']:.1f} ms dirty={st['ndirty']}")
+
+    capi.profile_reset()
+    capi.profile_enable(True)
+    lat, stats = [], []
+    t_all = time.perf_counter()
+    for i in range(args.steps):
+        t = time.perf_counter()
+        st = pipe.snapshot(*batches[1 + args.warmup + i])
+        lat.append((time.perf_counter() - t) * 1e3)
+        stats.append(st)
+        log(f"step {i}: {lat[-1]:.1f} ms dirty={st['ndirty']} merged={st['nmerged']} "
+            f"push {st['ms_push']:.1f} ({st['push_rounds']} rounds) blocks {st['ms_blocks']:.1f} "
+            f"frpca {st['ms_frpca']:.1f} merge {st['ms_merge']:.1f} pops {st['push_pops']} relax {st['push_relax']} "
+            f"items {st['push_items']} wave-relax {st['push_wave_relax']}")
+    total_s = time.perf_counter() - t_all
+    capi.profile_enable(False)
+    prof = capi.profile_read()
+    value = args.steps / total_s
+    ndirty = float(np.mean([s["ndirty"] for s in stats]))
+    nmerged = float(np.mean([s["nmerged"] for s in stats]))
+    phases = {k: round(float(np.mean([s[k] for s in stats])), 3)
+              for k in ("ms_ingest", "ms_refresh", "ms_push", "ms_blocks", "ms_frpca", "ms_merge")}
+    roof = roofline_of(prof, w.fp64_state)
+    out = {
+        "metric": "update->embedding snapshots/s (plain GPU method), p50/p99 ms per update",
+        "value": round(value, 4), "unit": "snapshots/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(total_s / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if w.fp64_state else "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "n": w.n, "directed": w.directed,
+                   "edges": int(sum(s.size for s, _ in w.snapshots)), "sources": int(w.subset.size),
+                   "alpha": w.alpha, "rmax": w.rmax, "nparts": w.nparts, "fanout": w.fanout, "rank": w.rank,
+                   "q": w.q, "scale_down": args.scale_down, "method": "plain"},
+        "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "initial_snapshot_ms": round(init_ms, 1), "phase_ms": phases,
+        "dirty_blocks_per_step": ndirty, "merged_nodes_per_step": nmerged,
+        "roofline": roof,
+        "kernels": {k: {"launches": v[0], "ms": round(v[1], 3), "GB": round(v[2] / 1e9, 3),
+                        "GFLOP": round(v[3] / 1e9, 3)} for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])},
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(w, ndirty, nmerged, args.steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

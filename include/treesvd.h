@@ -44,6 +44,13 @@ int tsvd_device_info(char *name, int name_len, int64_t *hbm_total, int64_t *hbm_
 int tsvd_device_alloc(int64_t bytes, void **out);
 int tsvd_device_free(void *p);
 int tsvd_memcpy(void *dst, const void *src, int64_t bytes, int kind);
+/* Per-kernel timing (no reference counterpart; the reference prints phase
+ * wall-clock seconds, e.g. Tree_u_NC.cpp:526-563).  When enabled, each library
+ * launch records a HIP event pair on the stream it runs on; read returns a JSON
+ * object {"kernel": [launches, total_ms], ...}. */
+int tsvd_profile_enable(int on);
+int tsvd_profile_reset(void);
+int tsvd_profile_read(char *json, int len);
 
 /* ------------------------------------------------------------------ graph
  * Replaces UGraph::initializeDynamicUGraph / inputDynamicGraph
@@ -144,6 +151,8 @@ typedef struct {
     int32_t nmerged;      /* internal tree nodes recomputed (incl. root)    */
     int32_t reused;       /* 1: no dirty block, previous embedding kept     */
     int64_t push_rounds, push_pops, push_relax;
+    int64_t push_items;       /* (vertex, 64-source group) work items popped   */
+    int64_t push_wave_relax;  /* neighbour visits summed over those items      */
     double ms_total, ms_ingest, ms_refresh, ms_push, ms_blocks, ms_frpca, ms_merge;
 } tsvd_snapshot_stats;
 

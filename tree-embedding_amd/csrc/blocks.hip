@@ -274,6 +274,14 @@ void blocks_norms(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g
     int64_t maxw = 0;
     for (auto w : b.width) maxw = std::max(maxw, w);
     dim3 grid((unsigned)ceil_div(maxw, VPB), (unsigned)b.nparts);
+    if (g_profile) {
+        // reserves of every vertex (one or two directions) + the stored matrices
+        double stored = 0.0;
+        for (int32_t j = 0; j < b.nparts; j++) stored += (double)b.ncols[j];
+        const double dirs = p1 ? 2.0 : 1.0;
+        prof_add_work("blocks_norms", (dirs * b.n + stored) * b.nsrc * (double)b.esize, 0.0);
+    }
+    PROF("blocks_norms", st);
     if (b.precision == TSVD_FP64)
         hipLaunchKernelGGL(k_block_norms<double>, grid, dim3(TPB), 0, st, (const double *)p0, (const double *)p1,
                            (const double *)b.A.p, b.slot.p, b.n, b.nsrc, b.s_pad, b.nparts, b.common, b.transform,
@@ -302,6 +310,12 @@ void blocks_rebuild(Blocks &b, const std::vector<int32_t> &dirty, const void *p0
     int64_t maxw = 0;
     for (int32_t j : dirty) maxw = std::max(maxw, b.width[j]);
     dim3 grid((unsigned)ceil_div(maxw, TPB / 64), (unsigned)nd);
+    PROF("blocks_gather", st);
+    if (g_profile) {
+        double w = 0.0;
+        for (int32_t j : dirty) w += (double)b.width[j];
+        prof_add_work("blocks_gather", w * b.nsrc * (double)b.esize * (p1 ? 3.0 : 2.0), 0.0);
+    }
     if (b.precision == TSVD_FP64)
         hipLaunchKernelGGL(k_gather<double>, grid, dim3(TPB), 0, st, b.d_dirty.p, (const double *)p0,
                            (const double *)p1, (double *)b.A.p, b.colmap.p, b.d_ncols.p, b.nsrc, b.s_pad, b.common,

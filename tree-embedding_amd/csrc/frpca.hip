@@ -38,6 +38,7 @@ void Eig::solve(hipStream_t st, double *A, int n, int count, double *w)
     if (count <= 0) return;
     e_.ensure((size_t)n * count);
     info_.ensure((size_t)count);
+    PROF("eig_syevd", st);
     if (rocblas_set_stream(h_, st) != rocblas_status_success)
         throw Error{set_error(TSVD_EHIP, "rocblas_set_stream failed")};
     rocblas_status s = rocsolver_dsyevd_strided_batched(h_, rocblas_evect_original, rocblas_fill_upper, n, A, n,
@@ -69,7 +70,7 @@ static void orth(std::vector<FrpcaItem> &items, int precision, int32_t l, double
         p.M = l; p.N = l; p.K = it.m; p.alpha = 1.0; p.beta = 0.0; p.kchunk = 512;
         gp.push_back(p);
     }
-    gemm.run(st, gp, types_for(precision, T, T, DT::F64, DT::F64), l > 80);
+    gemm.run(st, gp, types_for(precision, T, T, DT::F64, DT::F64), l > 80, "gemm_gram");
     eig.solve(st, G, l, cnt, lam);
     const double eps_rel = precision == TSVD_FP64 ? 1e-13 : 1e-6;
     eig_rotation(W, G, lam, l, cnt, 1e-4, eps_rel, st);
@@ -83,7 +84,7 @@ static void orth(std::vector<FrpcaItem> &items, int precision, int32_t l, double
         p.M = it.m; p.N = l; p.K = l; p.alpha = 1.0; p.beta = 0.0;
         rp.push_back(p);
     }
-    gemm.run(st, rp, types_for(precision, T, DT::F64, T, DT::F64), l > 80);
+    gemm.run(st, rp, types_for(precision, T, DT::F64, T, DT::F64), l > 80, "gemm_rotate");
     for (auto &it : items) std::swap(it.Q, it.Q2);
 }
 
@@ -104,7 +105,7 @@ static void product(std::vector<FrpcaItem> &items, int precision, int32_t l, boo
         }
         ps.push_back(p);
     }
-    gemm.run(st, ps, GemmTypes{T, T, T, T}, l > 80);
+    gemm.run(st, ps, GemmTypes{T, T, T, T}, l > 80, transposed ? "gemm_sketch_t" : "gemm_sketch_n");
 }
 
 void frpca_core(std::vector<FrpcaItem> &items, int precision, int32_t l, int32_t q, double *G, double *W,
@@ -139,7 +140,7 @@ void frpca_core(std::vector<FrpcaItem> &items, int precision, int32_t l, int32_t
         p.M = l; p.N = l; p.K = it.nc; p.alpha = 1.0; p.beta = 0.0; p.kchunk = 512;
         gp.push_back(p);
     }
-    gemm.run(st, gp, GemmTypes{T, T, DT::F64, DT::F64}, l > 80);
+    gemm.run(st, gp, GemmTypes{T, T, DT::F64, DT::F64}, l > 80, "gemm_gram");
     eig.solve(st, G, l, cnt, lam);
 }
 
@@ -195,7 +196,7 @@ void frpca_blocks(FrpcaWork &w, Blocks &b, const std::vector<int32_t> &dirty, vo
         p.M = b.nsrc; p.N = k; p.K = l; p.alpha = 1.0; p.beta = 0.0;
         lp.push_back(p);
     }
-    gemm.run(st, lp, GemmTypes{T, DT::F64, T, DT::F64}, k > 80);
+    gemm.run(st, lp, GemmTypes{T, DT::F64, T, DT::F64}, k > 80, "gemm_leaf");
     std::vector<double> lam((size_t)cnt * l);
     TSVD_HIP(hipMemcpyAsync(lam.data(), w.lam.p, sizeof(double) * lam.size(), hipMemcpyDeviceToHost, st));
     TSVD_HIP(hipStreamSynchronize(st));

@@ -169,7 +169,7 @@ void launch_t(hipStream_t st, const GemmProb *d, int nprob, int64_t tiles, bool 
 
 }  // namespace
 
-void GemmRunner::run(hipStream_t st, std::vector<GemmProb> &probs, GemmTypes t, bool wide)
+void GemmRunner::run(hipStream_t st, std::vector<GemmProb> &probs, GemmTypes t, bool wide, const char *label)
 {
     const int BN = wide ? 144 : 80;
     int64_t tiles = 0;
@@ -196,6 +196,16 @@ void GemmRunner::run(hipStream_t st, std::vector<GemmProb> &probs, GemmTypes t, 
     const int np = (int)live.size();
     // the staging vector must outlive the copy
     TSVD_HIP(hipStreamSynchronize(st));
+    if (g_profile) {
+        auto sz = [](DT d) { return d == DT::F64 ? 8.0 : 4.0; };
+        double flops = 0.0, bytes = 0.0;
+        for (const auto &p : live) {
+            flops += 2.0 * p.M * p.N * p.K;
+            bytes += (double)p.M * p.K * sz(t.a) + (double)p.K * p.N * sz(t.b) + (double)p.M * p.N * sz(t.c);
+        }
+        prof_add_work(label, bytes, flops);
+    }
+    PROF(label, st);
     if (t.acc == DT::F32) {
         TSVD_REQUIRE(t.a == DT::F32 && t.b == DT::F32 && t.c == DT::F32, "FP32 accumulate takes FP32 operands");
         launch_t<float, float, float, float>(st, dprobs_.p, np, tiles, wide);

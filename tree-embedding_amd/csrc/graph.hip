@@ -285,7 +285,10 @@ void graph_append(tsvd_graph *g, const int32_t *src, const int32_t *dst, int64_t
         TSVD_HIP(hipStreamSynchronize(st));
         TSVD_REQUIRE(h[0] == 0, "edge batch holds a vertex id outside [0, %d)", g->n);
     }
-    for (int d = 0; d < g->ndir; d++) append_dir(g, d, dsrc, ddst, m);
+    {
+        PROF("graph_append", st);
+        for (int d = 0; d < g->ndir; d++) append_dir(g, d, dsrc, ddst, m);
+    }
     g->m += m;
     TSVD_HIP(hipStreamSynchronize(st));
 }

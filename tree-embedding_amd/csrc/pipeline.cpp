@@ -111,6 +111,8 @@ extern "C" int tsvd_pipeline_snapshot(tsvd_pipeline *p, const int32_t *src, cons
             S.push_rounds += p->ppr->last_rounds[d];
             S.push_pops += p->ppr->last_pops[d];
             S.push_relax += p->ppr->last_relax[d];
+            S.push_items += p->ppr->last_items[d];
+            S.push_wave_relax += p->ppr->last_wave_relax[d];
         }
         const double t3 = now_ms();
         S.ms_push = t3 - t2;

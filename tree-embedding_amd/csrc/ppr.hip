@@ -137,7 +137,12 @@ __global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict
     if (c2 && lane == 0) mark(c2, (uint32_t)i * ngroups + g, cm, list, count, cap, overflow);
 }
 
-// One synchronous push round: one wave per pending item.
+// One synchronous push round: one wave per pending item pops its lanes and
+// relaxes v's neighbours.  The atomic add returns the previous residue, so a
+// lane whose residue crosses rmax upward is detected where it happens; the
+// wave's ballot of crossing lanes is OR-ed into the next round's mask of
+// (u, g) and the first setter appends the item.  Push counters are summed per
+// workgroup in LDS before one global update.
 template <class T>
 __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__restrict__ R,
                                                     const uint32_t *__restrict__ items, uint32_t nitems,
@@ -150,77 +155,87 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                                                     int32_t s_pad, int32_t ngroups, double alpha, T rm, int directed,
                                                     unsigned long long *__restrict__ stats)
 {
+    __shared__ unsigned long long sstat[4];
+    if (threadIdx.x < 4) sstat[threadIdx.x] = 0ull;
+    __syncthreads();
     const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (w >= nitems) return;
-    const uint32_t item = items[w];
-    const int32_t v = (int32_t)(item / (uint32_t)ngroups);
-    const int32_t g = (int32_t)(item - (uint32_t)v * (uint32_t)ngroups);
-    const unsigned long long m = cm[item];
-    if (lane == 0) cm[item] = 0ull;
-    const int32_t s = g * 64 + lane;
-    const bool active = (m >> lane) & 1ull;
-    const int32_t dv = deg[v];
-    T *Rv = R + (int64_t)v * s_pad + s;
-    T *Pv = P + (int64_t)v * s_pad + s;
-    T x = T(0);
-    bool pushing = false, again = false;
-    if (active) {
-        x = xchg(Rv, T(0));
-        if (!(x > rm)) {
-            T old = atomicAdd(Rv, x);                     // not above rmax: give it back
-            again = old <= rm && old + x > rm && dv > 0;
-        } else if (dv == 0) {
-            if (!directed) *Pv = (T)(alpha * (double)x);  // ppr:1431-1441
-            atomicAdd(Rv, x);                             // residue kept, not re-queued
-        } else {
-            pushing = true;
-            *Pv = (T)((double)*Pv + alpha * (double)x);
+    if (w < nitems) {
+        const uint32_t item = items[w];
+        const int32_t v = (int32_t)(item / (uint32_t)ngroups);
+        const int32_t g = (int32_t)(item - (uint32_t)v * (uint32_t)ngroups);
+        const unsigned long long m = cm[item];
+        if (lane == 0) cm[item] = 0ull;
+        const int32_t s = g * 64 + lane;
+        const bool active = (m >> lane) & 1ull;
+        const int32_t dv = deg[v];
+        T *Rv = R + (int64_t)v * s_pad + s;
+        T *Pv = P + (int64_t)v * s_pad + s;
+        T x = T(0);
+        bool pushing = false, again = false;
+        if (active) {
+            x = xchg(Rv, T(0));
+            if (!(x > rm)) {
+                const T old = atomicAdd(Rv, x);                  // not above rmax: give it back
+                again = old <= rm && old + x > rm && dv > 0;
+            } else if (dv == 0) {
+                if (!directed) *Pv = (T)(alpha * (double)x);     // ppr:1431-1441
+                atomicAdd(Rv, x);                                // residue kept, not re-queued
+            } else {
+                pushing = true;
+                *Pv = (T)((double)*Pv + alpha * (double)x);
+            }
+        }
+        const unsigned long long ca = __ballot(again);
+        if (ca && lane == 0) mark(ca, item, nm, next, next_count, cap, overflow);
+        const unsigned long long pm = __ballot(pushing);
+        if (pm != 0ull) {
+            if (lane == 0) {
+                atomicAdd(&sstat[0], (unsigned long long)__popcll(pm));
+                atomicAdd(&sstat[1], (unsigned long long)__popcll(pm) * (unsigned long long)dv);
+                atomicAdd(&sstat[2], 1ull);
+                atomicAdd(&sstat[3], (unsigned long long)dv);
+            }
+            const T delta = pushing ? (T)((1.0 - alpha) * (double)x / (double)dv) : T(0);
+            const int32_t *nb = adj + rowoff[v];
+            T *Rs = R + s;
+            const uint32_t G = (uint32_t)ngroups;
+            constexpr int U = 8;
+            int32_t j = 0;
+            for (; j + U <= dv; j += U) {
+                int32_t u[U];
+                bool c[U];
+#pragma unroll
+                for (int q = 0; q < U; q++) u[q] = nb[j + q];
+#pragma unroll
+                for (int q = 0; q < U; q++) {
+                    c[q] = false;
+                    if (pushing) {
+                        const T o = atomicAdd(Rs + (int64_t)u[q] * s_pad, delta);
+                        c[q] = o <= rm && o + delta > rm;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < U; q++) {
+                    const unsigned long long b = __ballot(c[q]);
+                    if (b && lane == 0 && deg[u[q]] > 0)
+                        mark(b, (uint32_t)u[q] * G + g, nm, next, next_count, cap, overflow);
+                }
+            }
+            for (; j < dv; j++) {
+                const int32_t u = nb[j];
+                bool c = false;
+                if (pushing) {
+                    const T o = atomicAdd(Rs + (int64_t)u * s_pad, delta);
+                    c = o <= rm && o + delta > rm;
+                }
+                const unsigned long long b = __ballot(c);
+                if (b && lane == 0 && deg[u] > 0) mark(b, (uint32_t)u * G + g, nm, next, next_count, cap, overflow);
+            }
         }
     }
-    const unsigned long long ca = __ballot(again);
-    if (ca && lane == 0) mark(ca, item, nm, next, next_count, cap, overflow);
-    const unsigned long long pm = __ballot(pushing);
-    if (pm == 0ull) return;
-    if (lane == 0) {
-        atomicAdd(stats + 0, (unsigned long long)__popcll(pm));
-        atomicAdd(stats + 1, (unsigned long long)__popcll(pm) * (unsigned long long)dv);
-    }
-    const T delta = pushing ? (T)((1.0 - alpha) * (double)x / (double)dv) : T(0);
-    const int32_t *nb = adj + rowoff[v];
-    T *Rs = R + s;
-    int32_t j = 0;
-    for (; j + 4 <= dv; j += 4) {
-        const int32_t u0 = nb[j], u1 = nb[j + 1], u2 = nb[j + 2], u3 = nb[j + 3];
-        bool c0 = false, c1 = false, c2 = false, c3 = false;
-        if (pushing) {
-            T o0 = atomicAdd(Rs + (int64_t)u0 * s_pad, delta);
-            T o1 = atomicAdd(Rs + (int64_t)u1 * s_pad, delta);
-            T o2 = atomicAdd(Rs + (int64_t)u2 * s_pad, delta);
-            T o3 = atomicAdd(Rs + (int64_t)u3 * s_pad, delta);
-            c0 = o0 <= rm && o0 + delta > rm;
-            c1 = o1 <= rm && o1 + delta > rm;
-            c2 = o2 <= rm && o2 + delta > rm;
-            c3 = o3 <= rm && o3 + delta > rm;
-        }
-        const unsigned long long b0 = __ballot(c0), b1 = __ballot(c1), b2 = __ballot(c2), b3 = __ballot(c3);
-        if (lane == 0) {
-            if (b0 && deg[u0] > 0) mark(b0, (uint32_t)u0 * ngroups + g, nm, next, next_count, cap, overflow);
-            if (b1 && deg[u1] > 0) mark(b1, (uint32_t)u1 * ngroups + g, nm, next, next_count, cap, overflow);
-            if (b2 && deg[u2] > 0) mark(b2, (uint32_t)u2 * ngroups + g, nm, next, next_count, cap, overflow);
-            if (b3 && deg[u3] > 0) mark(b3, (uint32_t)u3 * ngroups + g, nm, next, next_count, cap, overflow);
-        }
-    }
-    for (; j < dv; j++) {
-        const int32_t u = nb[j];
-        bool c = false;
-        if (pushing) {
-            T o = atomicAdd(Rs + (int64_t)u * s_pad, delta);
-            c = o <= rm && o + delta > rm;
-        }
-        const unsigned long long b = __ballot(c);
-        if (b && lane == 0 && deg[u] > 0) mark(b, (uint32_t)u * ngroups + g, nm, next, next_count, cap, overflow);
-    }
+    __syncthreads();
+    if (threadIdx.x < 4 && sstat[threadIdx.x]) atomicAdd(stats + threadIdx.x, sstat[threadIdx.x]);
 }
 
 template <class T>
@@ -248,6 +263,17 @@ void refresh_t(tsvd_ppr *s, int mode, int d, int32_t s0, int32_t s1)
         const GraphDir &D = g->dir[d];
         if (D.ntouched > 0) {
             const int64_t waves = (int64_t)D.ntouched * s->ngroups;
+            // algorithmic bytes per (new slot, source): p RMW, r[from] and r[to] RMW,
+            // and the matrix entry of the reference (8 B each in FP32 terms)
+            int64_t slots = 0;
+            if (g_profile) {
+                std::vector<int32_t> c((size_t)g->n);
+                TSVD_HIP(hipMemcpyAsync(c.data(), D.cnt.p, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, st));
+                TSVD_HIP(hipStreamSynchronize(st));
+                for (int32_t v = 0; v < g->n; v++) slots += c[v];
+                prof_add_work("ppr_update", 4.0 * sizeof(T) * 2.0 * (double)slots * (s1 - s0), 0.0);
+            }
+            PROF("ppr_update", st);
             hipLaunchKernelGGL(k_update<T>, dim3((unsigned)ceil_div(waves, WPB)), dim3(TPB), 0, st, P, R,
                                D.touched.p, D.ntouched, D.rowoff.p, D.adj.p, D.deg.p, D.former.p, s->s_pad,
                                s->ngroups, s0, s1, s->alpha, (T)s->rmax, s->cm[d].p, s->F[d].p, cnt + d, cap,
@@ -276,14 +302,30 @@ void push_t(tsvd_ppr *s, int d)
     Timer tm;
     tm.start(st);
     int64_t rounds = 0;
+    unsigned long long prof_prev[2] = {0, 0};
     while (count > 0) {
         TSVD_HIP(hipMemsetAsync(cnt + 2, 0, sizeof(uint32_t), st));
-        hipLaunchKernelGGL(k_push_round<T>, dim3((unsigned)ceil_div(count, WPB)), dim3(TPB), 0, st, P, R, s->F[d].p,
-                           count, s->cm[d].p, s->nm.p, s->fnext.p, cnt + 2, cap, cnt + 3, D.rowoff.p, D.adj.p,
-                           D.deg.p, s->s_pad, s->ngroups, s->alpha, (T)s->rmax, s->directed, s->stats.p);
+        {
+            PROF("ppr_push_round", st);
+            hipLaunchKernelGGL(k_push_round<T>, dim3((unsigned)ceil_div(count, WPB)), dim3(TPB), 0, st, P, R,
+                               s->F[d].p, count, s->cm[d].p, s->nm.p, s->fnext.p, cnt + 2, cap, cnt + 3, D.rowoff.p,
+                               D.adj.p, D.deg.p, s->s_pad, s->ngroups, s->alpha, (T)s->rmax, s->directed,
+                               s->stats.p);
+        }
         TSVD_HIP(hipGetLastError());
         TSVD_HIP(hipMemcpyAsync(h + 2, cnt + 2, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        if (g_profile) TSVD_HIP(hipMemcpyAsync(h + 4, s->stats.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
         TSVD_HIP(hipStreamSynchronize(st));
+        if (g_profile) {
+            // algorithmic bytes per relaxation (column id + residue RMW) and per pop
+            // (row offsets, residue read/clear, reserve RMW)
+            const unsigned long long *hs = (const unsigned long long *)(h + 4);
+            const double relax = (double)(hs[1] - prof_prev[1]), pops = (double)(hs[0] - prof_prev[0]);
+            const double per_relax = sizeof(T) == 8 ? 20.0 : 12.0, per_pop = sizeof(T) == 8 ? 48.0 : 32.0;
+            prof_add_work("ppr_push_round", per_relax * relax + per_pop * pops, 0.0);
+            prof_prev[0] = hs[0];
+            prof_prev[1] = hs[1];
+        }
         TSVD_REQUIRE(h[3] == 0, "frontier overflow in push round %lld", (long long)rounds);
         std::swap(s->cm[d], s->nm);
         std::swap(s->F[d], s->fnext);
@@ -292,12 +334,14 @@ void push_t(tsvd_ppr *s, int d)
     }
     tm.stop(st);
     TSVD_HIP(hipMemsetAsync(cnt + d, 0, sizeof(uint32_t), st));
-    unsigned long long hs[2] = {0, 0};
+    unsigned long long hs[4] = {0, 0, 0, 0};
     TSVD_HIP(hipMemcpyAsync(hs, s->stats.p, sizeof(hs), hipMemcpyDeviceToHost, st));
     TSVD_HIP(hipStreamSynchronize(st));
     s->last_rounds[d] = rounds;
     s->last_pops[d] = (int64_t)hs[0];
     s->last_relax[d] = (int64_t)hs[1];
+    s->last_items[d] = (int64_t)hs[2];
+    s->last_wave_relax[d] = (int64_t)hs[3];
     s->last_ms[d] = tm.ms();
 }
 

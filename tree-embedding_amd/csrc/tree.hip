@@ -80,7 +80,7 @@ int32_t tree_update(Tree &t, const std::vector<int32_t> &dirty_leaves, int trans
             p.M = nc; p.N = nc; p.K = t.nsrc; p.alpha = 1.0; p.beta = 0.0; p.kchunk = 512;
             gp.push_back(p);
         }
-        gemm.run(st, gp, GemmTypes{T, T, DT::F64, DT::F64}, true);
+        gemm.run(st, gp, GemmTypes{T, T, DT::F64, DT::F64}, true, "gemm_merge_gram");
         eig.solve(st, t.G.p, nc, cnt, t.lam.p);
         const bool root = lvl == top;
         const int mode = !root ? 3 : (transform == TSVD_TRANSFORM_NC ? 1 : 2);
@@ -98,7 +98,7 @@ int32_t tree_update(Tree &t, const std::vector<int32_t> &dirty_leaves, int trans
             p.M = t.nsrc; p.N = k; p.K = nc; p.alpha = 1.0; p.beta = 0.0;
             pp.push_back(p);
         }
-        gemm.run(st, pp, GemmTypes{T, DT::F64, root ? DT::F64 : T, DT::F64}, k > 80);
+        gemm.run(st, pp, GemmTypes{T, DT::F64, root ? DT::F64 : T, DT::F64}, k > 80, "gemm_merge");
         if (root) {
             std::vector<double> lam((size_t)nc);
             TSVD_HIP(hipMemcpyAsync(lam.data(), t.lam.p, sizeof(double) * nc, hipMemcpyDeviceToHost, st));

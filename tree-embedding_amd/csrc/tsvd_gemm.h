@@ -32,7 +32,7 @@ struct GemmTypes {
 
 class GemmRunner {
 public:
-    void run(hipStream_t st, std::vector<GemmProb> &probs, GemmTypes t, bool wide);
+    void run(hipStream_t st, std::vector<GemmProb> &probs, GemmTypes t, bool wide, const char *label = "gemm");
     int64_t launches = 0;
 
 private:

@@ -5,12 +5,14 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdarg>
+#include <cstring>
 #include <string>
 #include <vector>
 #include <new>
 
 #define TREESVD_BUILDING 1
 #include "../../include/treesvd.h"
+#include "tsvd_profile.h"
 
 namespace tsvd {
 

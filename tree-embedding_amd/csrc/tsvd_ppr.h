@@ -24,6 +24,7 @@ struct tsvd_ppr {
     tsvd::DBuf<unsigned long long> stats;    // [0] pops, [1] relaxations
     uint32_t *h_counters = nullptr;          // pinned
     int64_t last_rounds[2] = {0, 0}, last_pops[2] = {0, 0}, last_relax[2] = {0, 0};
+    int64_t last_items[2] = {0, 0}, last_wave_relax[2] = {0, 0};
     double last_ms[2] = {0, 0};
     std::vector<int32_t> h_sources;
     ~tsvd_ppr()

@@ -20,6 +20,7 @@ TSVD_TRANSFORM_NC, TSVD_TRANSFORM_LP = 0, 1
 # Every symbol include/treesvd.h declares.
 EXPORTS = [
     "tsvd_last_error", "tsvd_device_info", "tsvd_device_alloc", "tsvd_device_free", "tsvd_memcpy",
+    "tsvd_profile_enable", "tsvd_profile_reset", "tsvd_profile_read",
     "tsvd_graph_create", "tsvd_graph_destroy", "tsvd_graph_append", "tsvd_graph_num_edges",
     "tsvd_graph_degrees", "tsvd_graph_export_csr", "tsvd_read_edge_file",
     "tsvd_ppr_create", "tsvd_ppr_destroy", "tsvd_ppr_refresh", "tsvd_ppr_push", "tsvd_ppr_read",
@@ -57,7 +58,8 @@ class Config(ctypes.Structure):
 class SnapshotStats(ctypes.Structure):
     _fields_ = [("snapshot", ctypes.c_int32), ("ndirty", ctypes.c_int32), ("nmerged", ctypes.c_int32),
                 ("reused", ctypes.c_int32), ("push_rounds", ctypes.c_int64), ("push_pops", ctypes.c_int64),
-                ("push_relax", ctypes.c_int64), ("ms_total", ctypes.c_double), ("ms_ingest", ctypes.c_double),
+                ("push_relax", ctypes.c_int64), ("push_items", ctypes.c_int64),
+                ("push_wave_relax", ctypes.c_int64), ("ms_total", ctypes.c_double), ("ms_ingest", ctypes.c_double),
                 ("ms_refresh", ctypes.c_double), ("ms_push", ctypes.c_double), ("ms_blocks", ctypes.c_double),
                 ("ms_frpca", ctypes.c_double), ("ms_merge", ctypes.c_double)]
 
@@ -80,6 +82,8 @@ def lib():
         L.tsvd_device_alloc.argtypes = [i64, vp]
         L.tsvd_device_free.argtypes = [vp]
         L.tsvd_memcpy.argtypes = [vp, vp, i64, ctypes.c_int]
+        L.tsvd_profile_enable.argtypes = [ctypes.c_int]
+        L.tsvd_profile_read.argtypes = [ctypes.c_char_p, ctypes.c_int]
         L.tsvd_graph_create.argtypes = [i32, ctypes.c_int, f64, vp]
         L.tsvd_graph_destroy.argtypes = [vp]
         L.tsvd_graph_append.argtypes = [vp, vp, vp, i64, ctypes.c_int]
@@ -360,3 +364,19 @@ class Pipeline:
         out = np.zeros((self.rank, self.nsrc))
         check(lib().tsvd_pipeline_node(self.h, level, index, _p(out)))
         return out.T.copy()
+
+
+def profile_enable(on: bool = True):
+    check(lib().tsvd_profile_enable(int(on)))
+
+
+def profile_reset():
+    check(lib().tsvd_profile_reset())
+
+
+def profile_read():
+    """{kernel: (launches, total_ms, algorithmic_bytes, flops)} since the last reset."""
+    import json
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(lib().tsvd_profile_read(buf, 1 << 16))
+    return {k: (int(v[0]), float(v[1]), float(v[2]), float(v[3])) for k, v in json.loads(buf.value.decode()).items()}
