@@ -51,6 +51,7 @@ def lib():
         L.orc_ppr_init_static.argtypes = [vp, i32]
         L.orc_ppr_refresh_dynamic.argtypes = [vp, vp, i32, i32]
         L.orc_ppr_push.argtypes = [vp, vp, i32, i32]
+        L.orc_ppr_counts.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
 
@@ -137,6 +138,12 @@ class PPR:
     def push(self, g: Graph, nthreads=1):
         for d in self.dirs():
             lib().orc_ppr_push(self.h, g.h, d, nthreads)
+
+    def counts(self):
+        """(pushes, relaxations) since the last call."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        lib().orc_ppr_counts(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
 
     def p(self, d=0):
         return np.ctypeslib.as_array(lib().orc_ppr_p(self.h, d), shape=(self.nsrc, self.n))

@@ -177,6 +177,7 @@ typedef struct {
     unsigned char *flag[2];     /* [nsrc * n] */
     int *q[2];                  /* per-source ring, capacity n + 1 */
     long long *qh[2], *qt[2];
+    long long pops, relax;      /* pushes with r > rmax and their edge relaxations */
 } orc_ppr;
 
 #define ST(x) (s->fp32 ? (double)(float)(x) : (x))
@@ -315,6 +316,7 @@ static void *push_job(void *arg)
     orc_ppr *s = J->s; const orc_graph *g = J->g; int d = J->dir;
     double a = s->alpha, rmax = s->rmax;
     const int *deg = g->deg[d];
+    long long pops = 0, relax = 0;
     for (int k = J->k0; k < J->k1; k++) {
         double *p = s->p[d] + IDX(k, 0), *r = s->r[d] + IDX(k, 0), *M = s->M + IDX(k, 0);
         unsigned char *fl = s->flag[d] + IDX(k, 0);
@@ -329,6 +331,8 @@ static void *push_job(void *arg)
                 continue;
             }
             if (r[v] > rmax) {
+                pops++;
+                relax += deg[v];
                 const int *nb = g->adj[d][v];
                 for (int j = 0; j < deg[v]; j++) {
                     int u = nb[j];
@@ -350,7 +354,18 @@ static void *push_job(void *arg)
         }
         s->qh[d][k] = s->qt[d][k] = 0;
     }
+    __sync_fetch_and_add(&s->pops, pops);
+    __sync_fetch_and_add(&s->relax, relax);
     return NULL;
+}
+
+/* Pushes (vertices popped with r > rmax) and edge relaxations since the last
+ * call; the counters are reset by the read. */
+ORC_EXPORT void orc_ppr_counts(orc_ppr *s, long long *pops, long long *relax)
+{
+    *pops = s->pops;
+    *relax = s->relax;
+    s->pops = s->relax = 0;
 }
 
 ORC_EXPORT void orc_ppr_push(orc_ppr *s, const orc_graph *g, int dir, int nthreads)

@@ -19,6 +19,18 @@ pytestmark = pytest.mark.gpu
 ALPHA = 0.2
 
 
+@pytest.fixture(params=["hybrid", "push", "pull"], autouse=True)
+def round_mode(request, monkeypatch):
+    """Every push test runs with the library's hybrid schedule, with push
+    rounds only, and with pull rounds only (TSVD_PULL_FRAC)."""
+    frac = {"hybrid": None, "push": "0", "pull": "1e-9"}[request.param]
+    if frac is None:
+        monkeypatch.delenv("TSVD_PULL_FRAC", raising=False)
+    else:
+        monkeypatch.setenv("TSVD_PULL_FRAC", frac)
+    return request.param
+
+
 def pick_sources(deg, k, seed):
     rng = np.random.Generator(np.random.PCG64(seed))
     return rng.choice(np.flatnonzero(deg > 0), size=k, replace=False).astype(np.int32)

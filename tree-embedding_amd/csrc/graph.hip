@@ -290,6 +290,7 @@ void graph_append(tsvd_graph *g, const int32_t *src, const int32_t *dst, int64_t
         for (int d = 0; d < g->ndir; d++) append_dir(g, d, dsrc, ddst, m);
     }
     g->m += m;
+    g->version++;
     TSVD_HIP(hipStreamSynchronize(st));
 }
 

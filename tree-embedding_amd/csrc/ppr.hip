@@ -32,6 +32,9 @@
 // reserves agree within the push error bound, not bitwise.
 #include "tsvd_ppr.h"
 
+#include <hipcub/hipcub.hpp>
+#include <algorithm>
+
 namespace tsvd {
 
 namespace {
@@ -238,6 +241,155 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
     if (threadIdx.x < 4 && sstat[threadIdx.x]) atomicAdd(stats + threadIdx.x, sstat[threadIdx.x]);
 }
 
+
+// ---------------------------------------------------------------- pull rounds
+// When the frontier is large, a round runs as pop -> pull: the pop publishes
+// each popped lane's pushed amount (1-alpha) r / deg in delta[v] and flags the
+// group in pm[v]; every vertex u then sums delta[v][g] over its pull
+// neighbours (in-neighbours of the push direction) for each flagged group,
+// accumulating in LDS without atomics, and adds the sums to r[u] once.  Hub
+// rows are split into chunks of PULL_CHUNK neighbours; a chunk of a split row
+// adds with atomics (crossing detected from the returned value), an unsplit
+// row is owned by one wave (plain read-modify-write, absolute test).
+constexpr int PULL_CHUNK = 2048;
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_pull_pop(T *__restrict__ P, T *__restrict__ R, T *__restrict__ D,
+                                                  unsigned long long *__restrict__ pm,
+                                                  const uint32_t *__restrict__ items, uint32_t nitems,
+                                                  unsigned long long *__restrict__ cm,
+                                                  const int32_t *__restrict__ deg, int32_t s_pad, int32_t ngroups,
+                                                  double alpha, T rm, int directed,
+                                                  unsigned long long *__restrict__ stats)
+{
+    __shared__ unsigned long long sstat[4];
+    if (threadIdx.x < 4) sstat[threadIdx.x] = 0ull;
+    __syncthreads();
+    const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w < nitems) {
+        const uint32_t item = items[w];
+        const int32_t v = (int32_t)(item / (uint32_t)ngroups);
+        const int32_t g = (int32_t)(item - (uint32_t)v * (uint32_t)ngroups);
+        const unsigned long long m = cm[item];
+        if (lane == 0) cm[item] = 0ull;
+        const int64_t at = (int64_t)v * s_pad + g * 64 + lane;
+        const int32_t dv = deg[v];
+        T d = T(0);
+        bool pushing = false;
+        if ((m >> lane) & 1ull) {
+            const T x = R[at];
+            if (x > rm) {
+                if (dv == 0) {
+                    if (!directed) P[at] = (T)(alpha * (double)x);   // ppr:1431-1441, residue kept
+                } else {
+                    pushing = true;
+                    R[at] = T(0);
+                    P[at] = (T)((double)P[at] + alpha * (double)x);
+                    d = (T)((1.0 - alpha) * (double)x / (double)dv);
+                }
+            }
+        }
+        D[at] = d;
+        const unsigned long long pmask = __ballot(pushing);
+        if (pmask && lane == 0) {
+            atomicOr(pm + v, 1ull << g);
+            atomicAdd(&sstat[0], (unsigned long long)__popcll(pmask));
+            atomicAdd(&sstat[1], (unsigned long long)__popcll(pmask) * (unsigned long long)dv);
+            atomicAdd(&sstat[2], 1ull);
+            atomicAdd(&sstat[3], (unsigned long long)dv);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && sstat[threadIdx.x]) atomicAdd(stats + threadIdx.x, sstat[threadIdx.x]);
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_pull(T *__restrict__ R, const T *__restrict__ D,
+                                              const unsigned long long *__restrict__ pm,
+                                              const uint32_t *__restrict__ wl_u, const uint32_t *__restrict__ wl_c,
+                                              uint32_t nwork, const int64_t *__restrict__ rowoff,
+                                              const int32_t *__restrict__ adj, const int32_t *__restrict__ deg_pull,
+                                              const int32_t *__restrict__ deg_push, int32_t s_pad, int32_t ngroups,
+                                              T rm, unsigned long long *__restrict__ nm, uint32_t *__restrict__ next,
+                                              uint32_t *__restrict__ next_count, uint32_t cap,
+                                              uint32_t *__restrict__ overflow)
+{
+    extern __shared__ unsigned char smem[];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    T *acc = (T *)smem + (size_t)wv * ngroups * 64;
+    const uint32_t w = blockIdx.x * WPB + wv;
+    if (w >= nwork) return;
+    const int32_t u = (int32_t)wl_u[w];
+    const int32_t c = (int32_t)wl_c[w];
+    const int32_t du = deg_pull[u];
+    const int32_t j0 = c * PULL_CHUNK, j1 = min(du, j0 + PULL_CHUNK);
+    const bool split = du > PULL_CHUNK;
+    for (int g = 0; g < ngroups; g++) acc[g * 64 + lane] = T(0);
+    const int32_t *nb = adj + rowoff[u];
+    unsigned long long uni = 0ull;
+    int32_t j = j0;
+    for (; j + 4 <= j1; j += 4) {
+        const int32_t v0 = nb[j], v1 = nb[j + 1], v2 = nb[j + 2], v3 = nb[j + 3];
+        unsigned long long m0 = pm[v0], m1 = pm[v1], m2 = pm[v2], m3 = pm[v3];
+        uni |= m0 | m1 | m2 | m3;
+        while (m0) { const int g = __ffsll((long long)m0) - 1; m0 &= m0 - 1ull; acc[g * 64 + lane] += D[(int64_t)v0 * s_pad + g * 64 + lane]; }
+        while (m1) { const int g = __ffsll((long long)m1) - 1; m1 &= m1 - 1ull; acc[g * 64 + lane] += D[(int64_t)v1 * s_pad + g * 64 + lane]; }
+        while (m2) { const int g = __ffsll((long long)m2) - 1; m2 &= m2 - 1ull; acc[g * 64 + lane] += D[(int64_t)v2 * s_pad + g * 64 + lane]; }
+        while (m3) { const int g = __ffsll((long long)m3) - 1; m3 &= m3 - 1ull; acc[g * 64 + lane] += D[(int64_t)v3 * s_pad + g * 64 + lane]; }
+    }
+    for (; j < j1; j++) {
+        const int32_t v = nb[j];
+        unsigned long long m = pm[v];
+        uni |= m;
+        while (m) { const int g = __ffsll((long long)m) - 1; m &= m - 1ull; acc[g * 64 + lane] += D[(int64_t)v * s_pad + g * 64 + lane]; }
+    }
+    const bool can_push = deg_push[u] > 0;
+    while (uni) {
+        const int g = __ffsll((long long)uni) - 1;
+        uni &= uni - 1ull;
+        const T a = acc[g * 64 + lane];
+        T *r = R + (int64_t)u * s_pad + g * 64 + lane;
+        bool cross;
+        if (split) {
+            const T old = atomicAdd(r, a);
+            cross = old <= rm && old + a > rm;
+        } else {
+            const T nv = *r + a;
+            *r = nv;
+            cross = nv > rm;
+        }
+        const unsigned long long b = __ballot(cross);
+        if (b && lane == 0 && can_push) mark(b, (uint32_t)u * (uint32_t)ngroups + g, nm, next, next_count, cap, overflow);
+    }
+}
+
+__global__ void k_pull_clear(unsigned long long *__restrict__ pm, const uint32_t *__restrict__ items, uint32_t nitems,
+                             int32_t ngroups)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nitems) pm[items[t] / (uint32_t)ngroups] = 0ull;
+}
+
+__global__ void k_chunk_count(const int32_t *__restrict__ deg, int32_t n, int32_t *__restrict__ cnt)
+{
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) cnt[v] = (deg[v] + PULL_CHUNK - 1) / PULL_CHUNK;
+    if (v == n) cnt[v] = 0;
+}
+
+__global__ void k_chunk_fill(const int32_t *__restrict__ cnt, const int32_t *__restrict__ off, int32_t n,
+                             uint32_t *__restrict__ wl_u, uint32_t *__restrict__ wl_c)
+{
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int32_t o = off[v];
+    for (int32_t c = 0; c < cnt[v]; c++) {
+        wl_u[o + c] = (uint32_t)v;
+        wl_c[o + c] = (uint32_t)c;
+    }
+}
+
 template <class T>
 void refresh_t(tsvd_ppr *s, int mode, int d, int32_t s0, int32_t s1)
 {
@@ -284,6 +436,42 @@ void refresh_t(tsvd_ppr *s, int mode, int d, int32_t s0, int32_t s1)
     TSVD_HIP(hipStreamSynchronize(st));
 }
 
+
+// The pull adjacency of push direction d: in-edges for the forward push,
+// out-edges for the transpose push, the adjacency itself when undirected.
+static const GraphDir &pull_dir(tsvd_ppr *s, int d)
+{
+    return s->g->dir[s->directed ? 1 - d : 0];
+}
+
+static void build_worklist(tsvd_ppr *s, int d)
+{
+    tsvd_graph *g = s->g;
+    if (s->wl_version[d] == g->version) return;
+    hipStream_t st = g->stream;
+    const GraphDir &Q = pull_dir(s, d);
+    const int32_t n = g->n;
+    DBuf<int32_t> off((size_t)n + 1);
+    s->wl_cnt.ensure((size_t)n + 1);
+    hipLaunchKernelGGL(k_chunk_count, dim3((unsigned)ceil_div(n + 1, TPB)), dim3(TPB), 0, st, Q.deg.p, n,
+                       s->wl_cnt.p);
+    size_t tmp = 0;
+    TSVD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, s->wl_cnt.p, off.p, n + 1, st));
+    DBuf<uint8_t> t(tmp + 16);
+    TSVD_HIP(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, s->wl_cnt.p, off.p, n + 1, st));
+    int32_t total = 0;
+    TSVD_HIP(hipMemcpyAsync(&total, off.p + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    TSVD_HIP(hipStreamSynchronize(st));
+    s->wl_u[d].ensure((size_t)std::max(total, 1));
+    s->wl_c[d].ensure((size_t)std::max(total, 1));
+    hipLaunchKernelGGL(k_chunk_fill, dim3((unsigned)ceil_div(n, TPB)), dim3(TPB), 0, st, s->wl_cnt.p, off.p, n,
+                       s->wl_u[d].p, s->wl_c[d].p);
+    TSVD_HIP(hipGetLastError());
+    TSVD_HIP(hipStreamSynchronize(st));
+    s->wl_n[d] = total;
+    s->wl_version[d] = g->version;
+}
+
 template <class T>
 void push_t(tsvd_ppr *s, int d)
 {
@@ -299,13 +487,35 @@ void push_t(tsvd_ppr *s, int d)
     TSVD_HIP(hipStreamSynchronize(st));
     TSVD_REQUIRE(h[3] == 0, "frontier overflow");
     uint32_t count = h[d];
+    s->last_pull_rounds[d] = 0;
     Timer tm;
     tm.start(st);
     int64_t rounds = 0;
     unsigned long long prof_prev[2] = {0, 0};
     while (count > 0) {
         TSVD_HIP(hipMemsetAsync(cnt + 2, 0, sizeof(uint32_t), st));
-        {
+        const bool pull = s->ngroups <= 64 && s->pull_frac > 0.0 && (double)count > s->pull_frac * (double)s->n;
+        if (pull) {
+            build_worklist(s, d);
+            const GraphDir &Q = pull_dir(s, d);
+            {
+                PROF("ppr_pull_pop", st);
+                hipLaunchKernelGGL(k_pull_pop<T>, dim3((unsigned)ceil_div(count, WPB)), dim3(TPB), 0, st, P, R,
+                                   (T *)s->delta.p, s->pm.p, s->F[d].p, count, s->cm[d].p, D.deg.p, s->s_pad,
+                                   s->ngroups, s->alpha, (T)s->rmax, s->directed, s->stats.p);
+            }
+            {
+                PROF("ppr_pull", st);
+                const size_t shm = (size_t)WPB * s->ngroups * 64 * sizeof(T);
+                hipLaunchKernelGGL(k_pull<T>, dim3((unsigned)ceil_div(s->wl_n[d], WPB)), dim3(TPB), shm, st, R,
+                                   (const T *)s->delta.p, s->pm.p, s->wl_u[d].p, s->wl_c[d].p,
+                                   (uint32_t)s->wl_n[d], Q.rowoff.p, Q.adj.p, Q.deg.p, D.deg.p, s->s_pad,
+                                   s->ngroups, (T)s->rmax, s->nm.p, s->fnext.p, cnt + 2, cap, cnt + 3);
+            }
+            hipLaunchKernelGGL(k_pull_clear, dim3((unsigned)ceil_div(count, TPB)), dim3(TPB), 0, st, s->pm.p,
+                               s->F[d].p, count, s->ngroups);
+            s->last_pull_rounds[d]++;
+        } else {
             PROF("ppr_push_round", st);
             hipLaunchKernelGGL(k_push_round<T>, dim3((unsigned)ceil_div(count, WPB)), dim3(TPB), 0, st, P, R,
                                s->F[d].p, count, s->cm[d].p, s->nm.p, s->fnext.p, cnt + 2, cap, cnt + 3, D.rowoff.p,
@@ -322,7 +532,7 @@ void push_t(tsvd_ppr *s, int d)
             const unsigned long long *hs = (const unsigned long long *)(h + 4);
             const double relax = (double)(hs[1] - prof_prev[1]), pops = (double)(hs[0] - prof_prev[0]);
             const double per_relax = sizeof(T) == 8 ? 20.0 : 12.0, per_pop = sizeof(T) == 8 ? 48.0 : 32.0;
-            prof_add_work("ppr_push_round", per_relax * relax + per_pop * pops, 0.0);
+            prof_add_work(pull ? "ppr_pull" : "ppr_push_round", per_relax * relax + per_pop * pops, 0.0);
             prof_prev[0] = hs[0];
             prof_prev[1] = hs[1];
         }
@@ -398,6 +608,8 @@ extern "C" int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t ns
             s->rmax = rmax;
             s->precision = precision;
             s->esize = precision == TSVD_FP64 ? 8 : 4;
+            // rounds with more than pull_frac * n items run as pull rounds (0: never)
+            if (const char *e = getenv("TSVD_PULL_FRAC")) s->pull_frac = atof(e);
             s->h_sources.assign(sources, sources + nsrc);
             s->sources.alloc((size_t)nsrc);
             hipStream_t st = g->stream;
@@ -415,6 +627,11 @@ extern "C" int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t ns
             }
             s->nm.alloc(items);
             s->fnext.alloc(items);
+            if (s->ngroups <= 64) {
+                s->delta.alloc(cells * s->esize);
+                s->pm.alloc((size_t)g->n);
+                TSVD_HIP(hipMemsetAsync(s->pm.p, 0, s->pm.bytes(), st));
+            }
             TSVD_HIP(hipMemsetAsync(s->nm.p, 0, s->nm.bytes(), st));
             s->counters.alloc(8);
             s->stats.alloc(4);

@@ -23,6 +23,7 @@ struct tsvd_graph {
     double slack = 0.1;
     int64_t m = 0;
     int64_t rebuilds = 0;
+    int64_t version = 0;      // bumped by every append
     hipStream_t stream = nullptr;
     tsvd::GraphDir dir[2];
     tsvd::DBuf<int32_t> e_src, e_dst, k_in, v_in, k_out, v_out, scan_flag, scan_pos, segstart;

@@ -26,6 +26,15 @@ struct tsvd_ppr {
     int64_t last_rounds[2] = {0, 0}, last_pops[2] = {0, 0}, last_relax[2] = {0, 0};
     int64_t last_items[2] = {0, 0}, last_wave_relax[2] = {0, 0};
     double last_ms[2] = {0, 0};
+    int64_t last_pull_rounds[2] = {0, 0};
+    // pull rounds: per-vertex pushed amounts, popped-group masks, and the
+    // chunked work list over the pull adjacency (rebuilt when the graph changes)
+    tsvd::DBuf<uint8_t> delta;               // n * s_pad * esize
+    tsvd::DBuf<unsigned long long> pm;       // n
+    tsvd::DBuf<uint32_t> wl_u[2], wl_c[2];   // (vertex, chunk) per push direction
+    tsvd::DBuf<int32_t> wl_cnt;              // n + 1 scratch
+    int64_t wl_n[2] = {0, 0}, wl_version[2] = {-1, -1};
+    double pull_frac = 0.0;                  // pull when items > pull_frac * n (0: never)
     std::vector<int32_t> h_sources;
     ~tsvd_ppr()
     {
