@@ -128,7 +128,8 @@ def test_gpu_push_fp32_state(capi):
 
 
 def test_gpu_push_source_range(capi):
-    """Refreshing a sub-range of sources leaves the others untouched."""
+    """Refreshing and pushing a sub-range of sources [start, end), as the
+    reference's per-thread calls do, leaves the other sources untouched."""
     n = 500
     u, v = random_graph(n, 3000, 17, False)
     g = capi.Graph(n, False); g.append(u, v)
@@ -136,10 +137,10 @@ def test_gpu_push_source_range(capi):
     src = pick_sources(deg, 100, 17)
     ppr = capi.PPR(g, src, ALPHA, 1e-5, precision=capi.TSVD_FP64)
     ppr.refresh(capi.TSVD_REFRESH_INIT, 0, 0, 40)
-    ppr.push(0)
+    ppr.push(0, 0, 40)
     p = ppr.p()
     assert (p[:40].sum(1) > 0).all() and (p[40:] == 0).all()
     ppr.refresh(capi.TSVD_REFRESH_INIT, 0, 40, 100)
-    ppr.push(0)
+    ppr.push(0, 40, 100)
     p2 = ppr.p()
     assert np.array_equal(p2[:40], p[:40]) and (p2[40:].sum(1) > 0).all()

@@ -107,7 +107,7 @@ extern "C" int tsvd_pipeline_snapshot(tsvd_pipeline *p, const int32_t *src, cons
         const double t2 = now_ms();
         S.ms_refresh = t2 - t1;
         for (int d = 0; d < p->ppr->ndir; d++) {
-            ppr_push(p->ppr, d);
+            ppr_push(p->ppr, d, 0, c.nsrc);
             S.push_rounds += p->ppr->last_rounds[d];
             S.push_pops += p->ppr->last_pops[d];
             S.push_relax += p->ppr->last_relax[d];

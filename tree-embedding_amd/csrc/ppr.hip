@@ -140,6 +140,55 @@ __global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict
     if (c2 && lane == 0) mark(c2, (uint32_t)i * ngroups + g, cm, list, count, cap, overflow);
 }
 
+// Relax neighbours nb[j0, j1) of a popped (v, g) item: one coalesced 64-lane
+// atomic add per neighbour; a lane whose residue crosses rmax upward is
+// flagged, and the wave's ballot is OR-ed into the next round's mask of (u, g).
+template <class T>
+__device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, int32_t j1, bool pushing, T delta,
+                                   T *__restrict__ Rs, int32_t s_pad, const int32_t *__restrict__ deg, uint32_t G,
+                                   int32_t g, T rm, int lane, unsigned long long *__restrict__ nm,
+                                   uint32_t *__restrict__ next, uint32_t *__restrict__ next_count, uint32_t cap,
+                                   uint32_t *__restrict__ overflow)
+{
+    constexpr int U = 8;
+    int32_t j = j0;
+    for (; j + U <= j1; j += U) {
+        int32_t u[U];
+        bool c[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) u[q] = nb[j + q];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            c[q] = false;
+            if (pushing) {
+                const T o = atomicAdd(Rs + (int64_t)u[q] * s_pad, delta);
+                c[q] = o <= rm && o + delta > rm;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const unsigned long long b = __ballot(c[q]);
+            if (b && lane == 0 && deg[u[q]] > 0) mark(b, (uint32_t)u[q] * G + g, nm, next, next_count, cap, overflow);
+        }
+    }
+    for (; j < j1; j++) {
+        const int32_t u = nb[j];
+        bool c = false;
+        if (pushing) {
+            const T o = atomicAdd(Rs + (int64_t)u * s_pad, delta);
+            c = o <= rm && o + delta > rm;
+        }
+        const unsigned long long b = __ballot(c);
+        if (b && lane == 0 && deg[u] > 0) mark(b, (uint32_t)u * G + g, nm, next, next_count, cap, overflow);
+    }
+}
+
+// Rows longer than HUB_CHUNK neighbours are split: the popping wave relaxes
+// the first chunk and hands the rest, with its 64 pushed amounts, to
+// k_push_hub (one wave per further chunk), so a hub does not serialise a
+// round behind one wavefront.
+constexpr int HUB_CHUNK = 1024;
+
 // One synchronous push round: one wave per pending item pops its lanes and
 // relaxes v's neighbours.  The atomic add returns the previous residue, so a
 // lane whose residue crosses rmax upward is detected where it happens; the
@@ -156,7 +205,12 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                                                     const int64_t *__restrict__ rowoff,
                                                     const int32_t *__restrict__ adj, const int32_t *__restrict__ deg,
                                                     int32_t s_pad, int32_t ngroups, double alpha, T rm, int directed,
-                                                    unsigned long long *__restrict__ stats)
+                                                    T ride_min, int32_t ride_s0, int32_t ride_s1,
+                                                    unsigned long long *__restrict__ stats,
+                                                    uint32_t *__restrict__ hub_item, T *__restrict__ hub_delta,
+                                                    unsigned long long *__restrict__ hub_chunks,
+                                                    uint32_t *__restrict__ hub_counts, uint32_t hub_cap,
+                                                    uint32_t chunk_cap)
 {
     __shared__ unsigned long long sstat[4];
     if (threadIdx.x < 4) sstat[threadIdx.x] = 0ull;
@@ -191,7 +245,22 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
         }
         const unsigned long long ca = __ballot(again);
         if (ca && lane == 0) mark(ca, item, nm, next, next_count, cap, overflow);
-        const unsigned long long pm = __ballot(pushing);
+        unsigned long long pm = __ballot(pushing);
+        if (pm != 0ull && ride_min < rm && dv > 0) {
+            // The row is walked anyway: lanes whose residue is positive but at or
+            // below rmax ride along (same atomics, no extra traffic).  They only
+            // lower residues further, so the invariant and r <= rmax still hold.
+            if (!active && s >= ride_s0 && s < ride_s1 && *Rv > ride_min) {
+                x = xchg(Rv, T(0));
+                if (x > T(0)) {
+                    pushing = true;
+                    *Pv = (T)((double)*Pv + alpha * (double)x);
+                } else {
+                    atomicAdd(Rv, x);
+                }
+            }
+            pm = __ballot(pushing);
+        }
         if (pm != 0ull) {
             if (lane == 0) {
                 atomicAdd(&sstat[0], (unsigned long long)__popcll(pm));
@@ -201,46 +270,67 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
             }
             const T delta = pushing ? (T)((1.0 - alpha) * (double)x / (double)dv) : T(0);
             const int32_t *nb = adj + rowoff[v];
-            T *Rs = R + s;
-            const uint32_t G = (uint32_t)ngroups;
-            constexpr int U = 8;
-            int32_t j = 0;
-            for (; j + U <= dv; j += U) {
-                int32_t u[U];
-                bool c[U];
-#pragma unroll
-                for (int q = 0; q < U; q++) u[q] = nb[j + q];
-#pragma unroll
-                for (int q = 0; q < U; q++) {
-                    c[q] = false;
-                    if (pushing) {
-                        const T o = atomicAdd(Rs + (int64_t)u[q] * s_pad, delta);
-                        c[q] = o <= rm && o + delta > rm;
+            int32_t jend = dv;
+            if (dv > HUB_CHUNK) {
+                const uint32_t more = (uint32_t)((dv - 1) / HUB_CHUNK);   // chunks after the first
+                uint32_t slot = 0xFFFFFFFFu, cbase = 0;
+                if (lane == 0) {
+                    slot = atomicAdd(hub_counts + 0, 1u);
+                    if (slot < hub_cap) {
+                        cbase = atomicAdd(hub_counts + 1, more);
+                        if (cbase + more > chunk_cap) {
+                            atomicOr(overflow, 2u);
+                            slot = 0xFFFFFFFFu;
+                        }
+                    } else {
+                        slot = 0xFFFFFFFFu;
                     }
                 }
-#pragma unroll
-                for (int q = 0; q < U; q++) {
-                    const unsigned long long b = __ballot(c[q]);
-                    if (b && lane == 0 && deg[u[q]] > 0)
-                        mark(b, (uint32_t)u[q] * G + g, nm, next, next_count, cap, overflow);
+                slot = __shfl(slot, 0, 64);
+                cbase = __shfl(cbase, 0, 64);
+                if (slot != 0xFFFFFFFFu) {
+                    if (lane == 0) hub_item[slot] = item;
+                    hub_delta[(int64_t)slot * 64 + lane] = delta;
+                    for (uint32_t c = lane; c < more; c += 64)
+                        hub_chunks[cbase + c] = ((unsigned long long)slot << 32) | (unsigned long long)(c + 1);
+                    jend = HUB_CHUNK;
                 }
             }
-            for (; j < dv; j++) {
-                const int32_t u = nb[j];
-                bool c = false;
-                if (pushing) {
-                    const T o = atomicAdd(Rs + (int64_t)u * s_pad, delta);
-                    c = o <= rm && o + delta > rm;
-                }
-                const unsigned long long b = __ballot(c);
-                if (b && lane == 0 && deg[u] > 0) mark(b, (uint32_t)u * G + g, nm, next, next_count, cap, overflow);
-            }
+            relax_range(nb, 0, jend, pushing, delta, R + s, s_pad, deg, (uint32_t)ngroups, g, rm, lane, nm, next,
+                        next_count, cap, overflow);
         }
     }
     __syncthreads();
     if (threadIdx.x < 4 && sstat[threadIdx.x]) atomicAdd(stats + threadIdx.x, sstat[threadIdx.x]);
 }
 
+
+// Further chunks of split rows (see HUB_CHUNK): one wave per chunk.
+template <class T>
+__global__ void __launch_bounds__(TPB) k_push_hub(T *__restrict__ R, const uint32_t *__restrict__ hub_item,
+                                                  const T *__restrict__ hub_delta,
+                                                  const unsigned long long *__restrict__ hub_chunks, uint32_t nchunks,
+                                                  unsigned long long *__restrict__ nm, uint32_t *__restrict__ next,
+                                                  uint32_t *__restrict__ next_count, uint32_t cap,
+                                                  uint32_t *__restrict__ overflow,
+                                                  const int64_t *__restrict__ rowoff,
+                                                  const int32_t *__restrict__ adj, const int32_t *__restrict__ deg,
+                                                  int32_t s_pad, int32_t ngroups, T rm)
+{
+    const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= nchunks) return;
+    const unsigned long long e = hub_chunks[w];
+    const uint32_t slot = (uint32_t)(e >> 32), c = (uint32_t)(e & 0xFFFFFFFFull);
+    const uint32_t item = hub_item[slot];
+    const int32_t v = (int32_t)(item / (uint32_t)ngroups);
+    const int32_t g = (int32_t)(item - (uint32_t)v * (uint32_t)ngroups);
+    const T delta = hub_delta[(int64_t)slot * 64 + lane];
+    const int32_t dv = deg[v];
+    const int32_t j0 = (int32_t)c * HUB_CHUNK, j1 = min(dv, j0 + HUB_CHUNK);
+    relax_range(adj + rowoff[v], j0, j1, delta > T(0), delta, R + (int64_t)g * 64 + lane, s_pad, deg,
+                (uint32_t)ngroups, g, rm, lane, nm, next, next_count, cap, overflow);
+}
 
 // ---------------------------------------------------------------- pull rounds
 // When the frontier is large, a round runs as pop -> pull: the pop publishes
@@ -473,7 +563,7 @@ static void build_worklist(tsvd_ppr *s, int d)
 }
 
 template <class T>
-void push_t(tsvd_ppr *s, int d)
+void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
 {
     tsvd_graph *g = s->g;
     hipStream_t st = g->stream;
@@ -516,11 +606,33 @@ void push_t(tsvd_ppr *s, int d)
                                s->F[d].p, count, s->ngroups);
             s->last_pull_rounds[d]++;
         } else {
-            PROF("ppr_push_round", st);
-            hipLaunchKernelGGL(k_push_round<T>, dim3((unsigned)ceil_div(count, WPB)), dim3(TPB), 0, st, P, R,
-                               s->F[d].p, count, s->cm[d].p, s->nm.p, s->fnext.p, cnt + 2, cap, cnt + 3, D.rowoff.p,
-                               D.adj.p, D.deg.p, s->s_pad, s->ngroups, s->alpha, (T)s->rmax, s->directed,
-                               s->stats.p);
+            // split rows: at most one chunk list entry per (hub row, group) per HUB_CHUNK neighbours
+            const uint32_t hub_cap = (uint32_t)std::min<int64_t>((int64_t)cap, (int64_t)s->ngroups *
+                                                                 (ceil_div(D.cap_total, HUB_CHUNK) + 1));
+            const uint32_t chunk_cap = hub_cap;
+            s->hub_item.ensure(std::max<uint32_t>(hub_cap, 1));
+            s->hub_delta.ensure((size_t)std::max<uint32_t>(hub_cap, 1) * 64 * sizeof(T));
+            s->hub_chunks.ensure(std::max<uint32_t>(chunk_cap, 1));
+            TSVD_HIP(hipMemsetAsync(cnt + 4, 0, 2 * sizeof(uint32_t), st));
+            {
+                PROF("ppr_push_round", st);
+                hipLaunchKernelGGL(k_push_round<T>, dim3((unsigned)ceil_div(count, WPB)), dim3(TPB), 0, st, P, R,
+                                   s->F[d].p, count, s->cm[d].p, s->nm.p, s->fnext.p, cnt + 2, cap, cnt + 3,
+                                   D.rowoff.p, D.adj.p, D.deg.p, s->s_pad, s->ngroups, s->alpha, (T)s->rmax,
+                                   s->directed, (T)(std::min(s->ride_frac, 1.0) * s->rmax), s0, s1, s->stats.p,
+                                   s->hub_item.p, (T *)s->hub_delta.p, s->hub_chunks.p,
+                                   cnt + 4, hub_cap, chunk_cap);
+            }
+            TSVD_HIP(hipMemcpyAsync(h + 8, cnt + 4, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            TSVD_HIP(hipStreamSynchronize(st));
+            const uint32_t nchunks = std::min(h[9], chunk_cap);
+            if (nchunks > 0) {
+                PROF("ppr_push_hub", st);
+                hipLaunchKernelGGL(k_push_hub<T>, dim3((unsigned)ceil_div(nchunks, WPB)), dim3(TPB), 0, st, R,
+                                   s->hub_item.p, (const T *)s->hub_delta.p, s->hub_chunks.p, nchunks, s->nm.p,
+                                   s->fnext.p, cnt + 2, cap, cnt + 3, D.rowoff.p, D.adj.p, D.deg.p, s->s_pad,
+                                   s->ngroups, (T)s->rmax);
+            }
         }
         TSVD_HIP(hipGetLastError());
         TSVD_HIP(hipMemcpyAsync(h + 2, cnt + 2, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -567,11 +679,12 @@ void ppr_refresh(tsvd_ppr *s, int mode, int dir, int32_t s0, int32_t s1)
     else refresh_t<float>(s, mode, dir, s0, s1);
 }
 
-void ppr_push(tsvd_ppr *s, int dir)
+void ppr_push(tsvd_ppr *s, int dir, int32_t s0, int32_t s1)
 {
     TSVD_REQUIRE(dir >= 0 && dir < s->ndir, "bad direction");
-    if (s->precision == TSVD_FP64) push_t<double>(s, dir);
-    else push_t<float>(s, dir);
+    TSVD_REQUIRE(s0 >= 0 && s0 <= s1 && s1 <= s->nsrc, "bad source range");
+    if (s->precision == TSVD_FP64) push_t<double>(s, dir, s0, s1);
+    else push_t<float>(s, dir, s0, s1);
 }
 
 }  // namespace tsvd
@@ -610,6 +723,8 @@ extern "C" int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t ns
             s->esize = precision == TSVD_FP64 ? 8 : 4;
             // rounds with more than pull_frac * n items run as pull rounds (0: never)
             if (const char *e = getenv("TSVD_PULL_FRAC")) s->pull_frac = atof(e);
+            // lanes above ride_frac * rmax ride along with a pushing item (>= 1: off)
+            if (const char *e = getenv("TSVD_RIDE_FRAC")) s->ride_frac = atof(e);
             s->h_sources.assign(sources, sources + nsrc);
             s->sources.alloc((size_t)nsrc);
             hipStream_t st = g->stream;
@@ -663,8 +778,7 @@ extern "C" int tsvd_ppr_push(tsvd_ppr *s, int dir, int32_t src_begin, int32_t sr
 {
     return guard([&] {
         TSVD_REQUIRE(s, "null ppr");
-        TSVD_REQUIRE(src_begin >= 0 && src_begin <= src_end && src_end <= s->nsrc, "bad source range");
-        ppr_push(s, dir);
+        ppr_push(s, dir, src_begin, src_end);
     });
 }
 

@@ -20,7 +20,11 @@ struct tsvd_ppr {
     tsvd::DBuf<uint8_t> P[2], R[2];
     tsvd::DBuf<unsigned long long> cm[2], nm;
     tsvd::DBuf<uint32_t> F[2], fnext;
-    tsvd::DBuf<uint32_t> counters;           // [0],[1] pending per dir, [2] next, [3] overflow
+    tsvd::DBuf<uint32_t> counters;           // [0],[1] pending per dir, [2] next, [3] overflow,
+                                             // [4] split rows, [5] their further chunks
+    tsvd::DBuf<uint32_t> hub_item;           // split rows of a push round: item,
+    tsvd::DBuf<uint8_t> hub_delta;           //   its 64 pushed amounts,
+    tsvd::DBuf<unsigned long long> hub_chunks;  //   (slot, chunk) work entries
     tsvd::DBuf<unsigned long long> stats;    // [0] pops, [1] relaxations
     uint32_t *h_counters = nullptr;          // pinned
     int64_t last_rounds[2] = {0, 0}, last_pops[2] = {0, 0}, last_relax[2] = {0, 0};
@@ -35,6 +39,7 @@ struct tsvd_ppr {
     tsvd::DBuf<int32_t> wl_cnt;              // n + 1 scratch
     int64_t wl_n[2] = {0, 0}, wl_version[2] = {-1, -1};
     double pull_frac = 0.0;                  // pull when items > pull_frac * n (0: never)
+    double ride_frac = 0.0;                  // sub-threshold lanes above ride_frac*rmax ride along (>= 1: off)
     std::vector<int32_t> h_sources;
     ~tsvd_ppr()
     {
@@ -46,5 +51,5 @@ struct tsvd_ppr {
 
 namespace tsvd {
 void ppr_refresh(tsvd_ppr *s, int mode, int dir, int32_t s0, int32_t s1);
-void ppr_push(tsvd_ppr *s, int dir);
+void ppr_push(tsvd_ppr *s, int dir, int32_t s0, int32_t s1);
 }
