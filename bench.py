@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 FP32_MFMA_TFLOPS = 157.3       # MI355X dense FP32 matrix peak
 FP64_MFMA_TFLOPS = 78.6
+WORKLOAD_TAG = "batched"       # workload the committed counter summary was taken on
 
 
 def log(msg):
@@ -57,6 +58,19 @@ def roofline_of(prof, precision_fp64):
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "launches": launches, "avg_launch_ms": round(ms / launches, 4),
             "algorithmic_per_launch": nbytes / launches}
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (two --pmc passes, FETCH_SIZE and WRITE_SIZE, calibrated on a copy
+    of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"push_round_{WORKLOAD_TAG}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def cpu_baseline(w, gpu_ndirty, gpu_nmerged, n_steps_hint, budget_s=25.0):
@@ -111,6 +125,8 @@ def main():
     ap.add_argument("--workload", default="batched", choices=["plain", "batched", "precision"])
     ap.add_argument("--scale-down", type=int, default=0, help="divide nodes/edges by 2**k (smoke runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="first run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -128,9 +144,20 @@ def main():
     name, hbm_total, hbm_free = capi.device_info()
     log(f"device {name}, HBM {hbm_total / 2**30:.0f} GiB ({hbm_free / 2**30:.0f} free)")
 
+    if args.calibrate_copy:
+        nb = args.calibrate_copy << 20
+        a = capi.DeviceArray(np.zeros(nb // 4, np.float32))
+        b = capi.DeviceArray(np.ones(nb // 4, np.float32))
+        capi.check(capi.lib().tsvd_memcpy(a.ptr, b.ptr, nb, 2))
+        log(f"calibration copy of {args.calibrate_copy} MiB done")
+        a.free(); b.free()
     pipe = pipeline_for(capi, w)
     batches = [(capi.DeviceArray(s.astype(np.int32)), capi.DeviceArray(d.astype(np.int32)))
                for s, d in w.snapshots[:1 + need]]
+    # kernel events cover every launch of the run (as a rocprofv3 trace of this
+    # command does); the snapshot rate comes from the timed steps only
+    capi.profile_reset()
+    capi.profile_enable(True)
     t = time.perf_counter()
     st0 = pipe.snapshot(*batches[0])
     init_ms = (time.perf_counter() - t) * 1e3
@@ -140,8 +167,6 @@ def main():
         st = pipe.snapshot(*batches[1 + i])
         log(f"warmup {i}: {st['ms_total']:.1f} ms dirty={st['ndirty']}")
 
-    capi.profile_reset()
-    capi.profile_enable(True)
     lat, stats = [], []
     t_all = time.perf_counter()
     for i in range(args.steps):
@@ -162,6 +187,9 @@ def main():
     phases = {k: round(float(np.mean([s[k] for s in stats])), 3)
               for k in ("ms_ingest", "ms_refresh", "ms_push", "ms_blocks", "ms_frpca", "ms_merge")}
     roof = roofline_of(prof, w.fp64_state)
+    if args.workload == WORKLOAD_TAG and args.scale_down == 0 and roof["kernel"] == "ppr_push_round":
+        roof["traffic"] = profiled_traffic("k_push_round")
+        roof["traffic_source"] = f"profiles/push_round_{WORKLOAD_TAG}_hbm_traffic.md"
     out = {
         "metric": "update->embedding snapshots/s (plain GPU method), p50/p99 ms per update",
         "value": round(value, 4), "unit": "snapshots/s", "n_gpus": 1, "steps": args.steps,
