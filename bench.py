@@ -117,6 +117,80 @@ def cpu_baseline(w, gpu_ndirty, gpu_nmerged, n_steps_hint, budget_s=25.0):
                         "merge_one": round(t_merge, 3)}}
 
 
+def subspace_errors(G, U, k):
+    """Errors of an orthonormal |S| x k basis U against the dense FP64 SVD of A,
+    given only G = A A^T: subspace 1 - ||U_o^T U||_F^2 / k and reconstruction
+    ||A - U U^T A||_F^2 / ||A||_F^2 = 1 - tr(U^T G U) / tr(G)."""
+    lam, V = np.linalg.eigh(G)
+    Uo = V[:, ::-1][:, :k]
+    tr = float(np.trace(G))
+    sub = 1.0 - float(np.linalg.norm(Uo.T @ U) ** 2) / k
+    rec = 1.0 - float(np.trace(U.T @ G @ U)) / tr
+    best = 1.0 - float(lam[::-1][:k].sum()) / tr
+    return sub, rec, best
+
+
+def accuracy_report(pipe, w, seed=7):
+    """Root singular values, subspace and reconstruction error of the device
+    embedding against the dense FP64 SVD of the current |S| x n matrix (Gram
+    on the device in FP64, eigensolve in numpy), next to the reference
+    algorithm's own error: the oracle's FP64 frPCA (frpca.c) and exact merge
+    SVDs (Tree_u_NC.cpp:723-850) applied to the same stored blocks."""
+    from oracle import oracle as orc
+    k = w.rank
+    t = time.perf_counter()
+    G = pipe.dense_gram()
+    lam = np.linalg.eigvalsh(G)[::-1]
+    s_dense = np.sqrt(np.clip(lam[:k], 0, None))
+    t_gram = time.perf_counter() - t
+
+    def errs(sig, E, mode_nc):
+        U = E / np.sqrt(np.where(sig > 0, sig, 1.0))[None, :] if mode_nc else E
+        U, _ = np.linalg.qr(U)
+        rel = np.abs(sig - s_dense) / s_dense
+        sub, rec, best = subspace_errors(G, U, k)
+        return {"sigma_rel_err_max": float(rel.max()), "sigma_rel_err_top16_max": float(rel[:16].max()),
+                "sigma_rel_err_median": float(np.median(rel)), "subspace_err": sub,
+                "reconstruction_err": rec, "reconstruction_err_optimal_rank_k": best}
+    nc = w.transform == "NC"
+    dev = errs(pipe.root_sigma(), pipe.embedding(), nc)
+    # the reference algorithm (FP64 frPCA + exact merges) on the same stored blocks
+    t = time.perf_counter()
+    starts, widths = orc.block_map(w.n, w.nparts)
+    deg0 = pipe.graph.degrees(0)[0].astype(np.int64)
+    deg = deg0 + (pipe.graph.degrees(1)[0] if w.directed else 0)
+    act = deg > 0
+    act[w.subset] = True
+    leaves = []
+    for j in range(w.nparts):
+        Aj = pipe.block_matrix(j, int(widths[j]))
+        cols = np.flatnonzero(act[starts[j]:starts[j] + widths[j]])
+        Ac = Aj[:, cols]
+        if not Ac.any():
+            leaves.append(np.zeros((w.subset.size, k)))
+            continue
+        U, S, _ = orc.frpca(Ac, k, w.q, omega=orc.sketch(orc.leaf_seed(seed, 0, j), cols.size, k + 5))
+        leaves.append(U * S)
+    levels = orc.tree_levels(w.nparts, w.fanout)
+    cur = leaves
+    for lvl in range(1, len(levels)):
+        groups = [cur[i * w.fanout:min((i + 1) * w.fanout, len(cur))] for i in range(levels[lvl])]
+        if lvl == len(levels) - 1:
+            _, Ur, Sr = orc.merge(groups[0], k)
+        else:
+            cur = [orc.merge(ch, k)[0] for ch in groups]
+    if len(levels) == 1:
+        _, Ur, Sr = orc.merge(leaves, k)
+    ref = errs(Sr, Ur * np.sqrt(Sr)[None, :] if nc else Ur, nc)
+    t_ref = time.perf_counter() - t
+    gap = np.abs(pipe.root_sigma() - Sr) / Sr
+    return {"device": dev, "reference_algorithm_fp64": ref,
+            "device_vs_reference_sigma_rel_max": float(gap.max()),
+            "device_vs_reference_sigma_rel_top16_max": float(gap[:16].max()),
+            "dense_oracle": "FP64 Gram A A^T over all columns (device MFMA) + numpy eigh",
+            "seconds": {"dense_gram_and_eigh": round(t_gram, 1), "reference_algorithm": round(t_ref, 1)}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +199,8 @@ def main():
     ap.add_argument("--workload", default="batched", choices=["plain", "batched", "precision"])
     ap.add_argument("--scale-down", type=int, default=0, help="divide nodes/edges by 2**k (smoke runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="append errors against a dense FP64 SVD and the reference algorithm's own error")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="first run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -207,6 +283,9 @@ def main():
         "kernels": {k: {"launches": v[0], "ms": round(v[1], 3), "GB": round(v[2] / 1e9, 3),
                         "GFLOP": round(v[3] / 1e9, 3)} for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])},
     }
+    if args.accuracy:
+        log("accuracy against the dense FP64 oracle ...")
+        out["accuracy"] = accuracy_report(pipe, w)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(w, ndirty, nmerged, args.steps)

@@ -175,6 +175,10 @@ int tsvd_pipeline_block_norms(tsvd_pipeline *p, double *a_norm, double *e_norm, 
 /* Host copy of one tree node's factor U*S (nsrc x rank, col-major FP64);
  * level 0 = leaves. */
 int tsvd_pipeline_node(tsvd_pipeline *p, int32_t level, int32_t index, double *out);
+/* FP64 Gram A A^T (nsrc x nsrc, row-major) of the current transformed matrix
+ * over all n columns, computed on the device.  For accuracy reports against a
+ * dense FP64 SVD; the reference has no counterpart (it never forms it).     */
+int tsvd_pipeline_dense_gram(tsvd_pipeline *p, double *G);
 
 #ifdef __cplusplus
 }

@@ -359,3 +359,46 @@ void eig_select(double *W, const double *V, const double *lam, int l, int k, int
 }
 
 }  // namespace tsvd
+
+namespace tsvd {
+
+namespace {
+
+// X[(v - v0) * s_pad + s] = transform of the reserves at vertex v (FP64), the
+// padding rows zero: a vertex-major slab of the full |S| x n matrix.
+template <class T>
+__global__ void k_transform_chunk(const T *__restrict__ p0, const T *__restrict__ p1, double *__restrict__ X,
+                                  int64_t v0, int64_t nv, int32_t nsrc, int32_t s_pad, int transform, T rm)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv * s_pad) return;
+    const int64_t vv = t / s_pad;
+    const int32_t s = (int32_t)(t - vv * s_pad);
+    double a = 0.0;
+    if (s < nsrc) {
+        const int64_t at = (v0 + vv) * s_pad + s;
+        T x = p0[at];
+        if (p1) x += p1[at];
+        a = (double)xform<T>(x, rm, transform);
+    }
+    X[t] = a;
+}
+
+}  // namespace
+
+void transform_chunk(const Blocks &b, const void *p0, const void *p1, double *X, int64_t v0, int64_t nv,
+                     hipStream_t st)
+{
+    const int64_t tot = nv * b.s_pad;
+    if (tot <= 0) return;
+    if (b.precision == TSVD_FP64)
+        hipLaunchKernelGGL(k_transform_chunk<double>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st,
+                           (const double *)p0, (const double *)p1, X, v0, nv, b.nsrc, b.s_pad, b.transform, b.rmax);
+    else
+        hipLaunchKernelGGL(k_transform_chunk<float>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st,
+                           (const float *)p0, (const float *)p1, X, v0, nv, b.nsrc, b.s_pad, b.transform,
+                           (float)b.rmax);
+    TSVD_HIP(hipGetLastError());
+}
+
+}  // namespace tsvd

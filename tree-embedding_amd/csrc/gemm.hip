@@ -101,10 +101,13 @@ __global__ void __launch_bounds__(TPB) k_gemm(const GemmProb *__restrict__ probs
 #pragma unroll
         for (int q = 0; q < B_PER; q++) {
             const int e = tid + q * TPB;
-            const int jj = e / BK, kk = e % BK;
+            int jj, kk;
+            if (P.transB) { kk = e / BN; jj = e % BN; }
+            else { jj = e / BK; kk = e % BK; }
             const int gj = j0 + jj, gk = k0 + kk;
             TACC v = TACC(0);
-            if (e < BN * BK && gj < N && gk < kend) v = (TACC)B[(int64_t)gj * ldb + gk];
+            if (e < BN * BK && gj < N && gk < kend)
+                v = (TACC)(P.transB ? B[(int64_t)gk * ldb + gj] : B[(int64_t)gj * ldb + gk]);
             rb[q] = v;
         }
     };
@@ -120,7 +123,10 @@ __global__ void __launch_bounds__(TPB) k_gemm(const GemmProb *__restrict__ probs
 #pragma unroll
         for (int q = 0; q < B_PER; q++) {
             const int e = tid + q * TPB;
-            if (e < BN * BK) Bs[e % BK][e / BK] = rb[q];
+            if (e < BN * BK) {
+                if (P.transB) Bs[e / BN][e % BN] = rb[q];
+                else Bs[e % BK][e / BK] = rb[q];
+            }
         }
     };
 

@@ -246,3 +246,33 @@ extern "C" int tsvd_pipeline_node(tsvd_pipeline *p, int32_t level, int32_t index
             }
     });
 }
+
+// FP64 Gram A A^T (nsrc x nsrc, row-major) of the current transformed matrix
+// over all n columns, accumulated slab by slab on MFMA.
+extern "C" int tsvd_pipeline_dense_gram(tsvd_pipeline *p, double *G)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p && G, "null argument");
+        const Blocks &b = p->blocks;
+        hipStream_t st = p->g->stream;
+        const int64_t chunk = 65536;
+        const int32_t ns = b.nsrc;
+        DBuf<double> X((size_t)chunk * b.s_pad), Gd((size_t)ns * ns);
+        TSVD_HIP(hipMemsetAsync(Gd.p, 0, Gd.bytes(), st));
+        const void *p0 = p->ppr->p(0);
+        const void *p1 = p->ppr->ndir > 1 ? p->ppr->p(1) : nullptr;
+        for (int64_t v0 = 0; v0 < b.n; v0 += chunk) {
+            const int64_t nv = std::min<int64_t>(chunk, b.n - v0);
+            transform_chunk(b, p0, p1, X.p, v0, nv, st);
+            std::vector<GemmProb> gp(1);
+            gp[0] = GemmProb{};
+            gp[0].A = X.p; gp[0].lda = b.s_pad; gp[0].transA = 0;
+            gp[0].B = X.p; gp[0].ldb = b.s_pad; gp[0].transB = 1;
+            gp[0].C = Gd.p; gp[0].ldc = ns;
+            gp[0].M = ns; gp[0].N = ns; gp[0].K = (int32_t)nv; gp[0].alpha = 1.0; gp[0].beta = 1.0;
+            p->gemm.run(st, gp, GemmTypes{DT::F64, DT::F64, DT::F64, DT::F64}, true, "gemm_dense_gram");
+        }
+        TSVD_HIP(hipMemcpyAsync(G, Gd.p, Gd.bytes(), hipMemcpyDeviceToHost, st));
+        TSVD_HIP(hipStreamSynchronize(st));
+    });
+}

@@ -18,6 +18,7 @@ struct GemmProb {
     void *C;
     int64_t lda, ldb, ldc;
     int32_t M, N, K, transA;
+    int32_t transB;   // 1: B is stored N x K, B(k,j) = B[k*ldb + j]
     double alpha, beta;
     int32_t kchunk;
     int32_t tiles_m, tiles_n, tiles_k;

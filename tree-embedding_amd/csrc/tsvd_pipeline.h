@@ -105,6 +105,11 @@ void eig_rotation(double *W, const double *V, const double *lam, int l, int coun
 void eig_select(double *W, const double *V, const double *lam, int l, int k, int count, int mode, int ascending,
                 double eps, hipStream_t st);
 
+// FP64 vertex-major slab [v0, v0+nv) of the transformed |S| x n matrix
+// (X[(v - v0) * s_pad + s]; padding rows zero).
+void transform_chunk(const Blocks &b, const void *p0, const void *p1, double *X, int64_t v0, int64_t nv,
+                     hipStream_t st);
+
 // One column block (or one drop-in matrix) for the grouped frPCA core.
 struct FrpcaItem {
     const void *A;    // m x nc, column-major, leading dimension lda

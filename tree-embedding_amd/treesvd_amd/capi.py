@@ -29,6 +29,7 @@ EXPORTS = [
     "tsvd_pipeline_create", "tsvd_pipeline_destroy", "tsvd_pipeline_snapshot",
     "tsvd_pipeline_embedding", "tsvd_pipeline_root_sigma", "tsvd_pipeline_graph", "tsvd_pipeline_ppr",
     "tsvd_pipeline_block_matrix", "tsvd_pipeline_block_norms", "tsvd_pipeline_node",
+    "tsvd_pipeline_dense_gram",
 ]
 
 
@@ -117,6 +118,7 @@ def lib():
             L.tsvd_pipeline_block_matrix.argtypes = [vp, i32, vp]
             L.tsvd_pipeline_block_norms.argtypes = [vp, vp, vp, vp, vp]
             L.tsvd_pipeline_node.argtypes = [vp, i32, i32, vp]
+            L.tsvd_pipeline_dense_gram.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -364,6 +366,12 @@ class Pipeline:
         out = np.zeros((self.rank, self.nsrc))
         check(lib().tsvd_pipeline_node(self.h, level, index, _p(out)))
         return out.T.copy()
+
+    def dense_gram(self):
+        """FP64 A A^T of the current transformed matrix (nsrc x nsrc)."""
+        G = np.zeros((self.nsrc, self.nsrc))
+        check(lib().tsvd_pipeline_dense_gram(self.h, _p(G)))
+        return G
 
 
 def profile_enable(on: bool = True):
