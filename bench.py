@@ -117,44 +117,82 @@ def cpu_baseline(w, gpu_ndirty, gpu_nmerged, n_steps_hint, budget_s=25.0):
                         "merge_one": round(t_merge, 3)}}
 
 
-def subspace_errors(G, U, k):
-    """Errors of an orthonormal |S| x k basis U against the dense FP64 SVD of A,
-    given only G = A A^T: subspace 1 - ||U_o^T U||_F^2 / k and reconstruction
-    ||A - U U^T A||_F^2 / ||A||_F^2 = 1 - tr(U^T G U) / tr(G)."""
-    lam, V = np.linalg.eigh(G)
-    Uo = V[:, ::-1][:, :k]
-    tr = float(np.trace(G))
-    sub = 1.0 - float(np.linalg.norm(Uo.T @ U) ** 2) / k
-    rec = 1.0 - float(np.trace(U.T @ G @ U)) / tr
-    best = 1.0 - float(lam[::-1][:k].sum()) / tr
-    return sub, rec, best
+class DenseOracle:
+    """Exact top-k eigenpairs of G = A A^T for the |S| x n transformed matrix A
+    in FP64.  Small cases form G on the device and call numpy's eigh; large
+    ones run block subspace iteration on G through gram_apply (A (A^T Y), no
+    Gram) until every top-k Ritz pair's residual ||G u - theta u|| is below
+    `tol` * theta: a symmetric matrix then has an eigenvalue within that
+    residual of theta, so the reported bound certifies the sigma values."""
+
+    def __init__(self, pipe, k, a_norms, gram_limit=4e13, tol=1e-9, max_iter=120, seed=11):
+        self.pipe, self.k = pipe, k
+        ns = pipe.nsrc
+        t = time.perf_counter()
+        if float(ns) * ns * pipe.n <= gram_limit:
+            self.kind = "FP64 Gram A A^T over all columns (device MFMA) + numpy eigh"
+            self.G = pipe.dense_gram()
+            lam, V = np.linalg.eigh(self.G)
+            self.lam, self.U = lam[::-1][:k].copy(), V[:, ::-1][:, :k].copy()
+            self.tr = float(np.trace(self.G))
+            self.bound, self.iters = 0.0, 0
+        else:
+            self.G = None
+            self.tr = float(np.sum(np.asarray(a_norms, dtype=np.float64) ** 2))
+            l = min(ns, 2 * k + 64)
+            rng = np.random.Generator(np.random.PCG64(seed))
+            Y, _ = np.linalg.qr(rng.standard_normal((ns, l)))
+            for it in range(1, max_iter + 1):
+                Z = pipe.gram_apply(Y)
+                H = Y.T @ Z
+                th, W = np.linalg.eigh(0.5 * (H + H.T))
+                th, W = th[::-1], W[:, ::-1]
+                R = Z @ W[:, :k] - (Y @ W[:, :k]) * th[None, :k]
+                rel = np.linalg.norm(R, axis=0) / np.maximum(th[:k], 1e-300)
+                self.lam, self.U = th[:k].copy(), Y @ W[:, :k]
+                self.bound, self.iters = float(rel.max()), it
+                log(f"oracle subspace iteration {it}: l={l} max residual/theta over top {k} = {self.bound:.2e} "
+                    f"({time.perf_counter() - t:.1f}s)")
+                if self.bound < tol:
+                    break
+                Y, _ = np.linalg.qr(Z)
+            self.kind = (f"FP64 block subspace iteration on A A^T (device MFMA products, l={l}, {self.iters} "
+                         f"iterations, max top-{k} residual/theta {self.bound:.1e})")
+        self.sigma = np.sqrt(np.clip(self.lam, 0, None))
+        self.seconds = time.perf_counter() - t
+
+    def apply(self, U):
+        return self.G @ U if self.G is not None else self.pipe.gram_apply(U)
+
+    def errors(self, sig, E, mode_nc):
+        """sigma relative errors, subspace error 1 - ||U_o^T U||_F^2 / k and
+        reconstruction ||A - U U^T A||_F^2 / ||A||_F^2 = 1 - tr(U^T G U) / tr(G)
+        of an embedding (NC: U sqrt(S), LP: U) against the oracle."""
+        k = self.k
+        U = E / np.sqrt(np.where(sig > 0, sig, 1.0))[None, :] if mode_nc else E
+        U, _ = np.linalg.qr(U)
+        rel = np.abs(sig - self.sigma) / self.sigma
+        sub = 1.0 - float(np.linalg.norm(self.U.T @ U) ** 2) / k
+        rec = 1.0 - float(np.sum(U * self.apply(U))) / self.tr
+        best = 1.0 - float(self.lam.sum()) / self.tr
+        return {"sigma_rel_err_max": float(rel.max()), "sigma_rel_err_top16_max": float(rel[:16].max()),
+                "sigma_rel_err_median": float(np.median(rel)), "subspace_err": sub,
+                "reconstruction_err": rec, "reconstruction_err_optimal_rank_k": best}
 
 
 def accuracy_report(pipe, w, seed=7):
     """Root singular values, subspace and reconstruction error of the device
-    embedding against the dense FP64 SVD of the current |S| x n matrix (Gram
-    on the device in FP64, eigensolve in numpy), next to the reference
-    algorithm's own error: the oracle's FP64 frPCA (frpca.c) and exact merge
-    SVDs (Tree_u_NC.cpp:723-850) applied to the same stored blocks."""
+    embedding against the exact FP64 SVD of the current |S| x n matrix, next
+    to the reference algorithm's own error: the oracle's FP64 frPCA (frpca.c)
+    and exact merge SVDs (Tree_u_NC.cpp:723-850) applied to the same stored
+    blocks."""
     from oracle import oracle as orc
     k = w.rank
-    t = time.perf_counter()
-    G = pipe.dense_gram()
-    lam = np.linalg.eigvalsh(G)[::-1]
-    s_dense = np.sqrt(np.clip(lam[:k], 0, None))
-    t_gram = time.perf_counter() - t
-
-    def errs(sig, E, mode_nc):
-        U = E / np.sqrt(np.where(sig > 0, sig, 1.0))[None, :] if mode_nc else E
-        U, _ = np.linalg.qr(U)
-        rel = np.abs(sig - s_dense) / s_dense
-        sub, rec, best = subspace_errors(G, U, k)
-        return {"sigma_rel_err_max": float(rel.max()), "sigma_rel_err_top16_max": float(rel[:16].max()),
-                "sigma_rel_err_median": float(np.median(rel)), "subspace_err": sub,
-                "reconstruction_err": rec, "reconstruction_err_optimal_rank_k": best}
     nc = w.transform == "NC"
-    dev = errs(pipe.root_sigma(), pipe.embedding(), nc)
-    # the reference algorithm (FP64 frPCA + exact merges) on the same stored blocks
+    a_norms = pipe.block_norms()[0]
+    orac = DenseOracle(pipe, k, a_norms)
+    log(f"oracle ready in {orac.seconds:.1f}s: {orac.kind}")
+    dev = orac.errors(pipe.root_sigma(), pipe.embedding(), nc)
     t = time.perf_counter()
     starts, widths = orc.block_map(w.n, w.nparts)
     deg0 = pipe.graph.degrees(0)[0].astype(np.int64)
@@ -166,11 +204,14 @@ def accuracy_report(pipe, w, seed=7):
         Aj = pipe.block_matrix(j, int(widths[j]))
         cols = np.flatnonzero(act[starts[j]:starts[j] + widths[j]])
         Ac = Aj[:, cols]
+        del Aj
         if not Ac.any():
             leaves.append(np.zeros((w.subset.size, k)))
-            continue
-        U, S, _ = orc.frpca(Ac, k, w.q, omega=orc.sketch(orc.leaf_seed(seed, 0, j), cols.size, k + 5))
-        leaves.append(U * S)
+        else:
+            U, S, _ = orc.frpca(Ac, k, w.q, omega=orc.sketch(orc.leaf_seed(seed, 0, j), cols.size, k + 5))
+            leaves.append(U * S)
+        if (j + 1) % 32 == 0 or j + 1 == w.nparts:
+            log(f"reference algorithm: {j + 1}/{w.nparts} leaf frPCAs ({time.perf_counter() - t:.1f}s)")
     levels = orc.tree_levels(w.nparts, w.fanout)
     cur = leaves
     for lvl in range(1, len(levels)):
@@ -179,16 +220,17 @@ def accuracy_report(pipe, w, seed=7):
             _, Ur, Sr = orc.merge(groups[0], k)
         else:
             cur = [orc.merge(ch, k)[0] for ch in groups]
+        log(f"reference algorithm: merge level {lvl} ({levels[lvl]} nodes)")
     if len(levels) == 1:
         _, Ur, Sr = orc.merge(leaves, k)
-    ref = errs(Sr, Ur * np.sqrt(Sr)[None, :] if nc else Ur, nc)
+    ref = orac.errors(Sr, Ur * np.sqrt(Sr)[None, :] if nc else Ur, nc)
     t_ref = time.perf_counter() - t
     gap = np.abs(pipe.root_sigma() - Sr) / Sr
     return {"device": dev, "reference_algorithm_fp64": ref,
             "device_vs_reference_sigma_rel_max": float(gap.max()),
             "device_vs_reference_sigma_rel_top16_max": float(gap[:16].max()),
-            "dense_oracle": "FP64 Gram A A^T over all columns (device MFMA) + numpy eigh",
-            "seconds": {"dense_gram_and_eigh": round(t_gram, 1), "reference_algorithm": round(t_ref, 1)}}
+            "dense_oracle": orac.kind, "oracle_sigma_rel_bound": 0.5 * orac.bound,
+            "seconds": {"oracle": round(orac.seconds, 1), "reference_algorithm": round(t_ref, 1)}}
 
 
 def main():

@@ -179,6 +179,11 @@ int tsvd_pipeline_node(tsvd_pipeline *p, int32_t level, int32_t index, double *o
  * over all n columns, computed on the device.  For accuracy reports against a
  * dense FP64 SVD; the reference has no counterpart (it never forms it).     */
 int tsvd_pipeline_dense_gram(tsvd_pipeline *p, double *G);
+/* Z = A A^T Y (nsrc x l, row-major FP64 in and out) for the same matrix,
+ * without forming A A^T: two FP64 products per slab of columns.  Subspace
+ * iteration on it gives the exact top singular values where the Gram is too
+ * large to form; no reference counterpart.                                */
+int tsvd_pipeline_gram_apply(tsvd_pipeline *p, const double *Y, int32_t l, double *Z);
 
 #ifdef __cplusplus
 }

@@ -134,3 +134,34 @@ def test_pipeline_vs_oracle_pipeline_and_dense_svd(capi):
     # agree to the push error (different push order, same sketches)
     assert err_gpu.max() <= err_orc.max() + 1e-3, msg
     assert diff[:k // 2].max() < 1e-3 and diff.max() < 2e-2, msg
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_dense_gram_gram_apply_and_iterative_oracle(capi, directed):
+    """The accuracy oracles on the device: the FP64 Gram A A^T and the Gram-free
+    product A (A^T Y) both equal the host products of the transformed device
+    reserves; subspace iteration through gram_apply reaches numpy's eigh of
+    that Gram (top-k eigenvalues, the spanned subspace and its certificate)."""
+    import bench
+    n, snaps, sub = small_workload(directed, seed=6)
+    rmax, k = 1e-6, 10
+    pipe = capi.Pipeline(n, directed, sub, 0.2, rmax, 4, 2, k, 6, precision=1, seed=3)
+    for s, d in snaps[:2]:
+        pipe.snapshot(s, d)
+    A = transform_of(pipe, n, directed, rmax, "NC")
+    G = A @ A.T
+    scale = np.abs(G).max()
+    assert np.abs(pipe.dense_gram() - G).max() <= 1e-12 * scale * n ** 0.5
+    Y = np.random.Generator(np.random.PCG64(0)).standard_normal((sub.size, 7))
+    assert np.abs(pipe.gram_apply(Y) - G @ Y).max() <= 1e-12 * scale * n ** 0.5 * np.abs(Y).max() * sub.size
+    lam, V = np.linalg.eigh(G)
+    lam, V = lam[::-1][:k], V[:, ::-1][:, :k]
+    it = bench.DenseOracle(pipe, k, np.sqrt((A.reshape(A.shape[0], -1) ** 2).sum()).reshape(1), gram_limit=0,
+                           tol=1e-10)
+    assert it.bound < 1e-10
+    assert np.abs(it.lam - lam).max() <= 1e-8 * lam[0]
+    assert abs(it.tr - np.trace(G)) <= 1e-10 * np.trace(G)
+    # same subspace: 1 - ||V^T U||^2 / k ~ 0
+    assert 1.0 - np.linalg.norm(V.T @ it.U) ** 2 / k < 1e-8
+    dense = bench.DenseOracle(pipe, k, None)
+    assert dense.G is not None and np.abs(dense.lam - lam).max() <= 1e-10 * lam[0]

@@ -276,3 +276,49 @@ extern "C" int tsvd_pipeline_dense_gram(tsvd_pipeline *p, double *G)
         TSVD_HIP(hipStreamSynchronize(st));
     });
 }
+
+// Z = A (A^T Y) in FP64 for an nsrc x l block Y, slab by slab: T = X^T Y
+// (nv x l), then Z += X T.  Subspace iteration on A A^T without the Gram.
+extern "C" int tsvd_pipeline_gram_apply(tsvd_pipeline *p, const double *Y, int32_t l, double *Z)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p && Y && Z, "null argument");
+        const Blocks &b = p->blocks;
+        TSVD_REQUIRE(l >= 1 && l <= b.nsrc, "l must be in [1, nsrc]");
+        hipStream_t st = p->g->stream;
+        const int64_t chunk = 65536;
+        const int32_t ns = b.nsrc;
+        // row-major host blocks <-> column-major device blocks (ld = nsrc)
+        std::vector<double> h((size_t)ns * l);
+        for (int32_t s = 0; s < ns; s++)
+            for (int32_t c = 0; c < l; c++) h[(size_t)c * ns + s] = Y[(size_t)s * l + c];
+        DBuf<double> X((size_t)chunk * b.s_pad), T((size_t)chunk * l), Yd(h.size()), Zd(h.size());
+        TSVD_HIP(hipMemcpyAsync(Yd.p, h.data(), Yd.bytes(), hipMemcpyHostToDevice, st));
+        TSVD_HIP(hipMemsetAsync(Zd.p, 0, Zd.bytes(), st));
+        const void *p0 = p->ppr->p(0);
+        const void *p1 = p->ppr->ndir > 1 ? p->ppr->p(1) : nullptr;
+        const GemmTypes f64{DT::F64, DT::F64, DT::F64, DT::F64};
+        for (int64_t v0 = 0; v0 < b.n; v0 += chunk) {
+            const int64_t nv = std::min<int64_t>(chunk, b.n - v0);
+            transform_chunk(b, p0, p1, X.p, v0, nv, st);
+            std::vector<GemmProb> gp(1);
+            gp[0] = GemmProb{};
+            gp[0].A = X.p; gp[0].lda = b.s_pad; gp[0].transA = 1;
+            gp[0].B = Yd.p; gp[0].ldb = ns;
+            gp[0].C = T.p; gp[0].ldc = nv;
+            gp[0].M = (int32_t)nv; gp[0].N = l; gp[0].K = ns; gp[0].alpha = 1.0; gp[0].beta = 0.0;
+            p->gemm.run(st, gp, f64, true, "gemm_gram_apply");
+            gp[0] = GemmProb{};
+            gp[0].A = X.p; gp[0].lda = b.s_pad; gp[0].transA = 0;
+            gp[0].B = T.p; gp[0].ldb = nv;
+            gp[0].C = Zd.p; gp[0].ldc = ns;
+            gp[0].M = ns; gp[0].N = l; gp[0].K = (int32_t)nv; gp[0].alpha = 1.0; gp[0].beta = 1.0;
+            gp[0].kchunk = 4096;  // split K across workgroups, partial sums added atomically
+            p->gemm.run(st, gp, f64, true, "gemm_gram_apply");
+        }
+        TSVD_HIP(hipMemcpyAsync(h.data(), Zd.p, Zd.bytes(), hipMemcpyDeviceToHost, st));
+        TSVD_HIP(hipStreamSynchronize(st));
+        for (int32_t s = 0; s < ns; s++)
+            for (int32_t c = 0; c < l; c++) Z[(size_t)s * l + c] = h[(size_t)c * ns + s];
+    });
+}

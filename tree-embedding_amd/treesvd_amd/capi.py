@@ -29,7 +29,7 @@ EXPORTS = [
     "tsvd_pipeline_create", "tsvd_pipeline_destroy", "tsvd_pipeline_snapshot",
     "tsvd_pipeline_embedding", "tsvd_pipeline_root_sigma", "tsvd_pipeline_graph", "tsvd_pipeline_ppr",
     "tsvd_pipeline_block_matrix", "tsvd_pipeline_block_norms", "tsvd_pipeline_node",
-    "tsvd_pipeline_dense_gram",
+    "tsvd_pipeline_dense_gram", "tsvd_pipeline_gram_apply",
 ]
 
 
@@ -119,6 +119,7 @@ def lib():
             L.tsvd_pipeline_block_norms.argtypes = [vp, vp, vp, vp, vp]
             L.tsvd_pipeline_node.argtypes = [vp, i32, i32, vp]
             L.tsvd_pipeline_dense_gram.argtypes = [vp, vp]
+            L.tsvd_pipeline_gram_apply.argtypes = [vp, vp, i32, vp]
         _lib = L
     return _lib
 
@@ -372,6 +373,15 @@ class Pipeline:
         G = np.zeros((self.nsrc, self.nsrc))
         check(lib().tsvd_pipeline_dense_gram(self.h, _p(G)))
         return G
+
+    def gram_apply(self, Y):
+        """Z = A A^T Y in FP64 for an nsrc x l block Y, without forming A A^T."""
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        if Y.ndim != 2 or Y.shape[0] != self.nsrc:
+            raise ValueError(f"Y must be {self.nsrc} x l")
+        Z = np.zeros_like(Y)
+        check(lib().tsvd_pipeline_gram_apply(self.h, _p(Y), Y.shape[1], _p(Z)))
+        return Z
 
 
 def profile_enable(on: bool = True):
