@@ -1,0 +1,307 @@
+"""Pins the oracle's block, transform, frPCA and merge functions (CPU only).
+
+The reference's MKL/Eigen code cannot be built here, and its frPCA draws its
+sketch from a time-seeded RNG (mkl.c:495), so no golden output of it exists.
+Each function is checked instead against what the mathematics fixes: closed
+forms, textbook routines (numpy / scipy LAPACK), published test vectors and
+hand-computed cases.  Every check is chosen so that a plausible slip (wrong
+singular-value selection, dropped row swaps, a strict/non-strict comparison,
+a transposed operand) fails it; see test_mutations_are_caught."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import oracle as orc
+
+
+def rng(seed=0):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def with_spectrum(m, n, sig, seed=0):
+    """m x n matrix with singular values `sig` and random orthonormal factors."""
+    r = rng(seed)
+    U, _ = np.linalg.qr(r.standard_normal((m, len(sig))))
+    V, _ = np.linalg.qr(r.standard_normal((n, len(sig))))
+    return (U * sig) @ V.T
+
+
+# ------------------------------------------------------------- block map
+@pytest.mark.parametrize("n,nparts", [(10, 3), (131072, 16), (2738013, 512), (7, 7), (5, 1)])
+def test_block_map_matches_vertex_rule(n, nparts):
+    """Tree_u_NC.cpp:396-432: block of v = min(v // common, B-1), inner column
+    v - block*common, common = n // B, last block takes the remainder."""
+    starts, widths = orc.block_map(n, nparts)
+    common = n // nparts
+    v = np.arange(n, dtype=np.int64)
+    blk = np.minimum(v // common, nparts - 1)
+    inner = v - blk * common
+    assert widths.sum() == n
+    assert np.all(np.bincount(blk, minlength=nparts) == widths)
+    assert np.all(starts[blk] + inner == v)
+    assert np.all(inner < widths[blk])
+
+
+def test_block_map_config_widths():
+    # the precision config's blocks: 5,347 columns, the last 5,696
+    _, w = orc.block_map(2738013, 512)
+    assert w[0] == 5347 and w[-1] == 5696
+
+
+def test_tree_levels():
+    assert orc.tree_levels(16, 2) == [16, 8, 4, 2, 1]
+    assert orc.tree_levels(256, 4) == [256, 64, 16, 4, 1]
+    assert orc.tree_levels(512, 4) == [512, 128, 32, 8, 2, 1]     # ragged: 512 is not a power of 4
+    assert orc.tree_levels(1, 4) == [1]
+    # heap-tree node count (ppr:225-245) when nparts = f^(L-1)
+    assert sum(orc.tree_levels(64, 4)) == (4 ** 4 - 1) // 3
+
+
+# ------------------------------------------------------------- transform
+def test_transform_nc_known_values():
+    rmax = 1e-7
+    x = np.array([0.0, -1e-3, rmax, rmax * (1 + 1e-9), 10 * rmax, 1000 * rmax, 1.0])
+    y = orc.transform(x, rmax, "NC")
+    assert y[0] == 0 and y[1] == 0
+    assert y[2] == 0.0                         # x == rmax is not > rmax
+    assert 0 < y[3] < 1e-8
+    assert y[4] == pytest.approx(1.0, abs=1e-15)
+    assert y[5] == pytest.approx(3.0, abs=1e-14)
+    assert y[6] == pytest.approx(7.0, abs=1e-14)
+
+
+def test_transform_lp_known_values():
+    rmax = 1e-6
+    x = np.array([0.0, 9 * rmax, 99 * rmax, 0.5 * rmax, 1e-12])
+    y = orc.transform(x, rmax, "LP")
+    assert y[0] == 0.0
+    assert y[1] == pytest.approx(1.0, abs=1e-15)
+    assert y[2] == pytest.approx(2.0, abs=1e-15)
+    assert y[3] == pytest.approx(np.log10(1.5), abs=1e-15)
+    assert 0 < y[4] < 1e-6                     # every non-zero is transformed
+
+
+# ------------------------------------------------------------ dirty test
+def test_dirty_test_hand_cases():
+    d = 0.65
+    A = np.diag([3.0, 4.0])                    # ||A||_F = 5
+    thr = d * np.sqrt(2.0) * 5.0               # 4.5962
+    # unchanged block, no residual: clean
+    dirty, an, en = orc.dirty_test(A, A, 0.0, d)
+    assert not dirty and an == 5.0 and en == 0.0
+    # fresh block (old = 0): e = 5 >= thr: dirty
+    dirty, _, en = orc.dirty_test(A, np.zeros_like(A), 0.0, d)
+    assert dirty and en == 5.0
+    # residual pushes it over: resid + e >= thr
+    B = A.copy(); B[0, 0] = 3.0 - 1.0          # ||A - B|| = 1
+    assert not orc.dirty_test(A, B, thr - 1.0 - 1e-9, d)[0]
+    assert orc.dirty_test(A, B, thr - 1.0 + 1e-9, d)[0]
+    # an all-zero new block is never clean (0 < 0 is false)
+    assert orc.dirty_test(np.zeros((2, 2)), np.zeros((2, 2)), 0.0, d)[0]
+
+
+# ------------------------------------------------------------ LU, eigSVD
+@pytest.mark.parametrize("m,n", [(40, 7), (200, 30), (9, 9)])
+def test_lu_fraction_is_the_row_swapped_unit_lower_factor(m, n):
+    """LUfraction (frpca.c:4-41): dgetrf with partial pivoting, L made unit
+    lower trapezoidal, row swaps undone: A = L_frac U with P^T L_frac unit
+    lower and |entries| <= 1."""
+    A = rng(m).standard_normal((m, n))
+    L = orc.lu_fraction(A)
+    P, Lu, U = scipy.linalg.lu(A)
+    assert np.allclose(L @ U, A, atol=1e-12)
+    Lp = P.T @ L
+    assert np.allclose(np.triu(Lp[:n], 1), 0) and np.allclose(np.diag(Lp[:n]), 1.0)
+    assert np.abs(L).max() <= 1.0 + 1e-12
+    # same column space as A
+    Qa, _ = np.linalg.qr(A)
+    assert np.allclose(Qa @ (Qa.T @ L), L, atol=1e-10)
+
+
+@pytest.mark.parametrize("m,n", [(50, 8), (300, 20)])
+def test_eig_svd_is_the_thin_svd_of_a_full_rank_tall_matrix(m, n):
+    """eigSVD (frpca.c:99-136) of a full-rank tall X is its thin SVD, with the
+    singular values ascending (dsyevd order)."""
+    X = rng(n).standard_normal((m, n))
+    U, S, V = orc.eig_svd(X)
+    s_ref = np.linalg.svd(X, compute_uv=False)
+    assert np.allclose(S, s_ref[::-1], rtol=1e-12)
+    assert np.all(np.diff(S) >= 0)
+    assert np.allclose(U.T @ U, np.eye(n), atol=1e-10)
+    assert np.allclose((U * S) @ V.T, X, atol=1e-10)
+
+
+def test_eig_svd_zeroes_tiny_singular_values():
+    X = with_spectrum(40, 6, np.array([5.0, 2.0, 1.0, 1e-6, 5e-5, 3.0]))
+    U, S, V = orc.eig_svd(X)
+    assert np.count_nonzero(S) == 4            # sigma <= 1e-4 -> 0 and column dropped
+    assert np.allclose(U[:, :2], 0.0)
+
+
+# ----------------------------------------------------------------- sketch
+def test_sketch_hash_is_splitmix64():
+    """The sketch's hash is SplitMix64's output function; its published
+    outputs for seed 1234567 (Vigna, splitmix64.c)."""
+    g = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        out = [int(orc._mix64(np.uint64(1234567) + np.uint64(i) * g)) for i in range(5)]
+    assert out == [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                   4593380528125082431, 16408922859458223821]
+
+
+def test_sketch_is_standard_normal_and_deterministic():
+    import scipy.stats
+    X = orc.sketch(42, 500, 400)
+    assert np.array_equal(X, orc.sketch(42, 500, 400))
+    assert not np.array_equal(X, orc.sketch(43, 500, 400))
+    assert abs(X.mean()) < 0.01 and abs(X.var() - 1.0) < 0.01
+    assert scipy.stats.kstest(X.ravel(), "norm").pvalue > 1e-3
+    # column-major counter: column c of a taller sketch starts the same stream
+    assert np.array_equal(orc.sketch(7, 10, 3)[:, 0], orc.sketch(7, 10, 1)[:, 0])
+
+
+# ------------------------------------------------------------------ frPCA
+@pytest.mark.parametrize("q", [2, 3, 4, 6])
+def test_frpca_exact_on_rank_deficient(q):
+    """Rank r <= k: the range finder captures A exactly, so U S V^T = A and the
+    k-r extra singular values are zero."""
+    sig = np.array([9.0, 4.0, 2.5, 1.0, 0.5])
+    A = with_spectrum(60, 90, sig, seed=q)
+    U, S, V = orc.frpca(A, 8, q, rng=rng(q))
+    assert np.allclose(S[-5:], np.sort(sig), rtol=1e-9)
+    assert np.allclose(S[:3], 0, atol=1e-6)
+    assert np.allclose((U * S) @ V.T, A, atol=1e-9)
+
+
+def test_frpca_top_k_and_ascending_order():
+    """Decaying spectrum: q=6 recovers the top k to 1e-9, ascending
+    (frpca.c:305-316 keeps SS[s..k+s), the largest k of an ascending list)."""
+    # sigma_l^2 well above eigSVD's 1e-4 cut, and a power basis whose Gram is
+    # well conditioned (eigSVD orthonormalises through A^T A)
+    sig = 100.0 * 0.67 ** np.arange(40)
+    A = with_spectrum(120, 200, sig, seed=3)
+    k = 10
+    U, S, V = orc.frpca(A, k, 6, omega=orc.sketch(5, 200, k + 5))
+    assert np.all(np.diff(S) > 0)
+    assert np.allclose(S, sig[:k][::-1], rtol=1e-9)
+    assert np.allclose(U.T @ U, np.eye(k), atol=1e-9)
+    assert np.allclose(V.T @ V, np.eye(k), atol=1e-9)
+    # V S = A^T U exactly (frpca.c:303-316: V, S from eigSVD(A^T Q), U = Q VV)
+    assert np.allclose(A.T @ U, V * S, atol=1e-9 * S.max())
+
+
+def test_frpca_drops_directions_below_the_eigsvd_cut_of_the_power_basis():
+    """frpca.c:293-296 calls eigSVD (absolute cut sigma <= 1e-4) on the power
+    iterate Q = A A^T Q, whose singular values scale as sigma^2: with q = 6 a
+    direction with sigma^2 below ~1e-4 is zeroed even though sigma is not."""
+    sig = 10.0 * 0.45 ** np.arange(40)        # sigma_10 = 7.6e-3, sigma_10^2 = 5.7e-5
+    A = with_spectrum(120, 200, sig, seed=3)
+    k = 10
+    _, S, _ = orc.frpca(A, k, 6, omega=orc.sketch(5, 200, k + 5))
+    assert S[0] == 0.0
+    assert np.allclose(S[-5:], sig[:5][::-1], rtol=1e-8)
+    # with q = 2 there is no power iterate and the direction survives
+    _, S2, _ = orc.frpca(A, k, 2, omega=orc.sketch(5, 200, k + 5))
+    assert S2[0] > 0
+
+
+@pytest.mark.parametrize("q", [4, 6])
+def test_frpca_eckart_young_ratio(q):
+    """Flat-ish spectrum: the rank-k error is within a few percent of the
+    optimum ||A - A_k||_F (paper Eqn 1, power iterations q >= 4)."""
+    sig = 1.0 / (1.0 + np.arange(80)) ** 0.7
+    A = with_spectrum(150, 300, sig, seed=q)
+    k = 12
+    U, S, V = orc.frpca(A, k, q, rng=rng(9))
+    err = np.linalg.norm(A - (U * S) @ V.T)
+    best = np.sqrt((sig[k:] ** 2).sum())
+    assert best <= err <= 1.05 * best
+
+
+# ------------------------------------------------------------------ merge
+def test_merge_is_the_rank_k_truncation_of_the_concatenation():
+    """parent parent^T = (C)_k (C)_k^T, with (C)_k taken from eigh of C C^T
+    (a different LAPACK route than the merge's SVD)."""
+    r = rng(1)
+    kids = [r.standard_normal((50, 6)) * (0.5 ** np.arange(6)) for _ in range(4)]
+    C = np.concatenate(kids, axis=1)
+    k = 6
+    parent, U, S = orc.merge(kids, k)
+    lam, W = np.linalg.eigh(C @ C.T)
+    Wk = W[:, ::-1][:, :k]
+    Ck = Wk @ (Wk.T @ C)
+    assert np.allclose(parent @ parent.T, Ck @ Ck.T, atol=1e-10)
+    assert np.allclose(S, np.sqrt(lam[::-1][:k]), rtol=1e-10)
+    assert np.all(np.diff(S) <= 0)             # dgesvd order: descending
+
+
+def test_merge_of_orthogonal_children_unions_their_spectra():
+    r = rng(2)
+    Q, _ = np.linalg.qr(r.standard_normal((40, 12)))
+    s1, s2 = np.array([5.0, 3.0, 1.0]), np.array([4.0, 2.0, 0.5])
+    c1 = Q[:, 0:3] * s1
+    c2 = Q[:, 3:6] * s2
+    _, _, S = orc.merge([c1, c2], 4)
+    assert np.allclose(S, [5.0, 4.0, 3.0, 2.0])
+
+
+def test_root_embedding_grams():
+    r = rng(3)
+    kids = [r.standard_normal((30, 5)) for _ in range(2)]
+    C = np.concatenate(kids, axis=1)
+    U, S, _ = np.linalg.svd(C, full_matrices=False)
+    k = 4
+    E = orc.root_embedding(kids, k, "NC")
+    assert np.allclose(E @ E.T, (U[:, :k] * S[:k]) @ U[:, :k].T, atol=1e-10)   # U S U^T
+    E = orc.root_embedding(kids, k, "LP")
+    P = E @ E.T
+    assert np.allclose(P @ P, P, atol=1e-10) and np.trace(P) == pytest.approx(k)
+
+
+def test_whole_tree_exact_when_rank_fits():
+    """If every leaf block has rank <= k and the full matrix has rank <= k, the
+    tree of frPCA leaves and exact merges reproduces the SVD of the whole
+    matrix (nothing is truncated anywhere)."""
+    sig = np.array([6.0, 3.0, 2.0, 1.0])
+    A = with_spectrum(40, 64, sig, seed=4)
+    starts, widths = orc.block_map(64, 4)
+    k = 6
+    leaves = []
+    for j in range(4):
+        B = A[:, starts[j]:starts[j] + widths[j]]
+        U, S, _ = orc.frpca(B, k, 6, omega=orc.sketch(orc.leaf_seed(1, 0, j), B.shape[1], k + 5))
+        leaves.append(U * S)
+    lv = [orc.merge(leaves[0:2], k)[0], orc.merge(leaves[2:4], k)[0]]
+    _, U, S = orc.merge(lv, k)
+    assert np.allclose(S[:4], sig, rtol=1e-9) and np.allclose(S[4:], 0, atol=1e-7)
+
+
+# ---------------------------------------------- the checks catch slips
+def test_mutations_are_caught():
+    """Plausible slips in the restatement fail the checks above: selecting the
+    smallest k of frPCA's ascending list, skipping LUfraction's row swaps, a
+    non-strict NC threshold, and a transposed merge operand."""
+    sig = 100.0 * 0.67 ** np.arange(40)
+    A = with_spectrum(120, 200, sig, seed=3)
+    k = 10
+    l = k + 5
+    om = orc.sketch(5, 200, l)
+    # (1) wrong selection: the first k of the ascending list
+    Q = A @ om
+    for _ in range(2):
+        Q, _ = np.linalg.qr(A @ (A.T @ Q))
+    _, SS, _ = orc.eig_svd(A.T @ Q)
+    assert not np.allclose(SS[:k], sig[:k][::-1], rtol=1e-3)
+    # (2) LU without the row swaps does not factor A
+    X = rng(5).standard_normal((40, 7))
+    _, Lu, U = scipy.linalg.lu(X)
+    assert not np.allclose(Lu @ U, X, atol=1e-6)
+    # (3) x >= rmax instead of x > rmax moves the boundary value
+    rmax = 1e-7
+    assert np.log10(rmax / rmax) == 0.0 and orc.transform(np.array([rmax]), rmax, "NC")[0] == 0.0
+    assert orc.transform(np.array([rmax * 1.0000001]), rmax, "NC")[0] > 0
+    # (4) a transposed child changes the merge result
+    kids = [rng(6).standard_normal((20, 20)) for _ in range(2)]
+    _, _, S1 = orc.merge(kids, 5)
+    _, _, S2 = orc.merge([kids[0].T, kids[1]], 5)
+    assert not np.allclose(S1, S2, rtol=1e-3)
