@@ -185,6 +185,44 @@ int tsvd_pipeline_dense_gram(tsvd_pipeline *p, double *G);
  * large to form; no reference counterpart.                                */
 int tsvd_pipeline_gram_apply(tsvd_pipeline *p, const double *Y, int32_t l, double *Z);
 
+/* ---------------------------------------------------------- served form
+ * No reference counterpart: the reference's drivers run the whole update once
+ * per snapshot file (Tree_u_NC.cpp:507-866).  A server owns one pipeline and a
+ * worker thread.  Clients submit insertion batches (requests) from any
+ * thread; each step the worker admits the queued requests in submit order,
+ * up to max_step_edges (0: half of the free HBM at 64 B per edge) and
+ * max_step_requests (0: no cap; 1: the plain per-snapshot method), appends
+ * them as one snapshot and completes them with that snapshot's embedding.
+ * Requests arriving during a step wait for the next one.                   */
+typedef struct tsvd_server tsvd_server;
+
+typedef struct {
+    int64_t max_step_edges;
+    int32_t max_step_requests;
+} tsvd_server_config;
+
+typedef struct {
+    int64_t ticket;
+    int32_t snapshot;        /* pipeline snapshot that applied the request      */
+    int32_t step_requests;   /* requests applied by that snapshot               */
+    int64_t step_edges;
+    int64_t last_ticket;     /* state version: every ticket <= this is applied  */
+    double submit_ms, start_ms, done_ms, latency_ms;   /* server clock, ms      */
+    tsvd_snapshot_stats step;
+} tsvd_request_result;
+
+int tsvd_server_create(const tsvd_config *cfg, const tsvd_server_config *scfg, tsvd_server **out);
+/* Completes every queued request, then frees the server and its pipeline. */
+int tsvd_server_destroy(tsvd_server *s);
+/* Copies the batch (host ids, validated here) and returns its ticket. */
+int tsvd_server_submit(tsvd_server *s, const int32_t *src, const int32_t *dst, int64_t m, int64_t *ticket);
+/* Blocks until the ticket's step is done; E (nsrc x rank row-major, may be
+ * NULL) receives that step's embedding.  Each ticket is collected once.   */
+int tsvd_server_wait(tsvd_server *s, int64_t ticket, double *E, tsvd_request_result *res);
+/* The server's pipeline, for read-only accessors while no request is queued. */
+tsvd_pipeline *tsvd_server_pipeline(tsvd_server *s);
+int64_t tsvd_server_edge_budget(const tsvd_server *s);
+
 #ifdef __cplusplus
 }
 #endif

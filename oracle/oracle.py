@@ -270,11 +270,12 @@ def sketch(seed, rows, cols, fp32=False):
     return x.astype(np.float32).astype(np.float64) if fp32 else x
 
 
-def leaf_seed(seed, it, j):
-    """Sketch seed of column block j at snapshot it (matches the device)."""
-    seed, it, j = int(seed), int(it), int(j)
-    s_it = (seed + 0x1000003 * it) & M64
-    return (s_it * 0x9E3779B97F4A7C15 + j * 0xD1B54A32D192ED03 + 1) & M64
+def leaf_seed(seed, j):
+    """Sketch seed of column block j (matches the device).  It does not depend
+    on the snapshot, so a block's factor does not depend on how insertions
+    were grouped into snapshots."""
+    seed, j = int(seed), int(j)
+    return (seed * 0x9E3779B97F4A7C15 + j * 0xD1B54A32D192ED03 + 1) & M64
 
 
 def frpca(A, k, q, rng=None, s=5, omega=None):
@@ -410,7 +411,7 @@ class Pipeline:
             c0, w = self.starts[j], self.widths[j]
             cols = np.flatnonzero(act[c0:c0 + w])
             Ac = self.A_old[j][:, cols]
-            om = sketch(leaf_seed(self.seed, it, j), cols.size, self.k + 5)
+            om = sketch(leaf_seed(self.seed, j), cols.size, self.k + 5)
             U, S, V = frpca(Ac, self.k, self.q, omega=om)
             self.nodes[0][j] = U * S
             if it >= self.lazy_start - 1:        # Tree_u_NC.cpp:158-204

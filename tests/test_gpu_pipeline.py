@@ -80,7 +80,7 @@ def test_pipeline_stagewise(capi, directed, prec, mode):
             # leaf j vs the oracle frPCA of the same compacted block and sketch
             cols = np.flatnonzero(act[starts[j]:starts[j] + widths[j]])
             Ac = Aj[:, cols]
-            om = orc.sketch(orc.leaf_seed(seed, it, j), cols.size, k + 5, fp32=fp32)
+            om = orc.sketch(orc.leaf_seed(seed, j), cols.size, k + 5, fp32=fp32)
             U, S, V = orc.frpca(Ac, k, q, omega=om)
             leaf = pipe.node(0, j)
             ref = (U * S) @ (U * S).T

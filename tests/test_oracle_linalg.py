@@ -269,7 +269,7 @@ def test_whole_tree_exact_when_rank_fits():
     leaves = []
     for j in range(4):
         B = A[:, starts[j]:starts[j] + widths[j]]
-        U, S, _ = orc.frpca(B, k, 6, omega=orc.sketch(orc.leaf_seed(1, 0, j), B.shape[1], k + 5))
+        U, S, _ = orc.frpca(B, k, 6, omega=orc.sketch(orc.leaf_seed(1, j), B.shape[1], k + 5))
         leaves.append(U * S)
     lv = [orc.merge(leaves[0:2], k)[0], orc.merge(leaves[2:4], k)[0]]
     _, U, S = orc.merge(lv, k)

@@ -142,7 +142,7 @@ extern "C" int tsvd_pipeline_snapshot(tsvd_pipeline *p, const int32_t *src, cons
         if (!factor.empty()) {
             std::vector<double> sig2;
             frpca_blocks(p->fw, b, factor, p->tree.level[0].p, p->gemm, *p->eig,
-                         c.seed + 0x1000003ull * (uint64_t)it, st, sig2);
+                         c.seed, st, sig2);
             if (it >= c.lazy_start - 1)
                 for (size_t i = 0; i < factor.size(); i++) {
                     const int32_t j = factor[i];

@@ -212,8 +212,10 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                                                     uint32_t *__restrict__ hub_counts, uint32_t hub_cap,
                                                     uint32_t chunk_cap)
 {
-    __shared__ unsigned long long sstat[4];
-    if (threadIdx.x < 4) sstat[threadIdx.x] = 0ull;
+    // [0] pops, [1] relaxations (lanes x degree), [2] items, [3] row visits,
+    // [4] relaxations done here (split rows hand the rest to k_push_hub)
+    __shared__ unsigned long long sstat[5];
+    if (threadIdx.x < 5) sstat[threadIdx.x] = 0ull;
     __syncthreads();
     const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -296,12 +298,13 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                     jend = HUB_CHUNK;
                 }
             }
+            if (lane == 0) atomicAdd(&sstat[4], (unsigned long long)__popcll(pm) * (unsigned long long)jend);
             relax_range(nb, 0, jend, pushing, delta, R + s, s_pad, deg, (uint32_t)ngroups, g, rm, lane, nm, next,
                         next_count, cap, overflow);
         }
     }
     __syncthreads();
-    if (threadIdx.x < 4 && sstat[threadIdx.x]) atomicAdd(stats + threadIdx.x, sstat[threadIdx.x]);
+    if (threadIdx.x < 5 && sstat[threadIdx.x]) atomicAdd(stats + threadIdx.x, sstat[threadIdx.x]);
 }
 
 
@@ -581,7 +584,7 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
     Timer tm;
     tm.start(st);
     int64_t rounds = 0;
-    unsigned long long prof_prev[2] = {0, 0};
+    unsigned long long prof_prev[3] = {0, 0, 0};
     while (count > 0) {
         TSVD_HIP(hipMemsetAsync(cnt + 2, 0, sizeof(uint32_t), st));
         const bool pull = s->ngroups <= 64 && s->pull_frac > 0.0 && (double)count > s->pull_frac * (double)s->n;
@@ -636,17 +639,22 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
         }
         TSVD_HIP(hipGetLastError());
         TSVD_HIP(hipMemcpyAsync(h + 2, cnt + 2, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        if (g_profile) TSVD_HIP(hipMemcpyAsync(h + 4, s->stats.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        if (g_profile)
+            TSVD_HIP(hipMemcpyAsync(h + 16, s->stats.p, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
         TSVD_HIP(hipStreamSynchronize(st));
         if (g_profile) {
             // algorithmic bytes per relaxation (column id + residue RMW) and per pop
-            // (row offsets, residue read/clear, reserve RMW)
-            const unsigned long long *hs = (const unsigned long long *)(h + 4);
+            // (row offsets, residue read/clear, reserve RMW); relaxations of split
+            // rows beyond their first chunk belong to k_push_hub
+            const unsigned long long *hs = (const unsigned long long *)(h + 16);
             const double relax = (double)(hs[1] - prof_prev[1]), pops = (double)(hs[0] - prof_prev[0]);
+            const double own = pull ? relax : (double)(hs[4] - prof_prev[2]);
             const double per_relax = sizeof(T) == 8 ? 20.0 : 12.0, per_pop = sizeof(T) == 8 ? 48.0 : 32.0;
-            prof_add_work(pull ? "ppr_pull" : "ppr_push_round", per_relax * relax + per_pop * pops, 0.0);
+            prof_add_work(pull ? "ppr_pull" : "ppr_push_round", per_relax * own + per_pop * pops, 0.0);
+            if (relax > own) prof_add_work("ppr_push_hub", per_relax * (relax - own), 0.0);
             prof_prev[0] = hs[0];
             prof_prev[1] = hs[1];
+            prof_prev[2] = hs[4];
         }
         TSVD_REQUIRE(h[3] == 0, "frontier overflow in push round %lld", (long long)rounds);
         std::swap(s->cm[d], s->nm);
@@ -749,9 +757,9 @@ extern "C" int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t ns
             }
             TSVD_HIP(hipMemsetAsync(s->nm.p, 0, s->nm.bytes(), st));
             s->counters.alloc(8);
-            s->stats.alloc(4);
+            s->stats.alloc(5);
             TSVD_HIP(hipMemsetAsync(s->counters.p, 0, s->counters.bytes(), st));
-            TSVD_HIP(hipHostMalloc((void **)&s->h_counters, 64, hipHostMallocDefault));
+            TSVD_HIP(hipHostMalloc((void **)&s->h_counters, 128, hipHostMallocDefault));
             TSVD_HIP(hipStreamSynchronize(st));
         } catch (...) {
             delete s;

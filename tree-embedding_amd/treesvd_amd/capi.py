@@ -30,6 +30,8 @@ EXPORTS = [
     "tsvd_pipeline_embedding", "tsvd_pipeline_root_sigma", "tsvd_pipeline_graph", "tsvd_pipeline_ppr",
     "tsvd_pipeline_block_matrix", "tsvd_pipeline_block_norms", "tsvd_pipeline_node",
     "tsvd_pipeline_dense_gram", "tsvd_pipeline_gram_apply",
+    "tsvd_server_create", "tsvd_server_destroy", "tsvd_server_submit", "tsvd_server_wait",
+    "tsvd_server_pipeline", "tsvd_server_edge_budget",
 ]
 
 
@@ -66,6 +68,22 @@ class SnapshotStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_step_edges", ctypes.c_int64), ("max_step_requests", ctypes.c_int32)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_int64), ("snapshot", ctypes.c_int32), ("step_requests", ctypes.c_int32),
+                ("step_edges", ctypes.c_int64), ("last_ticket", ctypes.c_int64), ("submit_ms", ctypes.c_double),
+                ("start_ms", ctypes.c_double), ("done_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
+                ("step", SnapshotStats)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "step"}
+        d["step"] = self.step.as_dict()
+        return d
 
 
 _lib = None
@@ -120,6 +138,14 @@ def lib():
             L.tsvd_pipeline_node.argtypes = [vp, i32, i32, vp]
             L.tsvd_pipeline_dense_gram.argtypes = [vp, vp]
             L.tsvd_pipeline_gram_apply.argtypes = [vp, vp, i32, vp]
+            L.tsvd_server_create.argtypes = [vp, vp, vp]
+            L.tsvd_server_destroy.argtypes = [vp]
+            L.tsvd_server_submit.argtypes = [vp, vp, vp, i64, vp]
+            L.tsvd_server_wait.argtypes = [vp, i64, vp, vp]
+            L.tsvd_server_pipeline.restype = vp
+            L.tsvd_server_pipeline.argtypes = [vp]
+            L.tsvd_server_edge_budget.restype = i64
+            L.tsvd_server_edge_budget.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -308,21 +334,26 @@ class Pipeline:
     """The plain per-snapshot method (tsvd_pipeline_*), one handle."""
 
     def __init__(self, n, directed, sources, alpha, rmax, nparts, fanout, rank, q=6, precision=TSVD_FP32,
-                 transform=TSVD_TRANSFORM_NC, delta=0.65, dyn_start=2, lazy_start=2, slack=0.1, seed=0):
+                 transform=TSVD_TRANSFORM_NC, delta=0.65, dyn_start=2, lazy_start=2, slack=0.1, seed=0,
+                 handle=None):
         self.sources = np.ascontiguousarray(sources, dtype=np.int32)
         self.nsrc, self.rank, self.n, self.nparts = self.sources.size, rank, n, nparts
         self.cfg = Config(n, int(directed), self.nsrc, self.sources.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                           alpha, rmax, precision, nparts, fanout, rank, q, transform, delta, dyn_start, lazy_start,
                           slack, seed)
-        h = ctypes.c_void_p()
-        check(lib().tsvd_pipeline_create(ctypes.byref(self.cfg), ctypes.byref(h)))
+        self.owned = handle is None
+        if handle is None:
+            h = ctypes.c_void_p()
+            check(lib().tsvd_pipeline_create(ctypes.byref(self.cfg), ctypes.byref(h)))
+        else:
+            h = handle
         self.h = h
         self.graph = Graph(n, directed, handle=ctypes.c_void_p(lib().tsvd_pipeline_graph(h)))
         self.ppr = PPR(self.graph, self.sources, alpha, rmax, precision,
                        handle=ctypes.c_void_p(lib().tsvd_pipeline_ppr(h)))
 
     def close(self):
-        if self.h:
+        if self.h and self.owned:
             lib().tsvd_pipeline_destroy(self.h)
         self.h = None
 
@@ -398,3 +429,57 @@ def profile_read():
     buf = ctypes.create_string_buffer(1 << 16)
     check(lib().tsvd_profile_read(buf, 1 << 16))
     return {k: (int(v[0]), float(v[1]), float(v[2]), float(v[3])) for k, v in json.loads(buf.value.decode()).items()}
+
+
+class Server:
+    """The served form (tsvd_server_*): continuous batching of insertion
+    batches over one resident pipeline.  max_step_requests=1 is the plain
+    per-snapshot method under the same arrival stream."""
+
+    def __init__(self, n, directed, sources, alpha, rmax, nparts, fanout, rank, q=6, precision=TSVD_FP32,
+                 transform=TSVD_TRANSFORM_NC, delta=0.65, dyn_start=2, lazy_start=2, slack=0.1, seed=0,
+                 max_step_edges=0, max_step_requests=0):
+        self.sources = np.ascontiguousarray(sources, dtype=np.int32)
+        self.nsrc, self.rank = self.sources.size, rank
+        self.cfg = Config(n, int(directed), self.nsrc, self.sources.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                          alpha, rmax, precision, nparts, fanout, rank, q, transform, delta, dyn_start, lazy_start,
+                          slack, seed)
+        self.scfg = ServerConfig(int(max_step_edges), int(max_step_requests))
+        h = ctypes.c_void_p()
+        check(lib().tsvd_server_create(ctypes.byref(self.cfg), ctypes.byref(self.scfg), ctypes.byref(h)))
+        self.h = h
+        self.pipeline = Pipeline(n, directed, self.sources, alpha, rmax, nparts, fanout, rank, q, precision,
+                                 transform, delta, dyn_start, lazy_start, slack, seed,
+                                 handle=ctypes.c_void_p(lib().tsvd_server_pipeline(h)))
+
+    @property
+    def edge_budget(self):
+        return int(lib().tsvd_server_edge_budget(self.h))
+
+    def submit(self, src, dst):
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        dst = np.ascontiguousarray(dst, dtype=np.int32)
+        if src.shape != dst.shape:
+            raise ValueError("src and dst differ in length")
+        t = ctypes.c_int64()
+        check(lib().tsvd_server_submit(self.h, _p(src), _p(dst), int(src.size), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket, embedding=True):
+        E = np.zeros((self.nsrc, self.rank)) if embedding else None
+        r = RequestResult()
+        check(lib().tsvd_server_wait(self.h, int(ticket), _p(E), ctypes.byref(r)))
+        return E, r.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().tsvd_server_destroy(self.h)
+        self.h = None
+        if getattr(self, "pipeline", None) is not None:
+            self.pipeline.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
