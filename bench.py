@@ -6,8 +6,13 @@ an edge-insertion batch, already resident in HBM, goes through ingest ->
 dynamic PPR update -> push -> block transform + lazy dirty test -> blockwise
 frPCA of the dirty blocks -> dirty-path merges -> root embedding.  The initial
 snapshot (static push over the first 60% of the edges) is setup and is timed
-separately.  Output: one JSON line (snapshots/s, p50/p99 ms per update, the
-dominant kernel's roofline, and the CPU restatement timed on this host).
+separately.  Then the served form is compared with the plain method on one
+request stream (the rest of the graph arriving as lognormal 10k-edge batches
+at Poisson times): each replays the stream on its own clock against a server,
+plain admitting one request per snapshot, served admitting every queued
+request into the next snapshot.  Output: one JSON line (snapshots/s, p50/p99
+ms per update, served vs plain, the dominant kernel's roofline, and the CPU
+restatement timed on this host).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batched]
 """
@@ -235,6 +240,52 @@ def accuracy_report(pipe, w, seed=7):
             "seconds": {"oracle": round(orac.seconds, 1), "reference_algorithm": round(t_ref, 1)}}
 
 
+def served_run(capi, w, reqs, arrivals, cap, label, seed=7):
+    """Replays a request stream against a server on its own clock: a feeder
+    thread submits request i at arrivals[i] seconds, this thread collects the
+    results in order.  cap = 1 is the plain per-snapshot method, cap = 0 the
+    served form (every queued request admitted into the next step)."""
+    import threading
+    srv = capi.Server(w.n, w.directed, w.subset, w.alpha, w.rmax, w.nparts, w.fanout, w.rank, w.q,
+                      precision=capi.TSVD_FP64 if w.fp64_state else capi.TSVD_FP32,
+                      transform=capi.TSVD_TRANSFORM_NC if w.transform == "NC" else capi.TSVD_TRANSFORM_LP,
+                      delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed, max_step_requests=cap)
+    t = time.perf_counter()
+    srv.wait(srv.submit(*w.snapshots[0]), embedding=False)
+    log(f"{label}: initial snapshot {time.perf_counter() - t:.1f}s")
+    tickets = [None] * len(reqs)
+    ready = [threading.Event() for _ in reqs]
+    start = time.perf_counter()
+
+    def feeder():
+        for i, (s, d) in enumerate(reqs):
+            delay = start + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[i] = srv.submit(s, d)
+            ready[i].set()
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    res = []
+    for i in range(len(reqs)):
+        ready[i].wait()
+        _, r = srv.wait(tickets[i], embedding=False)
+        res.append(r)
+        if r["last_ticket"] == r["ticket"]:
+            log(f"{label}: step {r['snapshot']} done: {r['step_requests']} requests, {r['step_edges']} edges, "
+                f"{r['done_ms'] - r['start_ms']:.0f} ms, latency of request {i}: {r['latency_ms']:.0f} ms")
+    th.join()
+    srv.close()
+    lat = np.array([r["latency_ms"] for r in res])
+    span = (max(r["done_ms"] for r in res) - min(r["submit_ms"] for r in res)) / 1e3
+    steps = sorted({r["snapshot"] for r in res})
+    return {"snapshots_per_s": round(len(res) / span, 4), "p50_ms": round(float(np.percentile(lat, 50)), 1),
+            "p99_ms": round(float(np.percentile(lat, 99)), 1), "mean_ms": round(float(lat.mean()), 1),
+            "steps": len(steps), "requests_per_step": round(len(res) / len(steps), 2),
+            "makespan_s": round(span, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,6 +296,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--accuracy", action="store_true",
                     help="append errors against a dense FP64 SVD and the reference algorithm's own error")
+    ap.add_argument("--served-requests", type=int, default=24,
+                    help="requests in the served-vs-plain stream (0: skip that comparison)")
+    ap.add_argument("--served-rate", type=float, default=4.0, help="Poisson arrival rate of that stream (1/s)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="first run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -311,7 +365,7 @@ def main():
         roof["traffic"] = profiled_traffic("k_push_round")
         roof["traffic_source"] = f"profiles/push_round_{WORKLOAD_TAG}_hbm_traffic.md"
     out = {
-        "metric": "update->embedding snapshots/s (plain GPU method), p50/p99 ms per update",
+        "metric": "update->embedding snapshots/s (plain GPU method), p50/p99 ms per update; served vs plain",
         "value": round(value, 4), "unit": "snapshots/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(total_s / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -330,6 +384,23 @@ def main():
     if args.accuracy:
         log("accuracy against the dense FP64 oracle ...")
         out["accuracy"] = accuracy_report(pipe, w)
+    pipe.close()
+    for b in batches:
+        for x in b:
+            x.free()
+    if args.served_requests > 0:
+        reqs, arr = gen.request_stream(w, args.served_requests, args.served_rate, seed=11)
+        log(f"served vs plain: {len(reqs)} requests ({sum(s.size for s, _ in reqs)} edges, lognormal median 10k) "
+            f"at {args.served_rate}/s Poisson")
+        plain = served_run(capi, w, reqs, arr, 1, "plain (one snapshot per request)")
+        served = served_run(capi, w, reqs, arr, 0, "served (continuous batching)")
+        out["served_vs_plain"] = {
+            "stream": {"requests": len(reqs), "edges": int(sum(s.size for s, _ in reqs)),
+                       "rate_per_s": args.served_rate, "sizes": "lognormal median 10k, sigma 1, clip [1k, 1M]",
+                       "graph": f"{args.workload} workload after its initial snapshot"},
+            "plain": plain, "served": served,
+            "throughput_ratio": round(served["snapshots_per_s"] / plain["snapshots_per_s"], 2),
+            "p99_ratio": round(served["p99_ms"] / plain["p99_ms"], 4)}
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(w, ndirty, nmerged, args.steps)

@@ -96,6 +96,12 @@ int tsvd_ppr_push(tsvd_ppr *s, int dir, int32_t src_begin, int32_t src_end);
 /* Host copy of p or r (what = 0: reserve p, 1: residue r) as FP64, source-
  * major [nsrc][n] like the reference's float** pi / residue.              */
 int tsvd_ppr_read(tsvd_ppr *s, int dir, int what, double *out);
+/* Threshold side of refresh and push: 0 (default) pushes r > rmax only, as the
+ * reference does (ppr:1445, 2981, 3390); 1 pushes |r| > rmax, which also
+ * drains the negative residues the insertion update leaves, so reserves stay
+ * within the push error bound of the exact PPR after dynamic updates.  No
+ * reference counterpart. */
+int tsvd_ppr_set_two_sided(tsvd_ppr *s, int on);
 /* Counters of the last push on `dir`: rounds, popped (vertex, source) pairs,
  * edge relaxations (pairs x degree), and the push kernel time in ms. */
 int tsvd_ppr_last_push_stats(tsvd_ppr *s, int dir, int64_t *rounds, int64_t *pops, int64_t *relax,
@@ -143,6 +149,7 @@ typedef struct {
     int32_t lazy_start;   /* first snapshot with the dirty test            */
     double slack;         /* CSR append slack fraction                     */
     uint64_t seed;        /* sketch seed                                   */
+    int32_t two_sided;    /* 0: push r > rmax (reference), 1: |r| > rmax   */
 } tsvd_config;
 
 typedef struct {

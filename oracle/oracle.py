@@ -52,6 +52,7 @@ def lib():
         L.orc_ppr_refresh_dynamic.argtypes = [vp, vp, i32, i32]
         L.orc_ppr_push.argtypes = [vp, vp, i32, i32]
         L.orc_ppr_counts.argtypes = [vp, vp, vp]
+        L.orc_ppr_set_two_sided.argtypes = [vp, i32]
         _lib = L
     return _lib
 
@@ -109,7 +110,7 @@ class PPR:
     """Per-source reserves/residues and the FP64 matrix accumulator of the
     drivers (Tree_u_NC.cpp:459-489, d_row_tree_mkl::dense_mat_mapping)."""
 
-    def __init__(self, n, sources, directed, alpha, rmax, fp32=True, quirk_int_ratio=False):
+    def __init__(self, n, sources, directed, alpha, rmax, fp32=True, quirk_int_ratio=False, two_sided=False):
         self.n, self.directed = n, directed
         self.sources = np.ascontiguousarray(sources, dtype=np.int32)
         self.nsrc = self.sources.size
@@ -118,6 +119,7 @@ class PPR:
                                    alpha, rmax, int(fp32), int(quirk_int_ratio))
         if not self.h:
             raise MemoryError("oracle PPR state")
+        lib().orc_ppr_set_two_sided(self.h, int(two_sided))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -347,9 +349,9 @@ class Pipeline:
 
     def __init__(self, n, directed, sources, alpha, rmax, nparts, fanout, k, q,
                  mode="NC", delta=0.65, dyn_start=2, lazy_start=2, fp32=True,
-                 seed=0, nthreads=os.cpu_count() or 1):
+                 seed=0, nthreads=os.cpu_count() or 1, two_sided=False):
         self.g = Graph(n, directed)
-        self.ppr = PPR(n, sources, directed, alpha, rmax, fp32=fp32)
+        self.ppr = PPR(n, sources, directed, alpha, rmax, fp32=fp32, two_sided=two_sided)
         self.sources = np.asarray(sources)
         self.n, self.directed, self.rmax = n, directed, rmax
         self.nparts, self.fanout, self.k, self.q = nparts, fanout, k, q

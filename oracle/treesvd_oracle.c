@@ -170,6 +170,7 @@ ORC_EXPORT long long orc_read_edge_file(const char *path, int dup_last, int *src
 
 typedef struct {
     int n, nsrc, ndir, fp32, quirk;
+    int two;                    /* 1: push |r| > rmax (negative residues too) */
     double alpha, rmax;
     int *src;
     double *p[2], *r[2];        /* [nsrc * n], row k = source k */
@@ -181,6 +182,9 @@ typedef struct {
 } orc_ppr;
 
 #define ST(x) (s->fp32 ? (double)(float)(x) : (x))
+/* Residue beyond the push threshold: r > rmax (the reference, positive side
+ * only), or |r| > rmax with two-sided pushing. */
+#define OUT(x) ((x) > rmax || (s->two && (x) < -rmax))
 #define IDX(k, v) ((size_t)(k) * (size_t)s->n + (size_t)(v))
 
 ORC_EXPORT orc_ppr *orc_ppr_new(int n, int nsrc, const int *src, int directed,
@@ -291,8 +295,8 @@ static void *refresh_job(void *arg)
                 else p[i] = ST(p[i] * ((double)(j + 1) / (double)j));
                 r[i] = ST(r[i] - p[i] / (j + 1) / a);
                 r[to] = ST(r[to] + (1 - a) * p[i] / (j + 1) / a);
-                if (r[i] > rmax && !fl[i]) { q_push(s, d, k, i); fl[i] = 1; }
-                if (r[to] > rmax && !fl[to]) { q_push(s, d, k, to); fl[to] = 1; }
+                if (OUT(r[i]) && !fl[i]) { q_push(s, d, k, i); fl[i] = 1; }
+                if (OUT(r[to]) && !fl[to]) { q_push(s, d, k, to); fl[to] = 1; }
             }
         }
     }
@@ -330,7 +334,7 @@ static void *push_job(void *arg)
                 if (!g->directed) p[v] = ST(a * r[v]);
                 continue;
             }
-            if (r[v] > rmax) {
+            if (OUT(r[v])) {
                 pops++;
                 relax += deg[v];
                 const int *nb = g->adj[d][v];
@@ -344,7 +348,7 @@ static void *push_job(void *arg)
                         }
                         continue;
                     }
-                    if (r[u] > rmax && !fl[u]) { q_push(s, d, k, u); fl[u] = 1; }
+                    if (OUT(r[u]) && !fl[u]) { q_push(s, d, k, u); fl[u] = 1; }
                 }
                 p[v] = ST(p[v] + a * r[v]);
                 M[v] += a * r[v];
@@ -357,6 +361,14 @@ static void *push_job(void *arg)
     __sync_fetch_and_add(&s->pops, pops);
     __sync_fetch_and_add(&s->relax, relax);
     return NULL;
+}
+
+/* Two-sided pushing (|r| > rmax): an option beside the reference's positive-
+ * only threshold (ppr:1445); the dynamic update leaves negative residues,
+ * which only a two-sided push drains. */
+ORC_EXPORT void orc_ppr_set_two_sided(orc_ppr *s, int on)
+{
+    s->two = on ? 1 : 0;
 }
 
 /* Pushes (vertices popped with r > rmax) and edge relaxations since the last

@@ -40,9 +40,10 @@ def dirs(directed):
     return (0, 1) if directed else (0,)
 
 
+@pytest.mark.parametrize("two", [False, True], ids=["one-sided", "two-sided"])
 @pytest.mark.parametrize("directed", [False, True])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_gpu_push_tiny_exact_invariant(capi, directed, prec):
+def test_gpu_push_tiny_exact_invariant(capi, directed, prec, two):
     n = 48
     u, v = random_graph(n, 150, 7, directed)
     g = capi.Graph(n, directed); g.append(u[:110], v[:110])
@@ -50,6 +51,7 @@ def test_gpu_push_tiny_exact_invariant(capi, directed, prec):
     src = pick_sources(deg, 10, 7)
     rmax = 1e-5
     ppr = capi.PPR(g, src, ALPHA, rmax, precision=prec)
+    ppr.set_two_sided(two)
     tol = 1e-12 if prec == 1 else 2e-6
     for snap in range(2):
         if snap == 1:
@@ -62,11 +64,17 @@ def test_gpu_push_tiny_exact_invariant(capi, directed, prec):
             p, r = ppr.p(d), ppr.r(d)
             dd = np.diff(rp)
             assert (r[:, dd > 0] <= rmax * (1 + 1e-6)).all()
+            if two:
+                assert (r[:, dd > 0] >= -rmax * (1 + 1e-6)).all()
             assert np.abs(Pi[src] - p - r @ Pi).max() < tol
 
 
+@pytest.mark.parametrize("two", [False, True], ids=["one-sided", "two-sided"])
 @pytest.mark.parametrize("directed", [False, True])
-def test_gpu_push_matches_oracle_multi_snapshot(capi, directed):
+def test_gpu_push_matches_oracle_multi_snapshot(capi, directed, two):
+    """With the reference's positive-side threshold the insertion update leaves
+    negative residues, so the entrywise bound is loose; two-sided pushing
+    keeps |r| <= rmax and the bound tight (|p_gpu - p_oracle| <= 2 rmax c)."""
     n = 4000
     u, v = random_graph(n, 40000, 11, directed)
     cuts = [0, 28000, 32000, 36000, u.size]
@@ -76,8 +84,9 @@ def test_gpu_push_matches_oracle_multi_snapshot(capi, directed):
     deg, _ = g.degrees(0)
     src = pick_sources(deg, 70, 11)        # two source groups, one partial
     ppr = capi.PPR(g, src, ALPHA, rmax, precision=capi.TSVD_FP64)
+    ppr.set_two_sided(two)
     og = orc.Graph(n, directed); og.append(u[:cuts[1]], v[:cuts[1]])
-    opp = orc.PPR(n, src, directed, ALPHA, rmax, fp32=False)
+    opp = orc.PPR(n, src, directed, ALPHA, rmax, fp32=False, two_sided=two)
     for k in range(len(cuts) - 1):
         if k > 0:
             g.append(u[cuts[k]:cuts[k + 1]], v[cuts[k]:cuts[k + 1]])
@@ -95,6 +104,8 @@ def test_gpu_push_matches_oracle_multi_snapshot(capi, directed):
             p, r = ppr.p(d), ppr.r(d)
             dg = np.diff(rp)
             assert (r[:, dg > 0] <= rmax * (1 + 1e-9)).all()
+            if two:
+                assert (r[:, dg > 0] >= -rmax * (1 + 1e-9)).all()
             mv = reserve_mass(n, rp, col, ALPHA)
             assert np.abs(p.sum(1) + r @ mv - mv[src]).max() < 1e-9
             c = column_mass(n, rp, col, ALPHA)

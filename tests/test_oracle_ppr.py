@@ -104,3 +104,60 @@ def test_mass_conservation_larger_graph():
         p, r = ppr.p(d), ppr.r(d)
         lhs = p.sum(1) + r @ mv
         assert np.abs(lhs - mv[src]).max() < 1e-10
+
+
+def _updated_state(n, u, v, cut, directed, rmax, two_sided, groups):
+    """Static push on the first `cut` edges, then the rest inserted in
+    `groups` consecutive batches (update + push after each)."""
+    g = orc.Graph(n, directed); g.append(u[:cut], v[:cut])
+    src = sources_with_edges(g, 6, 11)
+    ppr = orc.PPR(n, src, directed, ALPHA, rmax, fp32=False, two_sided=two_sided)
+    ppr.init_static(); ppr.push(g)
+    for part in np.array_split(np.arange(cut, u.size), groups):
+        g.append(u[part], v[part])
+        ppr.refresh_dynamic(g)
+        ppr.push(g)
+    return g, ppr
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_two_sided_push_drains_negative_residues(directed):
+    """The insertion update lowers r at the updated row; the reference pushes
+    only r > rmax (ppr:1445), so those negative residues stay and the reserves
+    drift from the exact PPR by sum_v r(v) pi_v.  Two-sided pushing (|r| >
+    rmax) drains them: |r| <= rmax everywhere and, undirected, |p - pi| <=
+    rmax * deg entrywise."""
+    n, rmax = 60, 1e-6
+    u, v = random_graph(n, 300, 12, directed)
+    for two in (False, True):
+        g, ppr = _updated_state(n, u, v, 150, directed, rmax, two, 3)
+        for d in ppr.dirs():
+            assert invariant_residual(ppr, g, d, directed) < 1e-12
+            deg, _ = g.degrees(d)
+            r = ppr.r(d)[:, deg > 0]
+            rp, col = g.csr(d)
+            pi, _ = orc.exact_ppr(n, rp, col, ALPHA, ppr.sources, directed)
+            err = np.abs(ppr.p(d) - pi)
+            if two:
+                assert np.abs(r).max() <= rmax * (1 + 1e-12)
+                if not directed:
+                    assert (err <= rmax * np.maximum(deg, 1)[None, :] + 1e-15).all()
+            else:
+                assert r.min() < -100 * rmax          # left in place by the positive-side push
+                assert err.max() > 1e3 * rmax * deg.max()
+
+
+def test_two_sided_state_does_not_depend_on_batch_grouping():
+    """Inserting the same edges in one batch or in several gives reserves
+    within 2 rmax deg of each other with two-sided pushing; with the
+    reference's positive-side push the grouping leaves a much larger gap."""
+    n, rmax = 80, 1e-7
+    u, v = random_graph(n, 400, 13, False)
+    gaps = {}
+    for two in (False, True):
+        _, a = _updated_state(n, u, v, 200, False, rmax, two, 1)
+        g, b = _updated_state(n, u, v, 200, False, rmax, two, 5)
+        deg, _ = g.degrees(0)
+        gaps[two] = (np.abs(a.p(0) - b.p(0)) / (rmax * np.maximum(deg, 1))[None, :]).max()
+    assert gaps[True] <= 2.0
+    assert gaps[False] > 1e3

@@ -62,6 +62,7 @@ extern "C" int tsvd_pipeline_create(const tsvd_config *cfg, tsvd_pipeline **out)
             if (rc) throw Error{rc};
             rc = tsvd_ppr_create(p->g, cfg->sources, cfg->nsrc, cfg->alpha, cfg->rmax, cfg->precision, &p->ppr);
             if (rc) throw Error{rc};
+            p->ppr->two_sided = cfg->two_sided ? 1 : 0;
             hipStream_t st = p->g->stream;
             blocks_setup(p->blocks, cfg->n, cfg->nparts, cfg->nsrc, p->ppr->s_pad, cfg->precision, cfg->transform,
                          cfg->rmax, p->sources, st);

@@ -56,6 +56,14 @@ __device__ inline void append_item(uint32_t item, uint32_t *list, uint32_t *coun
     else atomicOr(overflow, 1u);
 }
 
+// A residue beyond the push threshold: r > rmax (the reference's positive
+// side, ppr:1445), or |r| > rmax when pushing two-sided.
+template <class T>
+__device__ inline bool beyond(T x, T rm, int two)
+{
+    return x > rm || (two && x < -rm);
+}
+
 // Lane-0 sets `bits` in mask[item]; the first setter appends the item.
 __device__ inline void mark(unsigned long long bits, uint32_t item, unsigned long long *mask, uint32_t *list,
                             uint32_t *count, uint32_t cap, uint32_t *overflow)
@@ -96,7 +104,7 @@ __global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict
                                                 int32_t ntouched, const int64_t *__restrict__ rowoff,
                                                 const int32_t *__restrict__ adj, const int32_t *__restrict__ deg,
                                                 const int32_t *__restrict__ former, int32_t s_pad, int32_t ngroups,
-                                                int32_t s0, int32_t s1, double alpha, T rm,
+                                                int32_t s0, int32_t s1, double alpha, T rm, int two,
                                                 unsigned long long *__restrict__ cm, uint32_t *__restrict__ list,
                                                 uint32_t *__restrict__ count, uint32_t cap,
                                                 uint32_t *__restrict__ overflow)
@@ -124,7 +132,7 @@ __global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict
             atomicAdd(Ri, (T)(-pj));
             const T add = (T)((1.0 - alpha) * pj);
             T old = atomicAdd(R + (int64_t)to * s_pad + s, add);
-            cross = old <= rm && old + add > rm;
+            cross = !beyond(old, rm, two) && beyond(old + add, rm, two);
         }
         unsigned long long c = __ballot(cross);
         if (c && lane == 0 && deg[to] > 0)
@@ -134,7 +142,7 @@ __global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict
     if (active) {
         *Pi = pv;
         T x = atomicAdd(Ri, T(0));
-        self = x > rm && d1 > 0;
+        self = beyond(x, rm, two) && d1 > 0;
     }
     unsigned long long c2 = __ballot(self);
     if (c2 && lane == 0) mark(c2, (uint32_t)i * ngroups + g, cm, list, count, cap, overflow);
@@ -146,7 +154,7 @@ __global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict
 template <class T>
 __device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, int32_t j1, bool pushing, T delta,
                                    T *__restrict__ Rs, int32_t s_pad, const int32_t *__restrict__ deg, uint32_t G,
-                                   int32_t g, T rm, int lane, unsigned long long *__restrict__ nm,
+                                   int32_t g, T rm, int two, int lane, unsigned long long *__restrict__ nm,
                                    uint32_t *__restrict__ next, uint32_t *__restrict__ next_count, uint32_t cap,
                                    uint32_t *__restrict__ overflow)
 {
@@ -162,7 +170,7 @@ __device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, i
             c[q] = false;
             if (pushing) {
                 const T o = atomicAdd(Rs + (int64_t)u[q] * s_pad, delta);
-                c[q] = o <= rm && o + delta > rm;
+                c[q] = !beyond(o, rm, two) && beyond(o + delta, rm, two);
             }
         }
 #pragma unroll
@@ -176,7 +184,7 @@ __device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, i
         bool c = false;
         if (pushing) {
             const T o = atomicAdd(Rs + (int64_t)u * s_pad, delta);
-            c = o <= rm && o + delta > rm;
+            c = !beyond(o, rm, two) && beyond(o + delta, rm, two);
         }
         const unsigned long long b = __ballot(c);
         if (b && lane == 0 && deg[u] > 0) mark(b, (uint32_t)u * G + g, nm, next, next_count, cap, overflow);
@@ -204,8 +212,8 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                                                     uint32_t *__restrict__ overflow,
                                                     const int64_t *__restrict__ rowoff,
                                                     const int32_t *__restrict__ adj, const int32_t *__restrict__ deg,
-                                                    int32_t s_pad, int32_t ngroups, double alpha, T rm, int directed,
-                                                    T ride_min, int32_t ride_s0, int32_t ride_s1,
+                                                    int32_t s_pad, int32_t ngroups, double alpha, T rm, int two,
+                                                    int directed, T ride_min, int32_t ride_s0, int32_t ride_s1,
                                                     unsigned long long *__restrict__ stats,
                                                     uint32_t *__restrict__ hub_item, T *__restrict__ hub_delta,
                                                     unsigned long long *__restrict__ hub_chunks,
@@ -234,9 +242,9 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
         bool pushing = false, again = false;
         if (active) {
             x = xchg(Rv, T(0));
-            if (!(x > rm)) {
-                const T old = atomicAdd(Rv, x);                  // not above rmax: give it back
-                again = old <= rm && old + x > rm && dv > 0;
+            if (!beyond(x, rm, two)) {
+                const T old = atomicAdd(Rv, x);                  // within the threshold: give it back
+                again = !beyond(old, rm, two) && beyond(old + x, rm, two) && dv > 0;
             } else if (dv == 0) {
                 if (!directed) *Pv = (T)(alpha * (double)x);     // ppr:1431-1441
                 atomicAdd(Rv, x);                                // residue kept, not re-queued
@@ -252,15 +260,21 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
             // The row is walked anyway: lanes whose residue is positive but at or
             // below rmax ride along (same atomics, no extra traffic).  They only
             // lower residues further, so the invariant and r <= rmax still hold.
+            bool back = false;
             if (!active && s >= ride_s0 && s < ride_s1 && *Rv > ride_min) {
                 x = xchg(Rv, T(0));
                 if (x > T(0)) {
                     pushing = true;
                     *Pv = (T)((double)*Pv + alpha * (double)x);
                 } else {
-                    atomicAdd(Rv, x);
+                    // a negative push landed between the read and the exchange
+                    // (two-sided only): give it back, re-queue on a crossing
+                    const T old = atomicAdd(Rv, x);
+                    back = !beyond(old, rm, two) && beyond(old + x, rm, two);
                 }
             }
+            const unsigned long long cb = __ballot(back);
+            if (cb && lane == 0) mark(cb, item, nm, next, next_count, cap, overflow);
             pm = __ballot(pushing);
         }
         if (pm != 0ull) {
@@ -299,7 +313,7 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                 }
             }
             if (lane == 0) atomicAdd(&sstat[4], (unsigned long long)__popcll(pm) * (unsigned long long)jend);
-            relax_range(nb, 0, jend, pushing, delta, R + s, s_pad, deg, (uint32_t)ngroups, g, rm, lane, nm, next,
+            relax_range(nb, 0, jend, pushing, delta, R + s, s_pad, deg, (uint32_t)ngroups, g, rm, two, lane, nm, next,
                         next_count, cap, overflow);
         }
     }
@@ -318,7 +332,7 @@ __global__ void __launch_bounds__(TPB) k_push_hub(T *__restrict__ R, const uint3
                                                   uint32_t *__restrict__ overflow,
                                                   const int64_t *__restrict__ rowoff,
                                                   const int32_t *__restrict__ adj, const int32_t *__restrict__ deg,
-                                                  int32_t s_pad, int32_t ngroups, T rm)
+                                                  int32_t s_pad, int32_t ngroups, T rm, int two)
 {
     const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -331,8 +345,8 @@ __global__ void __launch_bounds__(TPB) k_push_hub(T *__restrict__ R, const uint3
     const T delta = hub_delta[(int64_t)slot * 64 + lane];
     const int32_t dv = deg[v];
     const int32_t j0 = (int32_t)c * HUB_CHUNK, j1 = min(dv, j0 + HUB_CHUNK);
-    relax_range(adj + rowoff[v], j0, j1, delta > T(0), delta, R + (int64_t)g * 64 + lane, s_pad, deg,
-                (uint32_t)ngroups, g, rm, lane, nm, next, next_count, cap, overflow);
+    relax_range(adj + rowoff[v], j0, j1, delta != T(0), delta, R + (int64_t)g * 64 + lane, s_pad, deg,
+                (uint32_t)ngroups, g, rm, two, lane, nm, next, next_count, cap, overflow);
 }
 
 // ---------------------------------------------------------------- pull rounds
@@ -521,8 +535,8 @@ void refresh_t(tsvd_ppr *s, int mode, int d, int32_t s0, int32_t s1)
             PROF("ppr_update", st);
             hipLaunchKernelGGL(k_update<T>, dim3((unsigned)ceil_div(waves, WPB)), dim3(TPB), 0, st, P, R,
                                D.touched.p, D.ntouched, D.rowoff.p, D.adj.p, D.deg.p, D.former.p, s->s_pad,
-                               s->ngroups, s0, s1, s->alpha, (T)s->rmax, s->cm[d].p, s->F[d].p, cnt + d, cap,
-                               cnt + 3);
+                               s->ngroups, s0, s1, s->alpha, (T)s->rmax, s->two_sided, s->cm[d].p, s->F[d].p,
+                               cnt + d, cap, cnt + 3);
         }
     }
     TSVD_HIP(hipGetLastError());
@@ -587,7 +601,9 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
     unsigned long long prof_prev[3] = {0, 0, 0};
     while (count > 0) {
         TSVD_HIP(hipMemsetAsync(cnt + 2, 0, sizeof(uint32_t), st));
-        const bool pull = s->ngroups <= 64 && s->pull_frac > 0.0 && (double)count > s->pull_frac * (double)s->n;
+        // pull rounds implement the reference's positive-side threshold only
+        const bool pull = !s->two_sided && s->ngroups <= 64 && s->pull_frac > 0.0 &&
+                          (double)count > s->pull_frac * (double)s->n;
         if (pull) {
             build_worklist(s, d);
             const GraphDir &Q = pull_dir(s, d);
@@ -622,7 +638,8 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
                 hipLaunchKernelGGL(k_push_round<T>, dim3((unsigned)ceil_div(count, WPB)), dim3(TPB), 0, st, P, R,
                                    s->F[d].p, count, s->cm[d].p, s->nm.p, s->fnext.p, cnt + 2, cap, cnt + 3,
                                    D.rowoff.p, D.adj.p, D.deg.p, s->s_pad, s->ngroups, s->alpha, (T)s->rmax,
-                                   s->directed, (T)(std::min(s->ride_frac, 1.0) * s->rmax), s0, s1, s->stats.p,
+                                   s->two_sided, s->directed, (T)(std::min(s->ride_frac, 1.0) * s->rmax), s0, s1,
+                                   s->stats.p,
                                    s->hub_item.p, (T *)s->hub_delta.p, s->hub_chunks.p,
                                    cnt + 4, hub_cap, chunk_cap);
             }
@@ -634,7 +651,7 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
                 hipLaunchKernelGGL(k_push_hub<T>, dim3((unsigned)ceil_div(nchunks, WPB)), dim3(TPB), 0, st, R,
                                    s->hub_item.p, (const T *)s->hub_delta.p, s->hub_chunks.p, nchunks, s->nm.p,
                                    s->fnext.p, cnt + 2, cap, cnt + 3, D.rowoff.p, D.adj.p, D.deg.p, s->s_pad,
-                                   s->ngroups, (T)s->rmax);
+                                   s->ngroups, (T)s->rmax, s->two_sided);
             }
         }
         TSVD_HIP(hipGetLastError());
@@ -818,5 +835,13 @@ extern "C" int tsvd_ppr_last_push_stats(tsvd_ppr *s, int dir, int64_t *rounds, i
         if (pops) *pops = s->last_pops[dir];
         if (relax) *relax = s->last_relax[dir];
         if (kernel_ms) *kernel_ms = s->last_ms[dir];
+    });
+}
+
+extern "C" int tsvd_ppr_set_two_sided(tsvd_ppr *s, int on)
+{
+    return guard([&] {
+        TSVD_REQUIRE(s, "null ppr");
+        s->two_sided = on ? 1 : 0;
     });
 }

@@ -38,6 +38,7 @@ struct tsvd_ppr {
     tsvd::DBuf<uint32_t> wl_u[2], wl_c[2];   // (vertex, chunk) per push direction
     tsvd::DBuf<int32_t> wl_cnt;              // n + 1 scratch
     int64_t wl_n[2] = {0, 0}, wl_version[2] = {-1, -1};
+    int two_sided = 0;                       // 1: push |r| > rmax, 0: r > rmax (the reference)
     double pull_frac = 0.0;                  // pull when items > pull_frac * n (0: never)
     double ride_frac = 0.0;                  // sub-threshold lanes above ride_frac*rmax ride along (>= 1: off)
     std::vector<int32_t> h_sources;

@@ -24,7 +24,7 @@ EXPORTS = [
     "tsvd_graph_create", "tsvd_graph_destroy", "tsvd_graph_append", "tsvd_graph_num_edges",
     "tsvd_graph_degrees", "tsvd_graph_export_csr", "tsvd_read_edge_file",
     "tsvd_ppr_create", "tsvd_ppr_destroy", "tsvd_ppr_refresh", "tsvd_ppr_push", "tsvd_ppr_read",
-    "tsvd_ppr_last_push_stats",
+    "tsvd_ppr_last_push_stats", "tsvd_ppr_set_two_sided",
     "frPCA", "tsvd_frpca_csr",
     "tsvd_pipeline_create", "tsvd_pipeline_destroy", "tsvd_pipeline_snapshot",
     "tsvd_pipeline_embedding", "tsvd_pipeline_root_sigma", "tsvd_pipeline_graph", "tsvd_pipeline_ppr",
@@ -55,7 +55,8 @@ class Config(ctypes.Structure):
                 ("rmax", ctypes.c_double), ("precision", ctypes.c_int32), ("nparts", ctypes.c_int32),
                 ("fanout", ctypes.c_int32), ("rank", ctypes.c_int32), ("q", ctypes.c_int32),
                 ("transform", ctypes.c_int32), ("delta", ctypes.c_double), ("dyn_start", ctypes.c_int32),
-                ("lazy_start", ctypes.c_int32), ("slack", ctypes.c_double), ("seed", ctypes.c_uint64)]
+                ("lazy_start", ctypes.c_int32), ("slack", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("two_sided", ctypes.c_int32)]
 
 
 class SnapshotStats(ctypes.Structure):
@@ -117,6 +118,7 @@ def lib():
         L.tsvd_ppr_push.argtypes = [vp, ctypes.c_int, i32, i32]
         L.tsvd_ppr_read.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
         L.tsvd_ppr_last_push_stats.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
+        L.tsvd_ppr_set_two_sided.argtypes = [vp, ctypes.c_int]
         if hasattr(L, "frPCA"):
             L.frPCA.restype = None
             L.frPCA.argtypes = [ctypes.POINTER(MatCSR), vp, vp, vp, ctypes.c_int, ctypes.c_int]
@@ -281,6 +283,9 @@ class PPR:
         except Exception:
             pass
 
+    def set_two_sided(self, on=True):
+        check(lib().tsvd_ppr_set_two_sided(self.h, int(on)))
+
     def refresh(self, mode, d=0, s0=0, s1=None):
         check(lib().tsvd_ppr_refresh(self.h, mode, d, s0, self.nsrc if s1 is None else s1))
 
@@ -335,12 +340,12 @@ class Pipeline:
 
     def __init__(self, n, directed, sources, alpha, rmax, nparts, fanout, rank, q=6, precision=TSVD_FP32,
                  transform=TSVD_TRANSFORM_NC, delta=0.65, dyn_start=2, lazy_start=2, slack=0.1, seed=0,
-                 handle=None):
+                 handle=None, two_sided=False):
         self.sources = np.ascontiguousarray(sources, dtype=np.int32)
         self.nsrc, self.rank, self.n, self.nparts = self.sources.size, rank, n, nparts
         self.cfg = Config(n, int(directed), self.nsrc, self.sources.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                           alpha, rmax, precision, nparts, fanout, rank, q, transform, delta, dyn_start, lazy_start,
-                          slack, seed)
+                          slack, seed, int(two_sided))
         self.owned = handle is None
         if handle is None:
             h = ctypes.c_void_p()
@@ -438,19 +443,19 @@ class Server:
 
     def __init__(self, n, directed, sources, alpha, rmax, nparts, fanout, rank, q=6, precision=TSVD_FP32,
                  transform=TSVD_TRANSFORM_NC, delta=0.65, dyn_start=2, lazy_start=2, slack=0.1, seed=0,
-                 max_step_edges=0, max_step_requests=0):
+                 max_step_edges=0, max_step_requests=0, two_sided=False):
         self.sources = np.ascontiguousarray(sources, dtype=np.int32)
         self.nsrc, self.rank = self.sources.size, rank
         self.cfg = Config(n, int(directed), self.nsrc, self.sources.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                           alpha, rmax, precision, nparts, fanout, rank, q, transform, delta, dyn_start, lazy_start,
-                          slack, seed)
+                          slack, seed, int(two_sided))
         self.scfg = ServerConfig(int(max_step_edges), int(max_step_requests))
         h = ctypes.c_void_p()
         check(lib().tsvd_server_create(ctypes.byref(self.cfg), ctypes.byref(self.scfg), ctypes.byref(h)))
         self.h = h
         self.pipeline = Pipeline(n, directed, self.sources, alpha, rmax, nparts, fanout, rank, q, precision,
                                  transform, delta, dyn_start, lazy_start, slack, seed,
-                                 handle=ctypes.c_void_p(lib().tsvd_server_pipeline(h)))
+                                 handle=ctypes.c_void_p(lib().tsvd_server_pipeline(h)), two_sided=two_sided)
 
     @property
     def edge_budget(self):

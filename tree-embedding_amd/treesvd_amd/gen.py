@@ -185,3 +185,26 @@ def workload(name: str, scale_down: int = 0, nsub: int | None = None,
         sub = sample_subset(snaps[0][0], nsub or 3000, 2)
         return Workload(name, n, True, snaps, sub, 0.2, 1e-7, 512, 4, 128, 6, False)
     raise ValueError(f"unknown workload {name!r}")
+
+
+def request_stream(w: Workload, count: int, rate: float, seed: int, median: float = 10_000, sigma: float = 1.0,
+                   lo: int = 1_000, hi: int = 1_000_000):
+    """A served-mode stream over a workload's graph: after its initial
+    snapshot, the remaining edges (in their shuffled order) arrive as `count`
+    insertion batches with lognormal sizes (median 10k, sigma 1, clipped to
+    [1k, 1M], the served configuration's stream) at Poisson times of the given
+    rate (requests/s).  Returns ([(src, dst)], arrival offsets in seconds)."""
+    s = np.concatenate([x[0] for x in w.snapshots[1:]])
+    d = np.concatenate([x[1] for x in w.snapshots[1:]])
+    sizes = lognormal_sizes(count, median, sigma, seed, lo, hi)
+    out, pos = [], 0
+    for b in sizes:
+        if pos >= s.size:
+            break
+        b = int(min(b, s.size - pos))
+        out.append((s[pos:pos + b], d[pos:pos + b]))
+        pos += b
+    rng = _rng(seed + 7)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(out)))
+    arrivals -= arrivals[0]
+    return out, arrivals
