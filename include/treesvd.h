@@ -150,6 +150,7 @@ typedef struct {
     double slack;         /* CSR append slack fraction                     */
     uint64_t seed;        /* sketch seed                                   */
     int32_t two_sided;    /* 0: push r > rmax (reference), 1: |r| > rmax   */
+    int32_t right_embedding; /* LP only: keep V_j = A_j^T U of dirty blocks */
 } tsvd_config;
 
 typedef struct {
@@ -186,6 +187,17 @@ int tsvd_pipeline_node(tsvd_pipeline *p, int32_t level, int32_t index, double *o
  * over all n columns, computed on the device.  For accuracy reports against a
  * dense FP64 SVD; the reference has no counterpart (it never forms it).     */
 int tsvd_pipeline_dense_gram(tsvd_pipeline *p, double *G);
+/* LP right embedding and link scores (Tree_u_LP.cpp:291-404, 1040-1122).
+ * With cfg.right_embedding (transform LP), each snapshot computes, for every
+ * dirty block j, V_j = A_j^T U with that snapshot's root U, and remembers
+ * which snapshot's U it used; a clean block keeps its V_j and its U.  The
+ * score of a pair (i, v), i a subset vertex, is U_j[row(i)] . V[v] with U_j
+ * the U that block(v)'s V was computed with (the reference's
+ * left_matrix_U_cache / left_matrix_index_mapping). */
+/* Host copy of V rows [v0, v1) (row-major, rank columns; inactive vertices 0). */
+int tsvd_pipeline_right_embedding(tsvd_pipeline *p, int32_t v0, int32_t v1, double *V);
+/* scores[e] for pairs (src[e], dst[e]); src must be subset vertices. */
+int tsvd_pipeline_lp_scores(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, int64_t m, double *scores);
 /* Z = A A^T Y (nsrc x l, row-major FP64 in and out) for the same matrix,
  * without forming A A^T: two FP64 products per slab of columns.  Subspace
  * iteration on it gives the exact top singular values where the Gram is too

@@ -329,6 +329,14 @@ def merge(children, k):
     return U * S, U, S
 
 
+def lp_precision(scores, positive, m):
+    """Link-prediction precision of Tree_u_LP.cpp:1092-1110: the m pairs with
+    the highest scores (nth_element + sort), the share of them that are
+    positive pairs.  Ties are broken by input order."""
+    order = np.argsort(-np.asarray(scores, dtype=np.float64), kind="stable")[:m]
+    return float(np.count_nonzero(np.asarray(positive)[order])) / m
+
+
 def root_embedding(children, k, mode="NC"):
     """Root of the merge loop: NC E = U_k diag(sqrt|S_k|) (Tree_u_NC.cpp:
     752-786); LP E = U_k (Tree_u_LP.cpp:907-912)."""
@@ -368,6 +376,8 @@ class Pipeline:
         self.nthreads = nthreads
         self.E = None
         self.root_sigma = None
+        self.V = None                             # LP right embedding (n x k)
+        self.U_of = None
         self.it = 0
         self.last_dirty = []
 
@@ -434,4 +444,24 @@ class Pipeline:
             _, U, S = merge([self.nodes[0][0]], self.k)
             self.E = U * np.sqrt(np.abs(S)) if self.mode == "NC" else U
             self.root_sigma = S
+        if self.mode == "LP":                     # Tree_u_LP.cpp:1040-1045, 291-404
+            if self.V is None:
+                self.V = np.zeros((self.n, self.k))
+                self.U_of = [None] * self.nparts
+            for j in dirty:
+                c0, w = self.starts[j], self.widths[j]
+                self.V[c0:c0 + w] = self.A_old[j].T @ self.E
+                self.U_of[j] = self.E
         return self.E
+
+    def lp_scores(self, src, dst):
+        """Tree_u_LP.cpp:1047-1090: score(i, v) = U_b[row(i)] . V[v] with U_b
+        the root U that block(v)'s V was computed with."""
+        row = {int(v): i for i, v in enumerate(self.sources)}
+        common = self.n // self.nparts
+        out = np.zeros(len(src))
+        for e, (i, v) in enumerate(zip(src, dst)):
+            b = min(int(v) // common, self.nparts - 1)
+            U = self.U_of[b]
+            out[e] = 0.0 if U is None else U[row[int(i)]] @ self.V[int(v)]
+        return out

@@ -1,0 +1,84 @@
+"""LP right embedding and link scores through the C ABI.
+
+The reference (Tree_u_LP.cpp:291-404, 1040-1122) computes, after each
+snapshot, V_j = A_j^T U for every block updated in that snapshot, with that
+snapshot's root U; a block that stays clean keeps its V_j, and scores of pairs
+(i, v) use the U that block(v)'s V was computed with (left_matrix_U_cache).
+Checked on the device's own inputs: the stored block matrices and the root U
+of each snapshot are read back, the bookkeeping is replayed on the host, and
+V and the scores must match it to FP64 rounding (FP32 blocks: 1e-5)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from treesvd_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+
+def workload(directed, seed):
+    if directed:
+        s, d = gen.chung_lu_edges(3000, 24000, seed, 2.1, 2.3)
+        n = 3000
+    else:
+        s, d, n = gen.rmat_edges(12, 24000, seed)
+    snaps = gen.shuffle_split(s, d, seed, 0.7, [2400] * 3)
+    sub = gen.sample_subset(snaps[0][0], 70, seed)
+    return n, snaps, sub
+
+
+@pytest.mark.parametrize("directed,prec,delta", [(False, 1, 0.65), (False, 0, 0.3), (True, 1, 0.65)])
+def test_right_embedding_and_scores_follow_the_reference_bookkeeping(capi, directed, prec, delta):
+    n, snaps, sub = workload(directed, 8)
+    k, nparts = 10, 8
+    pipe = capi.Pipeline(n, directed, sub, 0.2, 1e-6, nparts, 2, k, 6, precision=prec,
+                         transform=capi.TSVD_TRANSFORM_LP, delta=delta, seed=4, right_embedding=True)
+    starts, widths = orc.block_map(n, nparts)
+    V_ref = np.zeros((n, k))
+    U_of = [None] * nparts
+    rng = np.random.Generator(np.random.PCG64(3))
+    tol = 1e-10 if prec == 1 else 1e-5
+    saw_clean = False
+    for it, (s, d) in enumerate(snaps):
+        st = pipe.snapshot(s, d)
+        _, _, _, dirty = pipe.block_norms()
+        U = pipe.embedding()                      # LP root: E = U
+        if it > 0 and (~dirty).any():
+            saw_clean = True
+        for j in np.flatnonzero(dirty):
+            A = pipe.block_matrix(j, int(widths[j]))
+            V_ref[starts[j]:starts[j] + widths[j]] = A.T @ U
+            U_of[j] = U
+        V = pipe.right_embedding()
+        scale = max(np.abs(V_ref).max(), 1e-30)
+        assert np.abs(V - V_ref).max() <= tol * scale, f"snapshot {it}: V differs"
+        # scores of random (subset vertex, vertex) pairs, and of the sources themselves
+        src = rng.choice(sub, 300)
+        dst = rng.integers(0, n, 300).astype(np.int32)
+        dst[:20] = sub[:20]
+        sc = pipe.lp_scores(src, dst)
+        row = {int(v): i for i, v in enumerate(sub)}
+        blk = np.minimum(dst // (n // nparts), nparts - 1)
+        exp = np.array([U_of[b][row[int(i)]] @ V_ref[v] if U_of[b] is not None else 0.0
+                        for i, v, b in zip(src, dst, blk)])
+        assert np.abs(sc - exp).max() <= tol * max(np.abs(exp).max(), 1e-30), f"snapshot {it}: scores differ"
+    assert saw_clean or delta < 0.5, "the lazy test should keep some block clean"
+
+
+def test_lp_scores_reject_bad_pairs_and_need_the_option(capi):
+    n, snaps, sub = workload(False, 9)
+    pipe = capi.Pipeline(n, False, sub, 0.2, 1e-6, 4, 2, 6, 6, precision=1, transform=capi.TSVD_TRANSFORM_LP,
+                         right_embedding=True)
+    pipe.snapshot(*snaps[0])
+    non_subset = np.setdiff1d(np.arange(n), sub)[0]
+    with pytest.raises(capi.TreeSVDError):
+        pipe.lp_scores(np.array([non_subset], np.int32), np.array([0], np.int32))
+    with pytest.raises(capi.TreeSVDError):
+        pipe.lp_scores(np.array([sub[0]], np.int32), np.array([n], np.int32))
+    plain = capi.Pipeline(n, False, sub, 0.2, 1e-6, 4, 2, 6, 6, precision=1, transform=capi.TSVD_TRANSFORM_LP)
+    plain.snapshot(*snaps[0])
+    with pytest.raises(capi.TreeSVDError):
+        plain.lp_scores(np.array([sub[0]], np.int32), np.array([0], np.int32))
+    with pytest.raises(capi.TreeSVDError):
+        capi.Pipeline(n, False, sub, 0.2, 1e-6, 4, 2, 6, 6, precision=1, transform=capi.TSVD_TRANSFORM_NC,
+                      right_embedding=True)

@@ -305,3 +305,39 @@ def test_mutations_are_caught():
     _, _, S1 = orc.merge(kids, 5)
     _, _, S2 = orc.merge([kids[0].T, kids[1]], 5)
     assert not np.allclose(S1, S2, rtol=1e-3)
+
+
+# ------------------------------------------------- LP right embedding
+def test_lp_precision_hand_case():
+    scores = np.array([0.9, 0.1, 0.8, 0.3, 0.7])
+    pos = np.array([True, False, False, True, True])
+    assert orc.lp_precision(scores, pos, 3) == pytest.approx(2 / 3)   # top 3: 0.9 (+), 0.8 (-), 0.7 (+)
+    assert orc.lp_precision(scores, pos, 1) == 1.0
+    assert orc.lp_precision(scores, pos, 5) == pytest.approx(3 / 5)
+
+
+def test_lp_scores_are_the_projection_when_every_block_is_refactored():
+    """With the lazy test off every block's V uses the current root U, so the
+    score of (i, v) is (U U^T A)[row(i), v]: the projection of the LP matrix
+    onto the root's left subspace."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "tree-embedding_amd"))
+    from treesvd_amd import gen
+    s, d, n = gen.rmat_edges(10, 5000, 2)
+    snaps = gen.shuffle_split(s, d, 2, 0.7, [700, 800])
+    sub = gen.sample_subset(snaps[0][0], 30, 2)
+    k = 6
+    pipe = orc.Pipeline(n, False, sub, 0.2, 1e-5, 4, 2, k, 6, mode="LP", delta=0.0, fp32=False, seed=1, nthreads=2)
+    for a, b in snaps:
+        pipe.snapshot(a, b)
+    A = orc.transform(pipe.matrix(), 1e-5, "LP")
+    U = pipe.E
+    P = U @ (U.T @ A)
+    rng = np.random.Generator(np.random.PCG64(0))
+    src = rng.choice(sub, 50)
+    dst = rng.integers(0, n, 50)
+    row = {int(v): i for i, v in enumerate(sub)}
+    got = pipe.lp_scores(src, dst)
+    exp = np.array([P[row[int(i)], int(v)] for i, v in zip(src, dst)])
+    assert np.allclose(got, exp, atol=1e-12 * np.abs(P).max())

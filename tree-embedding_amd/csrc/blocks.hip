@@ -329,6 +329,44 @@ void blocks_rebuild(Blocks &b, const std::vector<int32_t> &dirty, const void *p0
     TSVD_HIP(hipStreamSynchronize(st));
 }
 
+namespace {
+
+// One thread per pair: the dot product of the source's row of its block's U
+// with the target's row of V (Tree_u_LP.cpp:1058-1069).
+__global__ void k_lp_scores(const int32_t *__restrict__ slot, int32_t n, int32_t nparts, int32_t common,
+                            const double *__restrict__ V, const double *const *__restrict__ ublk, int32_t k,
+                            int32_t s_pad, const int32_t *__restrict__ rows, const int32_t *__restrict__ dst, int64_t m,
+                            double *__restrict__ scores)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    const int32_t v = dst[e];
+    const int32_t j = min(v / common, nparts - 1);
+    const int64_t start = (int64_t)j * common;
+    const int64_t width = (j == nparts - 1) ? n - start : common;
+    const int32_t c = slot[v];
+    const double *U = ublk[j];
+    double acc = 0.0;
+    if (c >= 0 && U != nullptr) {
+        const double *Vj = V + start * k + c;
+        const int32_t r = rows[e];
+        for (int32_t t = 0; t < k; t++) acc += U[(int64_t)t * s_pad + r] * Vj[(int64_t)t * width];
+    }
+    scores[e] = acc;
+}
+
+}  // namespace
+
+void lp_scores(const Blocks &b, const double *V, const double *const *ublk, int32_t k, const int32_t *rows,
+               const int32_t *dst, int64_t m, double *scores, hipStream_t st)
+{
+    if (m <= 0) return;
+    PROF("lp_scores", st);
+    hipLaunchKernelGGL(k_lp_scores, dim3((unsigned)ceil_div(m, TPB)), dim3(TPB), 0, st, b.slot.p, b.n, b.nparts,
+                       b.common, V, ublk, k, b.s_pad, rows, dst, m, scores);
+    TSVD_HIP(hipGetLastError());
+}
+
 void gaussian_fill(void *X, int precision, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, hipStream_t st)
 {
     const int64_t tot = rows * cols;

@@ -35,6 +35,46 @@ static void validate(const tsvd_config &c)
     TSVD_REQUIRE(c.q >= 1, "q must be >= 1");
     TSVD_REQUIRE(c.transform == TSVD_TRANSFORM_NC || c.transform == TSVD_TRANSFORM_LP, "bad transform");
     TSVD_REQUIRE(c.fanout * c.rank <= 4096, "fan-out * rank too large for the merge eigensolve");
+    TSVD_REQUIRE(!c.right_embedding || c.transform == TSVD_TRANSFORM_LP,
+                 "the right embedding is defined for the LP transform (Tree_u_LP.cpp:291-404)");
+}
+
+// V_j = A_j^T U for the listed blocks with the current root U (LP: E = U),
+// Tree_u_LP.cpp:1040-1045 / 291-404; each listed block now refers to this
+// snapshot's U, and U copies no block refers to any more are released.
+static void right_embedding_update(tsvd_pipeline *p, const std::vector<int32_t> &blocks, int32_t it,
+                                   hipStream_t st)
+{
+    if (blocks.empty() || !p->tree.have_root) return;
+    RightEmbedding &re = p->re;
+    const Blocks &b = p->blocks;
+    const int32_t k = p->cfg.rank;
+    DBuf<double> &U = re.U[it];
+    U.alloc((size_t)b.s_pad * k);
+    TSVD_HIP(hipMemcpyAsync(U.p, p->tree.E.p, U.bytes(), hipMemcpyDeviceToDevice, st));
+    std::vector<GemmProb> gp;
+    for (int32_t j : blocks) {
+        re.ver[j] = it;
+        if (b.ncols[j] == 0) continue;
+        GemmProb g{};
+        g.A = b.A.p + (size_t)b.start[j] * b.s_pad * b.esize; g.lda = b.s_pad; g.transA = 1;
+        g.B = U.p; g.ldb = b.s_pad;
+        g.C = re.V.p + (size_t)b.start[j] * k; g.ldc = b.width[j];
+        g.M = b.ncols[j]; g.N = k; g.K = b.nsrc; g.alpha = 1.0; g.beta = 0.0;
+        gp.push_back(g);
+    }
+    const DT T = b.precision == TSVD_FP64 ? DT::F64 : DT::F32;
+    p->gemm.run(st, gp, GemmTypes{T, DT::F64, DT::F64, DT::F64}, k > 80, "gemm_right");
+    std::vector<const double *> ptr(b.nparts, nullptr);
+    for (int32_t j = 0; j < b.nparts; j++)
+        if (re.ver[j] >= 0) ptr[j] = re.U.at(re.ver[j]).p;
+    TSVD_HIP(hipMemcpyAsync(re.ublk.p, ptr.data(), sizeof(const double *) * b.nparts, hipMemcpyHostToDevice, st));
+    TSVD_HIP(hipStreamSynchronize(st));
+    for (auto m = re.U.begin(); m != re.U.end();) {
+        bool live = false;
+        for (int32_t j = 0; j < b.nparts && !live; j++) live = re.ver[j] == m->first;
+        m = live ? std::next(m) : re.U.erase(m);
+    }
 }
 
 }  // namespace tsvd
@@ -69,6 +109,15 @@ extern "C" int tsvd_pipeline_create(const tsvd_config *cfg, tsvd_pipeline **out)
             frpca_setup(p->fw, p->blocks, cfg->rank, cfg->q);
             tree_setup(p->tree, cfg->nparts, cfg->fanout, cfg->rank, cfg->nsrc, p->ppr->s_pad, p->blocks.esize, st);
             p->eig = new Eig();
+            for (int32_t i = 0; i < cfg->nsrc; i++) p->re.row_of[cfg->sources[i]] = i;
+            if (cfg->right_embedding) {
+                p->re.V.alloc((size_t)cfg->n * cfg->rank);
+                TSVD_HIP(hipMemsetAsync(p->re.V.p, 0, p->re.V.bytes(), st));
+                p->re.ver.assign(cfg->nparts, -1);
+                p->re.ublk.alloc((size_t)cfg->nparts);
+                TSVD_HIP(hipMemsetAsync(p->re.ublk.p, 0, p->re.ublk.bytes(), st));
+                TSVD_HIP(hipStreamSynchronize(st));
+            }
         } catch (...) {
             delete p;
             throw;
@@ -157,6 +206,7 @@ extern "C" int tsvd_pipeline_snapshot(tsvd_pipeline *p, const int32_t *src, cons
         } else if (!factor.empty()) {
             S.nmerged = tree_update(p->tree, factor, c.transform, p->gemm, *p->eig, st);
         }
+        if (c.right_embedding) right_embedding_update(p, rebuild, it, st);
         const double t6 = now_ms();
         S.ms_merge = t6 - t5;
         S.ms_total = t6 - t0;
@@ -321,5 +371,53 @@ extern "C" int tsvd_pipeline_gram_apply(tsvd_pipeline *p, const double *Y, int32
         TSVD_HIP(hipStreamSynchronize(st));
         for (int32_t s = 0; s < ns; s++)
             for (int32_t c = 0; c < l; c++) Z[(size_t)s * l + c] = h[(size_t)c * ns + s];
+    });
+}
+
+extern "C" int tsvd_pipeline_right_embedding(tsvd_pipeline *p, int32_t v0, int32_t v1, double *V)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p && V, "null argument");
+        TSVD_REQUIRE(p->cfg.right_embedding, "the pipeline was created without right_embedding");
+        const Blocks &b = p->blocks;
+        TSVD_REQUIRE(v0 >= 0 && v0 <= v1 && v1 <= b.n, "bad vertex range");
+        const int32_t k = p->cfg.rank;
+        std::vector<int32_t> slot(b.n);
+        TSVD_HIP(hipMemcpy(slot.data(), b.slot.p, sizeof(int32_t) * b.n, hipMemcpyDeviceToHost));
+        std::vector<double> h((size_t)b.n * k);
+        TSVD_HIP(hipMemcpy(h.data(), p->re.V.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+        for (int32_t v = v0; v < v1; v++) {
+            const int32_t j = std::min(v / b.common, b.nparts - 1);
+            const int32_t c = slot[v];
+            for (int32_t t = 0; t < k; t++)
+                V[(size_t)(v - v0) * k + t] =
+                    (c >= 0 && p->re.ver[j] >= 0) ? h[(size_t)b.start[j] * k + (size_t)t * b.width[j] + c] : 0.0;
+        }
+    });
+}
+
+extern "C" int tsvd_pipeline_lp_scores(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, int64_t m,
+                                       double *scores)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p && (m == 0 || (src && dst && scores)) && m >= 0, "bad arguments");
+        TSVD_REQUIRE(p->cfg.right_embedding, "the pipeline was created without right_embedding");
+        if (m == 0) return;
+        std::vector<int32_t> rows((size_t)m);
+        for (int64_t e = 0; e < m; e++) {
+            auto it = p->re.row_of.find(src[e]);
+            TSVD_REQUIRE(it != p->re.row_of.end(), "pair %lld: source %d is not a subset vertex", (long long)e,
+                         src[e]);
+            TSVD_REQUIRE(dst[e] >= 0 && dst[e] < p->cfg.n, "pair %lld: target %d out of range", (long long)e, dst[e]);
+            rows[(size_t)e] = it->second;
+        }
+        hipStream_t st = p->g->stream;
+        DBuf<int32_t> d_rows((size_t)m), d_dst((size_t)m);
+        DBuf<double> d_sc((size_t)m);
+        TSVD_HIP(hipMemcpyAsync(d_rows.p, rows.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
+        TSVD_HIP(hipMemcpyAsync(d_dst.p, dst, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
+        lp_scores(p->blocks, p->re.V.p, p->re.ublk.p, p->cfg.rank, d_rows.p, d_dst.p, m, d_sc.p, st);
+        TSVD_HIP(hipMemcpyAsync(scores, d_sc.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+        TSVD_HIP(hipStreamSynchronize(st));
     });
 }

@@ -8,6 +8,9 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <map>
+#include <unordered_map>
+
 namespace tsvd {
 
 // Batched FP64 symmetric eigensolver on the device (rocSOLVER syevd).
@@ -105,6 +108,11 @@ void eig_rotation(double *W, const double *V, const double *lam, int l, int coun
 void eig_select(double *W, const double *V, const double *lam, int l, int k, int count, int mode, int ascending,
                 double eps, hipStream_t st);
 
+// LP link scores: scores[e] = sum_c U_j[c * s_pad + rows[e]] * V_j(slot(v), c)
+// for v = dst[e] in block j (0 when v is not a stored column or j has no V).
+void lp_scores(const Blocks &b, const double *V, const double *const *ublk, int32_t k, const int32_t *rows,
+               const int32_t *dst, int64_t m, double *scores, hipStream_t st);
+
 // FP64 vertex-major slab [v0, v0+nv) of the transformed |S| x n matrix
 // (X[(v - v0) * s_pad + s]; padding rows zero).
 void transform_chunk(const Blocks &b, const void *p0, const void *p1, double *X, int64_t v0, int64_t nv,
@@ -129,9 +137,21 @@ void frpca_core(std::vector<FrpcaItem> &items, int precision, int32_t l, int32_t
 
 }  // namespace tsvd
 
+// LP right embedding: V (n x k FP64; block j's rows in stored-column order
+// at V + start_j * k, leading dimension width_j) and the root U each block's
+// V was computed with (one FP64 s_pad x k copy per snapshot still in use).
+struct RightEmbedding {
+    tsvd::DBuf<double> V;
+    std::vector<int32_t> ver;                    // per block: snapshot of its U, -1 none
+    std::map<int32_t, tsvd::DBuf<double>> U;     // snapshot -> U
+    tsvd::DBuf<const double *> ublk;             // per block: device pointer of its U (or null)
+    std::unordered_map<int32_t, int32_t> row_of; // subset vertex -> row
+};
+
 struct tsvd_pipeline {
     tsvd_config cfg{};
     std::vector<int32_t> sources;
+    RightEmbedding re;
     tsvd_graph *g = nullptr;
     tsvd_ppr *ppr = nullptr;
     tsvd::Blocks blocks;

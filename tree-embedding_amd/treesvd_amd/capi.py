@@ -29,7 +29,8 @@ EXPORTS = [
     "tsvd_pipeline_create", "tsvd_pipeline_destroy", "tsvd_pipeline_snapshot",
     "tsvd_pipeline_embedding", "tsvd_pipeline_root_sigma", "tsvd_pipeline_graph", "tsvd_pipeline_ppr",
     "tsvd_pipeline_block_matrix", "tsvd_pipeline_block_norms", "tsvd_pipeline_node",
-    "tsvd_pipeline_dense_gram", "tsvd_pipeline_gram_apply",
+    "tsvd_pipeline_dense_gram", "tsvd_pipeline_gram_apply", "tsvd_pipeline_right_embedding",
+    "tsvd_pipeline_lp_scores",
     "tsvd_server_create", "tsvd_server_destroy", "tsvd_server_submit", "tsvd_server_wait",
     "tsvd_server_pipeline", "tsvd_server_edge_budget",
 ]
@@ -56,7 +57,7 @@ class Config(ctypes.Structure):
                 ("fanout", ctypes.c_int32), ("rank", ctypes.c_int32), ("q", ctypes.c_int32),
                 ("transform", ctypes.c_int32), ("delta", ctypes.c_double), ("dyn_start", ctypes.c_int32),
                 ("lazy_start", ctypes.c_int32), ("slack", ctypes.c_double), ("seed", ctypes.c_uint64),
-                ("two_sided", ctypes.c_int32)]
+                ("two_sided", ctypes.c_int32), ("right_embedding", ctypes.c_int32)]
 
 
 class SnapshotStats(ctypes.Structure):
@@ -140,6 +141,8 @@ def lib():
             L.tsvd_pipeline_node.argtypes = [vp, i32, i32, vp]
             L.tsvd_pipeline_dense_gram.argtypes = [vp, vp]
             L.tsvd_pipeline_gram_apply.argtypes = [vp, vp, i32, vp]
+            L.tsvd_pipeline_right_embedding.argtypes = [vp, i32, i32, vp]
+            L.tsvd_pipeline_lp_scores.argtypes = [vp, vp, vp, i64, vp]
             L.tsvd_server_create.argtypes = [vp, vp, vp]
             L.tsvd_server_destroy.argtypes = [vp]
             L.tsvd_server_submit.argtypes = [vp, vp, vp, i64, vp]
@@ -340,12 +343,12 @@ class Pipeline:
 
     def __init__(self, n, directed, sources, alpha, rmax, nparts, fanout, rank, q=6, precision=TSVD_FP32,
                  transform=TSVD_TRANSFORM_NC, delta=0.65, dyn_start=2, lazy_start=2, slack=0.1, seed=0,
-                 handle=None, two_sided=False):
+                 handle=None, two_sided=False, right_embedding=False):
         self.sources = np.ascontiguousarray(sources, dtype=np.int32)
         self.nsrc, self.rank, self.n, self.nparts = self.sources.size, rank, n, nparts
         self.cfg = Config(n, int(directed), self.nsrc, self.sources.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                           alpha, rmax, precision, nparts, fanout, rank, q, transform, delta, dyn_start, lazy_start,
-                          slack, seed, int(two_sided))
+                          slack, seed, int(two_sided), int(right_embedding))
         self.owned = handle is None
         if handle is None:
             h = ctypes.c_void_p()
@@ -410,6 +413,21 @@ class Pipeline:
         check(lib().tsvd_pipeline_dense_gram(self.h, _p(G)))
         return G
 
+    def right_embedding(self, v0=0, v1=None):
+        """V rows [v0, v1) of the LP right embedding (n x rank)."""
+        v1 = self.n if v1 is None else v1
+        V = np.zeros((v1 - v0, self.rank))
+        check(lib().tsvd_pipeline_right_embedding(self.h, int(v0), int(v1), _p(V)))
+        return V
+
+    def lp_scores(self, src, dst):
+        """Link scores U_j[row(src)] . V[dst] (Tree_u_LP.cpp:1047-1090)."""
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        dst = np.ascontiguousarray(dst, dtype=np.int32)
+        out = np.zeros(src.size)
+        check(lib().tsvd_pipeline_lp_scores(self.h, _p(src), _p(dst), int(src.size), _p(out)))
+        return out
+
     def gram_apply(self, Y):
         """Z = A A^T Y in FP64 for an nsrc x l block Y, without forming A A^T."""
         Y = np.ascontiguousarray(Y, dtype=np.float64)
@@ -443,19 +461,20 @@ class Server:
 
     def __init__(self, n, directed, sources, alpha, rmax, nparts, fanout, rank, q=6, precision=TSVD_FP32,
                  transform=TSVD_TRANSFORM_NC, delta=0.65, dyn_start=2, lazy_start=2, slack=0.1, seed=0,
-                 max_step_edges=0, max_step_requests=0, two_sided=False):
+                 max_step_edges=0, max_step_requests=0, two_sided=False, right_embedding=False):
         self.sources = np.ascontiguousarray(sources, dtype=np.int32)
         self.nsrc, self.rank = self.sources.size, rank
         self.cfg = Config(n, int(directed), self.nsrc, self.sources.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                           alpha, rmax, precision, nparts, fanout, rank, q, transform, delta, dyn_start, lazy_start,
-                          slack, seed, int(two_sided))
+                          slack, seed, int(two_sided), int(right_embedding))
         self.scfg = ServerConfig(int(max_step_edges), int(max_step_requests))
         h = ctypes.c_void_p()
         check(lib().tsvd_server_create(ctypes.byref(self.cfg), ctypes.byref(self.scfg), ctypes.byref(h)))
         self.h = h
         self.pipeline = Pipeline(n, directed, self.sources, alpha, rmax, nparts, fanout, rank, q, precision,
                                  transform, delta, dyn_start, lazy_start, slack, seed,
-                                 handle=ctypes.c_void_p(lib().tsvd_server_pipeline(h)), two_sided=two_sided)
+                                 handle=ctypes.c_void_p(lib().tsvd_server_pipeline(h)), two_sided=two_sided,
+                                 right_embedding=right_embedding)
 
     @property
     def edge_budget(self):
