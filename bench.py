@@ -337,8 +337,10 @@ def main():
     init_ms = (time.perf_counter() - t) * 1e3
     log(f"initial snapshot {init_ms:.0f} ms (push {st0['ms_push']:.0f} ms, {st0['push_rounds']} rounds, "
         f"frpca {st0['ms_frpca']:.0f} ms, {st0['ndirty']} blocks)")
+    warm = []
     for i in range(args.warmup):
         st = pipe.snapshot(*batches[1 + i])
+        warm.append(st)
         log(f"warmup {i}: {st['ms_total']:.1f} ms dirty={st['ndirty']}")
 
     lat, stats = [], []
@@ -354,6 +356,16 @@ def main():
             f"items {st['push_items']} wave-relax {st['push_wave_relax']}")
     total_s = time.perf_counter() - t_all
     value = args.steps / total_s
+    # the push's own ceiling: its work is 64-lane atomic adds, so its rate in
+    # wave-level atomics (row visits of popped items, round + hub kernels) is
+    # set against what a pure stream of such atomics reaches on this device
+    prof_plain = capi.profile_read()
+    push_ms = sum(prof_plain.get(k, (0, 0.0))[1] for k in ("ppr_push_round", "ppr_push_hub"))
+    wave_atomics = sum(s["push_wave_relax"] for s in [st0] + warm + stats)
+    ceiling = capi.probe_atomic_rate()
+    achieved_rate = wave_atomics / (push_ms * 1e-3) if push_ms > 0 else 0.0
+    log(f"push: {wave_atomics:.3e} wave-atomics in {push_ms:.0f} ms = {achieved_rate / 1e9:.2f} G/s; "
+        f"probe ceiling {ceiling / 1e9:.2f} G/s")
     ndirty = float(np.mean([s["ndirty"] for s in stats]))
     nmerged = float(np.mean([s["nmerged"] for s in stats]))
     phases = {k: round(float(np.mean([s[k] for s in stats])), 3)
@@ -371,6 +383,10 @@ def main():
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "initial_snapshot_ms": round(init_ms, 1), "phase_ms": phases,
         "dirty_blocks_per_step": ndirty, "merged_nodes_per_step": nmerged,
+        "push_atomic_roofline": {"achieved": round(achieved_rate / 1e9, 3), "ceiling": round(ceiling / 1e9, 3),
+                                 "unit": "G wave-atomics/s", "frac": round(achieved_rate / ceiling, 4),
+                                 "note": "plain-method push (round + hub kernels); ceiling = returning 64-lane "
+                                         "FP32 atomics at hashed rows (tsvd_probe_atomic_rate)"},
     }
     if args.accuracy:
         log("accuracy against the dense FP64 oracle ...")

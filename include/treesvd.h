@@ -51,6 +51,11 @@ int tsvd_memcpy(void *dst, const void *src, int64_t bytes, int kind);
 int tsvd_profile_enable(int on);
 int tsvd_profile_reset(void);
 int tsvd_profile_read(char *json, int len);
+/* Throughput ceiling of the push's memory operation, for its roofline: 64-lane
+ * returning FP32 atomic adds on 256-byte rows at hashed row indices over a
+ * `rows` x 256 B array, `iters` per wavefront.  Returns wave-level atomic
+ * instructions per second (each updates 64 consecutive floats). */
+int tsvd_probe_atomic_rate(int64_t rows, int32_t iters, double *wave_atomics_per_s);
 
 /* ------------------------------------------------------------------ graph
  * Replaces UGraph::initializeDynamicUGraph / inputDynamicGraph
@@ -76,6 +81,15 @@ int tsvd_graph_export_csr(const tsvd_graph *g, int dir, int64_t *rowptr, int32_t
  * pairs); dup_last=1 keeps the reference's duplicate of the last edge of a
  * file that ends in a newline.  With src == NULL only *count is set.       */
 int tsvd_read_edge_file(const char *path, int dup_last, int32_t *src, int32_t *dst, int64_t cap, int64_t *count);
+/* The drivers' other files.  Subset file (Tree_u_NC.cpp:363-371): integers
+ * read until extraction fails; ids may be NULL to count.  config.txt
+ * (Tree_u_NC.cpp:338-351): getline lines, NUL-terminated one after another
+ * in buf (buf may be NULL; *needed gives the size).  Embedding CSV
+ * (Tree_u_NC.cpp:63, 898-909): Eigen's IOFormat(StreamPrecision,
+ * DontAlignCols, ", ", "\n") of a row-major rows x cols matrix. */
+int tsvd_read_subset_file(const char *path, int32_t *ids, int64_t cap, int64_t *count);
+int tsvd_read_lines(const char *path, char *buf, int64_t len, int32_t *count, int64_t *needed);
+int tsvd_write_embedding_csv(const char *path, const double *U, int64_t rows, int32_t cols);
 
 /* -------------------------------------------------------------------- PPR
  * Replaces nodegree_DenseUndirected_Refresh_PPR_initialization

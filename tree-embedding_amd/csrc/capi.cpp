@@ -1,6 +1,8 @@
 // capi.cpp -- error state and device queries of the C ABI.
 #include "tsvd_internal.h"
 
+#include <fstream>
+
 #include <cstring>
 
 namespace tsvd {
@@ -107,6 +109,94 @@ extern "C" int tsvd_read_edge_file(const char *path, int dup_last, int32_t *src,
         }
         fclose(f);
         *count = cnt;
+    });
+}
+
+// ------------------------------------------------------ driver file formats
+// The subset file of the drivers (Tree_u_NC.cpp:363-371):
+//   while (infile2 >> node_number) { ... }   -- integers until extraction fails.
+extern "C" int tsvd_read_subset_file(const char *path, int32_t *ids, int64_t cap, int64_t *count)
+{
+    return tsvd::guard([&] {
+        TSVD_REQUIRE(path && count, "null argument");
+        FILE *f = fopen(path, "r");
+        TSVD_REQUIRE(f, "cannot open %s", path);
+        IntReader rd{f};
+        int64_t cnt = 0;
+        int32_t v = 0;
+        for (;;) {
+            const int r = rd.next(&v);
+            if (r == 2) break;
+            if (ids && cnt < cap) ids[cnt] = v;
+            cnt++;
+            if (r == 1) break;
+        }
+        fclose(f);
+        *count = cnt;
+    });
+}
+
+// config.txt of the drivers (Tree_u_NC.cpp:338-351): one path per line, read
+// with getline, so a final newline adds no line and a final line without one
+// still counts.  Lines go to buf NUL-terminated one after another; *needed is
+// the size that holds them all.
+extern "C" int tsvd_read_lines(const char *path, char *buf, int64_t len, int32_t *count, int64_t *needed)
+{
+    return tsvd::guard([&] {
+        TSVD_REQUIRE(path && count, "null argument");
+        FILE *f = fopen(path, "rb");
+        TSVD_REQUIRE(f, "cannot open %s", path);
+        std::string all, line;
+        int32_t n = 0;
+        int c;
+        bool pending = false;
+        while ((c = fgetc(f)) != EOF) {
+            if (c == '\n') {
+                all += line;
+                all += '\0';
+                line.clear();
+                n++;
+                pending = false;
+            } else {
+                line += (char)c;
+                pending = true;
+            }
+        }
+        if (pending) {
+            all += line;
+            all += '\0';
+            n++;
+        }
+        fclose(f);
+        *count = n;
+        if (needed) *needed = (int64_t)all.size();
+        if (buf) {
+            TSVD_REQUIRE((int64_t)all.size() <= len, "buffer of %lld bytes is too small (%zu needed)", (long long)len,
+                         all.size());
+            memcpy(buf, all.data(), all.size());
+        }
+    });
+}
+
+// The embedding file of the drivers (Tree_u_NC.cpp:63, 898-909): Eigen's
+// IOFormat(StreamPrecision, DontAlignCols, ", ", "\n") of the row-major
+// rows x cols matrix -- every coefficient through an ostream at its default
+// precision, ", " between coefficients, "\n" between rows, none at the end.
+extern "C" int tsvd_write_embedding_csv(const char *path, const double *U, int64_t rows, int32_t cols)
+{
+    return tsvd::guard([&] {
+        TSVD_REQUIRE(path && (U || rows == 0) && rows >= 0 && cols >= 0, "bad arguments");
+        std::ofstream out(path);
+        TSVD_REQUIRE(out.good(), "cannot write %s", path);
+        for (int64_t i = 0; i < rows; i++) {
+            if (i) out << "\n";
+            for (int32_t j = 0; j < cols; j++) {
+                if (j) out << ", ";
+                out << U[i * cols + j];
+            }
+        }
+        out.close();
+        TSVD_REQUIRE(!out.fail(), "write to %s failed", path);
     });
 }
 

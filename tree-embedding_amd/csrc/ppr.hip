@@ -845,3 +845,59 @@ extern "C" int tsvd_ppr_set_two_sided(tsvd_ppr *s, int on)
         s->two_sided = on ? 1 : 0;
     });
 }
+
+namespace tsvd {
+namespace {
+
+__device__ inline uint32_t probe_hash(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// Each wavefront adds 1 to `iters` rows at hashed indices with one returning
+// 64-lane atomic per row (the relaxation's access, without the graph).
+__global__ void __launch_bounds__(TPB) k_probe_atomic(float *__restrict__ R, uint32_t rows, int32_t iters,
+                                                      float *__restrict__ sink)
+{
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    float acc = 0.f;
+    for (int32_t i = 0; i < iters; i++) {
+        const uint32_t row = probe_hash(wave * 7919u + (uint32_t)i) % rows;
+        acc += atomicAdd(R + (size_t)row * 64 + lane, 1.0f);
+    }
+    if (acc < 0.f) sink[0] = acc;   // keeps the returned values live
+}
+
+}  // namespace
+}  // namespace tsvd
+
+extern "C" int tsvd_probe_atomic_rate(int64_t rows, int32_t iters, double *wave_atomics_per_s)
+{
+    using namespace tsvd;
+    return guard([&] {
+        TSVD_REQUIRE(wave_atomics_per_s && rows >= 1 && rows <= (int64_t)1 << 26 && iters >= 1 && iters <= 4096,
+                     "rows in [1, 2^26], iters in [1, 4096]");
+        DBuf<float> R((size_t)rows * 64), sink(1);
+        TSVD_HIP(hipMemset(R.p, 0, R.bytes()));
+        const int blocks = 8192;
+        Timer t;
+        double best = 0.0;
+        for (int rep = 0; rep < 3; rep++) {
+            t.start(nullptr);
+            hipLaunchKernelGGL(k_probe_atomic, dim3(blocks), dim3(TPB), 0, nullptr, R.p, (uint32_t)rows, iters,
+                               sink.p);
+            TSVD_HIP(hipGetLastError());
+            t.stop(nullptr);
+            const double ms = t.ms();
+            const double waves = (double)blocks * TPB / 64;
+            best = std::max(best, waves * iters / (ms * 1e-3));
+        }
+        *wave_atomics_per_s = best;
+    });
+}

@@ -22,7 +22,8 @@ EXPORTS = [
     "tsvd_last_error", "tsvd_device_info", "tsvd_device_alloc", "tsvd_device_free", "tsvd_memcpy",
     "tsvd_profile_enable", "tsvd_profile_reset", "tsvd_profile_read",
     "tsvd_graph_create", "tsvd_graph_destroy", "tsvd_graph_append", "tsvd_graph_num_edges",
-    "tsvd_graph_degrees", "tsvd_graph_export_csr", "tsvd_read_edge_file",
+    "tsvd_graph_degrees", "tsvd_graph_export_csr", "tsvd_read_edge_file", "tsvd_read_subset_file",
+    "tsvd_read_lines", "tsvd_write_embedding_csv",
     "tsvd_ppr_create", "tsvd_ppr_destroy", "tsvd_ppr_refresh", "tsvd_ppr_push", "tsvd_ppr_read",
     "tsvd_ppr_last_push_stats", "tsvd_ppr_set_two_sided",
     "frPCA", "tsvd_frpca_csr",
@@ -32,7 +33,7 @@ EXPORTS = [
     "tsvd_pipeline_dense_gram", "tsvd_pipeline_gram_apply", "tsvd_pipeline_right_embedding",
     "tsvd_pipeline_lp_scores",
     "tsvd_server_create", "tsvd_server_destroy", "tsvd_server_submit", "tsvd_server_wait",
-    "tsvd_server_pipeline", "tsvd_server_edge_budget",
+    "tsvd_server_pipeline", "tsvd_server_edge_budget", "tsvd_probe_atomic_rate",
 ]
 
 
@@ -113,6 +114,9 @@ def lib():
         L.tsvd_graph_degrees.argtypes = [vp, ctypes.c_int, vp, vp]
         L.tsvd_graph_export_csr.argtypes = [vp, ctypes.c_int, vp, vp]
         L.tsvd_read_edge_file.argtypes = [ctypes.c_char_p, ctypes.c_int, vp, vp, i64, vp]
+        L.tsvd_read_subset_file.argtypes = [ctypes.c_char_p, vp, i64, vp]
+        L.tsvd_read_lines.argtypes = [ctypes.c_char_p, vp, i64, vp, vp]
+        L.tsvd_write_embedding_csv.argtypes = [ctypes.c_char_p, vp, i64, i32]
         L.tsvd_ppr_create.argtypes = [vp, vp, i32, f64, f64, ctypes.c_int, vp]
         L.tsvd_ppr_destroy.argtypes = [vp]
         L.tsvd_ppr_refresh.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32, i32]
@@ -151,6 +155,7 @@ def lib():
             L.tsvd_server_pipeline.argtypes = [vp]
             L.tsvd_server_edge_budget.restype = i64
             L.tsvd_server_edge_budget.argtypes = [vp]
+            L.tsvd_probe_atomic_rate.argtypes = [i64, i32, vp]
         _lib = L
     return _lib
 
@@ -260,6 +265,33 @@ def read_edge_file(path: str, dup_last: bool = True):
     s = np.zeros(max(cnt.value, 1), np.int32); d = np.zeros(max(cnt.value, 1), np.int32)
     check(lib().tsvd_read_edge_file(path.encode(), int(dup_last), _p(s), _p(d), cnt.value, ctypes.byref(cnt)))
     return s[:cnt.value], d[:cnt.value]
+
+
+def read_subset_file(path: str):
+    """Subset vertex ids (Tree_u_NC.cpp:363-371)."""
+    cnt = ctypes.c_int64()
+    check(lib().tsvd_read_subset_file(path.encode(), None, 0, ctypes.byref(cnt)))
+    ids = np.zeros(max(cnt.value, 1), np.int32)
+    check(lib().tsvd_read_subset_file(path.encode(), _p(ids), cnt.value, ctypes.byref(cnt)))
+    return ids[:cnt.value]
+
+
+def read_lines(path: str):
+    """config.txt lines (Tree_u_NC.cpp:338-351)."""
+    cnt, need = ctypes.c_int32(), ctypes.c_int64()
+    check(lib().tsvd_read_lines(path.encode(), None, 0, ctypes.byref(cnt), ctypes.byref(need)))
+    buf = ctypes.create_string_buffer(max(need.value, 1))
+    check(lib().tsvd_read_lines(path.encode(), buf, need.value, ctypes.byref(cnt), ctypes.byref(need)))
+    raw = buf.raw[:need.value]
+    return [x.decode() for x in raw.split(b"\0")[:cnt.value]]
+
+
+def write_embedding_csv(path: str, U):
+    """The drivers' embedding file (Tree_u_NC.cpp:63, 898-909)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    if U.ndim != 2:
+        raise ValueError("U must be a matrix")
+    check(lib().tsvd_write_embedding_csv(path.encode(), _p(U), U.shape[0], U.shape[1]))
 
 
 class PPR:
@@ -444,6 +476,14 @@ def profile_enable(on: bool = True):
 
 def profile_reset():
     check(lib().tsvd_profile_reset())
+
+
+def probe_atomic_rate(rows=1 << 22, iters=64):
+    """Wave-level (64-lane, 256-byte) returning FP32 atomics per second on
+    this device (tsvd_probe_atomic_rate)."""
+    r = ctypes.c_double()
+    check(lib().tsvd_probe_atomic_rate(int(rows), int(iters), ctypes.byref(r)))
+    return r.value
 
 
 def profile_read():
