@@ -29,13 +29,16 @@ def test_run_config_matches_the_pipeline_and_writes_the_csv(capi, tmp_path, lazy
     cfg = write_dataset(tmp_path, snaps, sub)
     k = 8
     out = tmp_path / "U.csv"
+    # a snapshot skipped by the lazy start is only ingested, so the static
+    # push must come after it (dynamic_ppr_start_iter = lazy_update_start_iter)
     U_all = drivers.run_config(cfg, str(out), n, alpha=0.2, pass_=6, residuemax=1e-6, nParts=4, hierarchy_n=2,
-                               lazy_update_start_iter=lazy_start, d=k, precision=capi.TSVD_FP64, seed=2)
+                               dynamic_ppr_start_iter=lazy_start, lazy_update_start_iter=lazy_start, d=k,
+                               precision=capi.TSVD_FP64, seed=2)
     assert U_all.shape == (len(snaps) * sub.size, k)
     # the same snapshots straight through the pipeline (the reader keeps the
     # duplicated last edge of each file, as the reference's loop does)
-    pipe = capi.Pipeline(n, False, sub, 0.2, 1e-6, 4, 2, k, 6, precision=capi.TSVD_FP64, lazy_start=lazy_start,
-                         seed=2)
+    pipe = capi.Pipeline(n, False, sub, 0.2, 1e-6, 4, 2, k, 6, precision=capi.TSVD_FP64, dyn_start=lazy_start,
+                         lazy_start=lazy_start, seed=2)
     for it, (a, b) in enumerate(snaps, start=1):
         a2, b2 = np.r_[a, a[-1]], np.r_[b, b[-1]]
         pipe.snapshot(a2, b2)
@@ -44,6 +47,7 @@ def test_run_config_matches_the_pipeline_and_writes_the_csv(capi, tmp_path, lazy
             assert not rows.any()
             continue
         E = pipe.embedding()
+        assert E.any()
         G, H = E @ E.T, rows @ rows.T
         assert np.abs(G - H).max() <= 1e-8 * np.abs(G).max(), f"snapshot {it}"
     text = out.read_text()
