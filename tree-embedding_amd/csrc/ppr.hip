@@ -57,11 +57,12 @@ __device__ inline void append_item(uint32_t item, uint32_t *list, uint32_t *coun
 }
 
 // A residue beyond the push threshold: r > rmax (the reference's positive
-// side, ppr:1445), or |r| > rmax when pushing two-sided.
+// side, ppr:1445), or |r| > rmax when pushing two-sided.  Written without
+// short-circuits so that it compiles to compares and masks, not branches.
 template <class T>
 __device__ inline bool beyond(T x, T rm, int two)
 {
-    return x > rm || (two && x < -rm);
+    return (x > rm) | ((two != 0) & (x < -rm));
 }
 
 // Lane-0 sets `bits` in mask[item]; the first setter appends the item.
@@ -158,36 +159,64 @@ __device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, i
                                    uint32_t *__restrict__ next, uint32_t *__restrict__ next_count, uint32_t cap,
                                    uint32_t *__restrict__ overflow)
 {
+    // The neighbour ids of 64 consecutive slots arrive in one coalesced load
+    // (one id per lane) and are broadcast with readlane, so each relaxation
+    // costs one vector memory instruction (the atomic) instead of two.  Every
+    // lane of the wave runs this function (j0, j1 are wave-uniform).
     constexpr int U = 8;
-    int32_t j = j0;
-    for (; j + U <= j1; j += U) {
-        int32_t u[U];
-        bool c[U];
+    for (int32_t base = j0; base < j1; base += 64) {
+        const int32_t cnt = min(64, j1 - base);
+        const int32_t mine = lane < cnt ? nb[base + lane] : 0;
+        int32_t q0 = 0;
+        for (; q0 + U <= cnt; q0 += U) {
+            int32_t u[U];
+            T o[U];
+            bool c[U];
 #pragma unroll
-        for (int q = 0; q < U; q++) u[q] = nb[j + q];
-#pragma unroll
-        for (int q = 0; q < U; q++) {
-            c[q] = false;
+            for (int q = 0; q < U; q++) u[q] = __builtin_amdgcn_readlane(mine, q0 + q);
+            // all U atomics are issued before any result is used, so a wave
+            // keeps U of them in flight
             if (pushing) {
-                const T o = atomicAdd(Rs + (int64_t)u[q] * s_pad, delta);
-                c[q] = !beyond(o, rm, two) && beyond(o + delta, rm, two);
+#pragma unroll
+                for (int q = 0; q < U; q++) o[q] = atomicAdd(Rs + (int64_t)u[q] * s_pad, delta);
+            } else {
+#pragma unroll
+                for (int q = 0; q < U; q++) o[q] = T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++)
+                c[q] = pushing & !beyond(o[q], rm, two) & beyond(o[q] + delta, rm, two);
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const unsigned long long b = __ballot(c[q]);
+                if (b && lane == 0 && deg[u[q]] > 0)
+                    mark(b, (uint32_t)u[q] * G + g, nm, next, next_count, cap, overflow);
             }
         }
+        if (q0 < cnt) {
+            // the last cnt - q0 < U slots, the same way (slot q0 + q for q < r)
+            const int32_t r = cnt - q0;
+            int32_t u[U];
+            T o[U];
+            bool c[U];
 #pragma unroll
-        for (int q = 0; q < U; q++) {
-            const unsigned long long b = __ballot(c[q]);
-            if (b && lane == 0 && deg[u[q]] > 0) mark(b, (uint32_t)u[q] * G + g, nm, next, next_count, cap, overflow);
+            for (int q = 0; q < U; q++) u[q] = __builtin_amdgcn_readlane(mine, min(q0 + q, 63));
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                o[q] = T(0);
+                if (pushing && q < r) o[q] = atomicAdd(Rs + (int64_t)u[q] * s_pad, delta);
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++)
+                c[q] = pushing & (q < r) & !beyond(o[q], rm, two) & beyond(o[q] + delta, rm, two);
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                if (q >= r) break;
+                const unsigned long long b = __ballot(c[q]);
+                if (b && lane == 0 && deg[u[q]] > 0)
+                    mark(b, (uint32_t)u[q] * G + g, nm, next, next_count, cap, overflow);
+            }
         }
-    }
-    for (; j < j1; j++) {
-        const int32_t u = nb[j];
-        bool c = false;
-        if (pushing) {
-            const T o = atomicAdd(Rs + (int64_t)u * s_pad, delta);
-            c = !beyond(o, rm, two) && beyond(o + delta, rm, two);
-        }
-        const unsigned long long b = __ballot(c);
-        if (b && lane == 0 && deg[u] > 0) mark(b, (uint32_t)u * G + g, nm, next, next_count, cap, overflow);
     }
 }
 
