@@ -202,14 +202,12 @@ def accuracy_report(pipe, w, seed=7):
     dev = orac.errors(pipe.root_sigma(), pipe.embedding(), nc)
     t = time.perf_counter()
     starts, widths = orc.block_map(w.n, w.nparts)
-    deg0 = pipe.graph.degrees(0)[0].astype(np.int64)
-    deg = deg0 + (pipe.graph.degrees(1)[0] if w.directed else 0)
-    act = deg > 0
-    act[w.subset] = True
     leaves = []
     for j in range(w.nparts):
         Aj = pipe.block_matrix(j, int(widths[j]))
-        cols = np.flatnonzero(act[starts[j]:starts[j] + widths[j]])
+        # the block's stored columns (active at its last rebuild): the sketch
+        # has one row per stored column, as on the device
+        cols = pipe.block_columns(j) - starts[j]
         Ac = Aj[:, cols]
         del Aj
         if not Ac.any():

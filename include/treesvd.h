@@ -194,6 +194,10 @@ tsvd_ppr *tsvd_pipeline_ppr(tsvd_pipeline *p);
  * and of the per-block norms ||A||, ||A_new - A_old||, residual. */
 int tsvd_pipeline_block_matrix(tsvd_pipeline *p, int32_t j, double *A);
 int tsvd_pipeline_block_norms(tsvd_pipeline *p, double *a_norm, double *e_norm, double *resid, int32_t *dirty);
+/* Vertex ids of block j's stored columns (the active columns at its last
+ * rebuild, ascending; *count of them; cols may be NULL).  frPCA's sketch of
+ * block j has one row per stored column, in this order. */
+int tsvd_pipeline_block_columns(tsvd_pipeline *p, int32_t j, int32_t *cols, int32_t *count);
 /* Host copy of one tree node's factor U*S (nsrc x rank, col-major FP64);
  * level 0 = leaves. */
 int tsvd_pipeline_node(tsvd_pipeline *p, int32_t level, int32_t index, double *out);

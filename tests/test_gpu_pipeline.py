@@ -79,6 +79,7 @@ def test_pipeline_stagewise(capi, directed, prec, mode):
             A_prev[j] = Aj
             # leaf j vs the oracle frPCA of the same compacted block and sketch
             cols = np.flatnonzero(act[starts[j]:starts[j] + widths[j]])
+            assert np.array_equal(pipe.block_columns(j), starts[j] + cols)
             Ac = Aj[:, cols]
             om = orc.sketch(orc.leaf_seed(seed, j), cols.size, k + 5, fp32=fp32)
             U, S, V = orc.frpca(Ac, k, q, omega=om)

@@ -262,6 +262,19 @@ extern "C" int tsvd_pipeline_block_matrix(tsvd_pipeline *p, int32_t j, double *A
     });
 }
 
+extern "C" int tsvd_pipeline_block_columns(tsvd_pipeline *p, int32_t j, int32_t *cols, int32_t *count)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p && count, "null argument");
+        const Blocks &b = p->blocks;
+        TSVD_REQUIRE(j >= 0 && j < b.nparts, "block out of range");
+        const int32_t nc = b.ncols[j];
+        *count = nc;
+        if (cols && nc > 0)
+            TSVD_HIP(hipMemcpy(cols, b.colmap.p + b.start[j], sizeof(int32_t) * nc, hipMemcpyDeviceToHost));
+    });
+}
+
 extern "C" int tsvd_pipeline_block_norms(tsvd_pipeline *p, double *a_norm, double *e_norm, double *resid,
                                          int32_t *dirty)
 {

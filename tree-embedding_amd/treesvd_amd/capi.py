@@ -29,7 +29,7 @@ EXPORTS = [
     "frPCA", "tsvd_frpca_csr",
     "tsvd_pipeline_create", "tsvd_pipeline_destroy", "tsvd_pipeline_snapshot",
     "tsvd_pipeline_embedding", "tsvd_pipeline_root_sigma", "tsvd_pipeline_graph", "tsvd_pipeline_ppr",
-    "tsvd_pipeline_block_matrix", "tsvd_pipeline_block_norms", "tsvd_pipeline_node",
+    "tsvd_pipeline_block_matrix", "tsvd_pipeline_block_norms", "tsvd_pipeline_node", "tsvd_pipeline_block_columns",
     "tsvd_pipeline_dense_gram", "tsvd_pipeline_gram_apply", "tsvd_pipeline_right_embedding",
     "tsvd_pipeline_lp_scores",
     "tsvd_server_create", "tsvd_server_destroy", "tsvd_server_submit", "tsvd_server_wait",
@@ -142,6 +142,7 @@ def lib():
             L.tsvd_pipeline_ppr.argtypes = [vp]
             L.tsvd_pipeline_block_matrix.argtypes = [vp, i32, vp]
             L.tsvd_pipeline_block_norms.argtypes = [vp, vp, vp, vp, vp]
+            L.tsvd_pipeline_block_columns.argtypes = [vp, i32, vp, vp]
             L.tsvd_pipeline_node.argtypes = [vp, i32, i32, vp]
             L.tsvd_pipeline_dense_gram.argtypes = [vp, vp]
             L.tsvd_pipeline_gram_apply.argtypes = [vp, vp, i32, vp]
@@ -427,6 +428,14 @@ class Pipeline:
         A = np.zeros((width, self.nsrc))         # column-major nsrc x width
         check(lib().tsvd_pipeline_block_matrix(self.h, j, _p(A)))
         return A.T.copy()
+
+    def block_columns(self, j):
+        """Vertex ids of block j's stored columns, in sketch-row order."""
+        cnt = ctypes.c_int32()
+        check(lib().tsvd_pipeline_block_columns(self.h, int(j), None, ctypes.byref(cnt)))
+        cols = np.zeros(max(cnt.value, 1), np.int32)
+        check(lib().tsvd_pipeline_block_columns(self.h, int(j), _p(cols), ctypes.byref(cnt)))
+        return cols[:cnt.value]
 
     def block_norms(self):
         a, e, r = np.zeros(self.nparts), np.zeros(self.nparts), np.zeros(self.nparts)
