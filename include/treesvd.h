@@ -137,6 +137,12 @@ typedef struct { int nnz; int nrows, ncols; double *values; int *cols; int *poin
 void frPCA(mat_csr *A, mat **U, mat **S, mat **V, int k, int q);
 /* Same computation with a status code and an explicit sketch seed. */
 int tsvd_frpca_csr(const mat_csr *A, mat *U, mat *S, mat *V, int k, int q, uint64_t seed);
+/* `count` independent matrices in one grouped device run (item i with sketch
+ * seed seed + i): replaces the drivers' serial frPCA loop over dirty blocks
+ * (Tree_u_NC.cpp:679-710).  Per item the result equals tsvd_frpca_csr with
+ * that seed.  Not re-entrant (one set of solver handles per process). */
+int tsvd_frpca_csr_batch(const mat_csr *const *A, mat *const *U, mat *const *S, mat *const *V, int count, int k,
+                         int q, uint64_t seed);
 
 /* ------------------------------------------------------------- pipeline
  * The per-snapshot driver of Tree_u_NC.cpp:507-866, Tree_d_NC.cpp:611-1132,

@@ -82,3 +82,27 @@ def test_dropin_rejects_bad_csr(capi):
     bad.indices[0] = 40   # out of range (becomes 41, 1-based)
     with pytest.raises(capi.TreeSVDError):
         capi.frpca_csr(bad, 5, 6)
+
+
+def test_batched_dropin_equals_single_calls(capi):
+    """tsvd_frpca_csr_batch over matrices of different shapes (the drivers'
+    dirty blocks): each item equals tsvd_frpca_csr with seed + i, and the
+    oracle with that item's sketch."""
+    shapes = [(96, 300), (96, 157), (96, 41), (96, 520)]
+    mats = [make_matrix(m, n, 40 + i) for i, (m, n) in enumerate(shapes)]
+    k, q, seed = 12, 6, 900
+    out = capi.frpca_csr_batch(mats, k, q, seed)
+    for i, (A, (U, S, V)) in enumerate(zip(mats, out)):
+        U1, S1, V1 = capi.frpca_csr(A, k, q, seed + i)
+        assert np.allclose(S, S1, rtol=1e-12, atol=1e-14), f"item {i}"
+        _, So, _ = orc.frpca(A, k, q, omega=orc.sketch(seed + i, A.shape[1], k + 5))
+        assert np.allclose(S, So, rtol=1e-9, atol=1e-12), f"item {i} vs oracle"
+        assert np.allclose((U * S) @ V.T, (U1 * S1) @ V1.T, atol=1e-9 * np.abs(A).max())
+
+
+def test_batched_dropin_rejects_bad_arguments(capi):
+    mats = [make_matrix(50, 80, 1), make_matrix(50, 80, 2)]
+    with pytest.raises(capi.TreeSVDError):
+        capi.frpca_csr_batch(mats, 8, 0, 1)       # q must be positive
+    with pytest.raises(capi.TreeSVDError):
+        capi.frpca_csr_batch([], 8, 6, 1)         # empty batch

@@ -232,15 +232,33 @@ struct DropIn {
     Eig eig;
 };
 
-int frpca_csr_impl(const mat_csr *A, mat *U, mat *S, mat *V, int k, int q, uint64_t seed)
+// frPCA of `count` independent CSR matrices in one grouped run (item i uses
+// sketch seed seed + i): every product is one grouped launch over the items
+// and every eigensolve one batched call.
+int frpca_csr_batch_impl(const mat_csr *const *A, mat *const *U, mat *const *S, mat *const *V, int count, int k,
+                         int q, uint64_t seed)
 {
     return guard([&] {
-        TSVD_REQUIRE(A && U && S && V, "null matrix");
-        const int m = A->nrows, n = A->ncols, l = k + 5;
+        TSVD_REQUIRE(count >= 1 && A && U && S && V, "null argument or empty batch");
         TSVD_REQUIRE(k >= 1 && q >= 1, "k and q must be positive");
-        TSVD_REQUIRE(m >= 1 && n >= 1 && A->nnz >= 0, "bad matrix shape");
-        TSVD_REQUIRE(U->nrows == m && U->ncols == k && S->nrows == k && V->nrows == n && V->ncols == k,
-                     "output shapes must be U m x k, S k x 1, V n x k");
+        const int l = k + 5;
+        for (int i = 0; i < count; i++) {
+            TSVD_REQUIRE(A[i] && U[i] && S[i] && V[i], "null matrix in item %d", i);
+            const mat_csr *a = A[i];
+            const int m = a->nrows, n = a->ncols;
+            TSVD_REQUIRE(m >= 1 && n >= 1 && a->nnz >= 0, "bad matrix shape in item %d", i);
+            TSVD_REQUIRE(U[i]->nrows == m && U[i]->ncols == k && S[i]->nrows == k && V[i]->nrows == n &&
+                             V[i]->ncols == k,
+                         "item %d: output shapes must be U m x k, S k x 1, V n x k", i);
+            // the CSR is validated on the host (1-based, in range) before any copy
+            for (int r = 0; r < m; r++) {
+                TSVD_REQUIRE(a->pointerB[r] >= 1 && a->pointerE[r] >= a->pointerB[r] && a->pointerE[r] - 1 <= a->nnz,
+                             "item %d: bad row pointers at row %d", i, r);
+                for (int t = a->pointerB[r] - 1; t < a->pointerE[r] - 1; t++)
+                    TSVD_REQUIRE(a->cols[t] >= 1 && a->cols[t] <= n, "item %d: column index out of range at row %d",
+                                 i, r);
+            }
+        }
         hipStream_t st;
         TSVD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         struct StreamGuard {
@@ -249,54 +267,87 @@ int frpca_csr_impl(const mat_csr *A, mat *U, mat *S, mat *V, int k, int q, uint6
         } sg{st};
         static DropIn *ctx = nullptr;   // handles reused across calls (one per process)
         if (!ctx) ctx = new DropIn();
-        const int64_t ld = round_up(m, 64);
-        DBuf<double> dA((size_t)ld * n), dQt((size_t)n * l), dQ((size_t)ld * l), dQ2((size_t)ld * l);
-        DBuf<double> G((size_t)l * l), W((size_t)l * l), lam((size_t)l), Wk((size_t)l * k);
-        DBuf<double> vals((size_t)std::max(A->nnz, 1));
-        DBuf<int> cols((size_t)std::max(A->nnz, 1)), pb((size_t)m), pe((size_t)m);
-        TSVD_HIP(hipMemsetAsync(dA.p, 0, dA.bytes(), st));
-        TSVD_HIP(hipMemsetAsync(dQ.p, 0, dQ.bytes(), st));
-        TSVD_HIP(hipMemsetAsync(dQ2.p, 0, dQ2.bytes(), st));
-        if (A->nnz > 0) {
-            TSVD_HIP(hipMemcpyAsync(vals.p, A->values, sizeof(double) * A->nnz, hipMemcpyHostToDevice, st));
-            TSVD_HIP(hipMemcpyAsync(cols.p, A->cols, sizeof(int) * A->nnz, hipMemcpyHostToDevice, st));
+        struct Dev {
+            DBuf<double> A, Qt, Q, Q2, U, V, vals;
+            DBuf<int> cols, pb, pe;
+            int64_t ld;
+        };
+        std::vector<Dev> dv((size_t)count);
+        std::vector<FrpcaItem> items((size_t)count);
+        for (int i = 0; i < count; i++) {
+            const mat_csr *a = A[i];
+            const int m = a->nrows, n = a->ncols;
+            Dev &d = dv[(size_t)i];
+            d.ld = round_up(m, 64);
+            d.A.alloc((size_t)d.ld * n);
+            d.Qt.alloc((size_t)n * l);
+            d.Q.alloc((size_t)d.ld * l);
+            d.Q2.alloc((size_t)d.ld * l);
+            d.U.alloc((size_t)m * k);
+            d.V.alloc((size_t)n * k);
+            d.vals.alloc((size_t)std::max(a->nnz, 1));
+            d.cols.alloc((size_t)std::max(a->nnz, 1));
+            d.pb.alloc((size_t)m);
+            d.pe.alloc((size_t)m);
+            TSVD_HIP(hipMemsetAsync(d.A.p, 0, d.A.bytes(), st));
+            TSVD_HIP(hipMemsetAsync(d.Q.p, 0, d.Q.bytes(), st));
+            TSVD_HIP(hipMemsetAsync(d.Q2.p, 0, d.Q2.bytes(), st));
+            if (a->nnz > 0) {
+                TSVD_HIP(hipMemcpyAsync(d.vals.p, a->values, sizeof(double) * a->nnz, hipMemcpyHostToDevice, st));
+                TSVD_HIP(hipMemcpyAsync(d.cols.p, a->cols, sizeof(int) * a->nnz, hipMemcpyHostToDevice, st));
+            }
+            TSVD_HIP(hipMemcpyAsync(d.pb.p, a->pointerB, sizeof(int) * m, hipMemcpyHostToDevice, st));
+            TSVD_HIP(hipMemcpyAsync(d.pe.p, a->pointerE, sizeof(int) * m, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_csr_to_dense, dim3(m), dim3(256), 0, st, d.vals.p, d.cols.p, d.pb.p, d.pe.p, m, d.A.p,
+                               d.ld);
+            TSVD_HIP(hipGetLastError());
+            items[(size_t)i] = FrpcaItem{d.A.p, d.ld, m, n, d.Qt.p, n, d.Q.p, d.Q2.p, d.ld, seed + (uint64_t)i};
         }
-        TSVD_HIP(hipMemcpyAsync(pb.p, A->pointerB, sizeof(int) * m, hipMemcpyHostToDevice, st));
-        TSVD_HIP(hipMemcpyAsync(pe.p, A->pointerE, sizeof(int) * m, hipMemcpyHostToDevice, st));
-        // validate the CSR on the host side (1-based, in range)
-        for (int r = 0; r < m; r++) {
-            TSVD_REQUIRE(A->pointerB[r] >= 1 && A->pointerE[r] >= A->pointerB[r] && A->pointerE[r] - 1 <= A->nnz,
-                         "bad row pointers at row %d", r);
-            for (int t = A->pointerB[r] - 1; t < A->pointerE[r] - 1; t++)
-                TSVD_REQUIRE(A->cols[t] >= 1 && A->cols[t] <= n, "column index out of range at row %d", r);
-        }
-        hipLaunchKernelGGL(k_csr_to_dense, dim3(m), dim3(256), 0, st, vals.p, cols.p, pb.p, pe.p, m, dA.p, ld);
-        TSVD_HIP(hipGetLastError());
-        std::vector<FrpcaItem> items(1);
-        items[0] = FrpcaItem{dA.p, ld, m, n, dQt.p, n, dQ.p, dQ2.p, ld, seed};
+        DBuf<double> G((size_t)l * l * count), W((size_t)l * l * count), lam((size_t)l * count),
+            Wk((size_t)l * k * count);
         frpca_core(items, TSVD_FP64, l, q, G.p, W.p, lam.p, ctx->gemm, ctx->eig, st);
-        std::vector<double> hl((size_t)l);
-        TSVD_HIP(hipMemcpyAsync(hl.data(), lam.p, sizeof(double) * l, hipMemcpyDeviceToHost, st));
+        std::vector<double> hl((size_t)l * count);
+        TSVD_HIP(hipMemcpyAsync(hl.data(), lam.p, sizeof(double) * hl.size(), hipMemcpyDeviceToHost, st));
         // U = Q V[:, s..s+k-1] (frpca.c:314-316), ascending
-        eig_select(Wk.p, G.p, lam.p, l, k, 1, 3, 1, 1e-4, st);
-        DBuf<double> dU((size_t)m * k), dV((size_t)n * k);
-        std::vector<GemmProb> p1(1);
-        p1[0] = GemmProb{};
-        p1[0].A = items[0].Q; p1[0].lda = ld; p1[0].B = Wk.p; p1[0].ldb = l; p1[0].C = dU.p; p1[0].ldc = m;
-        p1[0].M = m; p1[0].N = k; p1[0].K = l; p1[0].alpha = 1.0;
-        ctx->gemm.run(st, p1, GemmTypes{DT::F64, DT::F64, DT::F64, DT::F64}, k > 80);
-        // V = Qt V[:, sel] / S (eigSVD's U of Qt, frpca.c:118-131)
-        eig_select(Wk.p, G.p, lam.p, l, k, 1, 2, 1, 1e-4, st);
-        p1[0].A = dQt.p; p1[0].lda = n; p1[0].C = dV.p; p1[0].ldc = n; p1[0].M = n;
-        ctx->gemm.run(st, p1, GemmTypes{DT::F64, DT::F64, DT::F64, DT::F64}, k > 80);
-        TSVD_HIP(hipMemcpyAsync(U->d, dU.p, sizeof(double) * m * k, hipMemcpyDeviceToHost, st));
-        TSVD_HIP(hipMemcpyAsync(V->d, dV.p, sizeof(double) * (size_t)n * k, hipMemcpyDeviceToHost, st));
-        TSVD_HIP(hipStreamSynchronize(st));
-        for (int i = 0; i < k; i++) {
-            const double s = std::sqrt(std::max(hl[(size_t)(l - k + i)], 0.0));
-            S->d[i] = s > 1e-4 ? s : 0.0;
+        eig_select(Wk.p, G.p, lam.p, l, k, count, 3, 1, 1e-4, st);
+        std::vector<GemmProb> pu((size_t)count), pv((size_t)count);
+        for (int i = 0; i < count; i++) {
+            const Dev &d = dv[(size_t)i];
+            const FrpcaItem &it = items[(size_t)i];
+            GemmProb g{};
+            g.A = it.Q; g.lda = d.ld; g.B = Wk.p + (size_t)i * l * k; g.ldb = l; g.C = d.U.p; g.ldc = it.m;
+            g.M = it.m; g.N = k; g.K = l; g.alpha = 1.0;
+            pu[(size_t)i] = g;
         }
+        ctx->gemm.run(st, pu, GemmTypes{DT::F64, DT::F64, DT::F64, DT::F64}, k > 80);
+        // V = Qt V[:, sel] / S (eigSVD's U of Qt, frpca.c:118-131)
+        eig_select(Wk.p, G.p, lam.p, l, k, count, 2, 1, 1e-4, st);
+        for (int i = 0; i < count; i++) {
+            const Dev &d = dv[(size_t)i];
+            const FrpcaItem &it = items[(size_t)i];
+            GemmProb g{};
+            g.A = d.Qt.p; g.lda = it.nc; g.B = Wk.p + (size_t)i * l * k; g.ldb = l; g.C = d.V.p; g.ldc = it.nc;
+            g.M = it.nc; g.N = k; g.K = l; g.alpha = 1.0;
+            pv[(size_t)i] = g;
+        }
+        ctx->gemm.run(st, pv, GemmTypes{DT::F64, DT::F64, DT::F64, DT::F64}, k > 80);
+        for (int i = 0; i < count; i++) {
+            const Dev &d = dv[(size_t)i];
+            TSVD_HIP(hipMemcpyAsync(U[i]->d, d.U.p, d.U.bytes(), hipMemcpyDeviceToHost, st));
+            TSVD_HIP(hipMemcpyAsync(V[i]->d, d.V.p, d.V.bytes(), hipMemcpyDeviceToHost, st));
+        }
+        TSVD_HIP(hipStreamSynchronize(st));
+        for (int i = 0; i < count; i++)
+            for (int c = 0; c < k; c++) {
+                const double sv = std::sqrt(std::max(hl[(size_t)i * l + (size_t)(l - k + c)], 0.0));
+                S[i]->d[c] = sv > 1e-4 ? sv : 0.0;
+            }
     });
+}
+
+int frpca_csr_impl(const mat_csr *A, mat *U, mat *S, mat *V, int k, int q, uint64_t seed)
+{
+    return frpca_csr_batch_impl(&A, &U, &S, &V, 1, k, q, seed);
 }
 
 }  // namespace
@@ -306,6 +357,12 @@ extern "C" int tsvd_frpca_csr(const mat_csr *A, mat *U, mat *S, mat *V, int k, i
     return frpca_csr_impl(A, U, S, V, k, q, seed);
 }
 
+extern "C" int tsvd_frpca_csr_batch(const mat_csr *const *A, mat *const *U, mat *const *S, mat *const *V,
+                                    int count, int k, int q, uint64_t seed)
+{
+    return frpca_csr_batch_impl(A, U, S, V, count, k, q, seed);
+}
+
 extern "C" void frPCA(mat_csr *A, mat **U, mat **S, mat **V, int k, int q)
 {
     static uint64_t calls = 0;
@@ -313,5 +370,10 @@ extern "C" void frPCA(mat_csr *A, mat **U, mat **S, mat **V, int k, int q)
         set_error(TSVD_EINVAL, "frPCA: null argument");
         return;
     }
-    frpca_csr_impl(A, *U, *S, *V, k, q, 0x5eed0000ull + (calls++));
+    if (frpca_csr_impl(A, *U, *S, *V, k, q, 0x5eed0000ull + (calls++)) != TSVD_OK) {
+        // void like the reference: the outputs are left zero, the reason is in tsvd_last_error()
+        mat *o[3] = {*U, *S, *V};
+        for (mat *x : o)
+            if (x->d && x->nrows > 0 && x->ncols > 0) memset(x->d, 0, sizeof(double) * (size_t)x->nrows * x->ncols);
+    }
 }

@@ -26,7 +26,7 @@ EXPORTS = [
     "tsvd_read_lines", "tsvd_write_embedding_csv",
     "tsvd_ppr_create", "tsvd_ppr_destroy", "tsvd_ppr_refresh", "tsvd_ppr_push", "tsvd_ppr_read",
     "tsvd_ppr_last_push_stats", "tsvd_ppr_set_two_sided",
-    "frPCA", "tsvd_frpca_csr",
+    "frPCA", "tsvd_frpca_csr", "tsvd_frpca_csr_batch",
     "tsvd_pipeline_create", "tsvd_pipeline_destroy", "tsvd_pipeline_snapshot",
     "tsvd_pipeline_embedding", "tsvd_pipeline_root_sigma", "tsvd_pipeline_graph", "tsvd_pipeline_ppr",
     "tsvd_pipeline_block_matrix", "tsvd_pipeline_block_norms", "tsvd_pipeline_node", "tsvd_pipeline_block_columns",
@@ -130,6 +130,8 @@ def lib():
         if hasattr(L, "tsvd_frpca_csr"):
             L.tsvd_frpca_csr.argtypes = [ctypes.POINTER(MatCSR), ctypes.POINTER(Mat), ctypes.POINTER(Mat),
                                          ctypes.POINTER(Mat), ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
+            L.tsvd_frpca_csr_batch.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_uint64]
         if hasattr(L, "tsvd_pipeline_create"):
             L.tsvd_pipeline_create.argtypes = [ctypes.POINTER(Config), vp]
             L.tsvd_pipeline_destroy.argtypes = [vp]
@@ -346,10 +348,9 @@ class PPR:
         return {"rounds": a.value, "pops": b.value, "relax": c.value, "ms": t.value}
 
 
-def frpca_csr(A_dense_or_csr, k: int, q: int, seed: int = 1):
-    """frPCA through the drop-in C ABI (tsvd_frpca_csr).  Accepts a dense
-    numpy matrix or a scipy CSR matrix; builds the reference's 1-based
-    mat_csr (pointerB/pointerE) and returns U (m x k), S (k, ascending), V."""
+def _mat_csr(A_dense_or_csr):
+    """The reference's 1-based mat_csr (pointerB/pointerE) of a dense or scipy
+    matrix; returns the struct and the arrays it points into."""
     import scipy.sparse as sp
     A = sp.csr_matrix(A_dense_or_csr)
     A.sort_indices()
@@ -363,12 +364,45 @@ def frpca_csr(A_dense_or_csr, k: int, q: int, seed: int = 1):
     csr = MatCSR(int(A.nnz), m, n, vals.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                  cols.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), pb.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                  pe.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return csr, (vals, cols, pb, pe), (m, n)
+
+
+def _outputs(m, n, k):
     U = np.zeros((k, m)); S = np.zeros(k); V = np.zeros((k, n))     # column-major buffers
-    mu = Mat(m, k, U.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
-    ms = Mat(k, 1, S.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
-    mv = Mat(n, k, V.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    mats = (Mat(m, k, U.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+            Mat(k, 1, S.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+            Mat(n, k, V.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return (U, S, V), mats
+
+
+def frpca_csr(A_dense_or_csr, k: int, q: int, seed: int = 1):
+    """frPCA through the drop-in C ABI (tsvd_frpca_csr).  Accepts a dense
+    numpy matrix or a scipy CSR matrix; builds the reference's 1-based
+    mat_csr (pointerB/pointerE) and returns U (m x k), S (k, ascending), V."""
+    csr, keep, (m, n) = _mat_csr(A_dense_or_csr)
+    (U, S, V), (mu, ms, mv) = _outputs(m, n, k)
     check(lib().tsvd_frpca_csr(ctypes.byref(csr), ctypes.byref(mu), ctypes.byref(ms), ctypes.byref(mv), k, q, seed))
     return U.T.copy(), S, V.T.copy()
+
+
+def frpca_csr_batch(mats, k: int, q: int, seed: int = 1):
+    """tsvd_frpca_csr_batch over a list of matrices; item i uses seed + i.
+    Returns [(U, S, V), ...]."""
+    csrs, keep, outs, om = [], [], [], []
+    for A in mats:
+        csr, arrs, (m, n) = _mat_csr(A)
+        csrs.append(csr); keep.append(arrs)
+        bufs, ms_ = _outputs(m, n, k)
+        outs.append(bufs); om.append(ms_)
+    cnt = len(csrs)
+    if cnt == 0:
+        check(lib().tsvd_frpca_csr_batch(None, None, None, None, 0, k, q, seed))
+    PA = (ctypes.POINTER(MatCSR) * cnt)(*[ctypes.pointer(c) for c in csrs])
+    PU = (ctypes.POINTER(Mat) * cnt)(*[ctypes.pointer(o[0]) for o in om])
+    PS = (ctypes.POINTER(Mat) * cnt)(*[ctypes.pointer(o[1]) for o in om])
+    PV = (ctypes.POINTER(Mat) * cnt)(*[ctypes.pointer(o[2]) for o in om])
+    check(lib().tsvd_frpca_csr_batch(PA, PU, PS, PV, cnt, k, q, seed))
+    return [(U.T.copy(), S, V.T.copy()) for U, S, V in outs]
 
 
 class Pipeline:
