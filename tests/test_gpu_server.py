@@ -15,9 +15,11 @@ are per block, so the grouping acts through the reserves alone:
     the top half) and the top-half subspace within 1e-5 at rmax = 1e-7;
   * the reference's positive-side threshold leaves the insertion update's
     negative residues in place, and those depend on the grouping: root sigma
-    within 5e-2 (2e-2 top half), top-half subspace within 1e-2.
+    within 6e-2 (2.5e-2 top half), top-half subspace within 1.5e-2.
 Both bounds were sized on the oracle pipeline (same graph, grouped 1+3+4 vs
-one by one), which shows 4.5e-4 / 1.1e-4 / 4e-7 and 3.3e-2 / 6.8e-3 / 3.4e-3."""
+one by one), which shows 4.5e-4 / 1.1e-4 / 4e-7 and 3.3e-2 / 6.8e-3 / 3.4e-3;
+the device, grouped 1+2+6, shows 3.7e-4 / 1.9e-4 / 3.5e-7 and 3.1e-2 /
+1.1e-2 / 5.1e-3."""
 import threading
 
 import numpy as np
@@ -37,7 +39,7 @@ def workload(seed=4):
 
 ARGS = dict(alpha=0.2, rmax=1e-7, nparts=8, fanout=2, rank=12, q=6, precision=1, delta=0.0, seed=5,
             two_sided=True)
-TOL = {True: (2e-3, 5e-4, 1e-5), False: (5e-2, 2e-2, 1e-2)}   # sigma all, sigma top half, subspace
+TOL = {True: (2e-3, 5e-4, 1e-5), False: (6e-2, 2.5e-2, 1.5e-2)}   # sigma all, sigma top half, subspace
 
 
 def sigma_of(E):
