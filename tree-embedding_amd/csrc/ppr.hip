@@ -73,6 +73,63 @@ __device__ inline void mark(unsigned long long bits, uint32_t item, unsigned lon
     if (old == 0ull) append_item(item, list, count, cap, overflow);
 }
 
+// Staged appends.  Returning atomics on one address retire at about 9e7 per
+// second for the whole device, so one global append per item would serialise
+// a round of millions of items on the list's counter.  A workgroup instead
+// collects its new items in LDS (LDS atomics) and reserves their place in
+// the list with a single global atomic when it finishes; past STAGE items it
+// appends directly.
+constexpr uint32_t STAGE = 4096;
+
+// Push counters are spread over STAT_SHARDS copies (by workgroup), for the
+// same reason: one address per counter would serialise a workgroup's end.
+constexpr int STAT_SHARDS = 32, STAT_FIELDS = 8;
+
+__device__ inline unsigned long long *stat_slot(unsigned long long *stats)
+{
+    return stats + (blockIdx.x % STAT_SHARDS) * STAT_FIELDS;
+}
+
+struct Stage {
+    uint32_t *items;    // LDS [STAGE]
+    uint32_t *n;        // LDS: items staged
+    uint32_t *base;     // LDS: reserved list position
+    uint32_t *list;     // the global item list, its counter and capacity
+    uint32_t *count;
+    uint32_t cap;
+    uint32_t *overflow;
+};
+
+__device__ inline void stage_mark(const Stage &sg, unsigned long long bits, uint32_t item, unsigned long long *mask)
+{
+    const unsigned long long old = atomicOr(mask + item, bits);
+    if (old != 0ull) return;
+    const uint32_t pos = atomicAdd(sg.n, 1u);
+    if (pos < STAGE) sg.items[pos] = item;
+    else append_item(item, sg.list, sg.count, sg.cap, sg.overflow);
+}
+
+// Every thread of the workgroup calls this once, after its last stage_mark.
+__device__ inline void stage_flush(const Stage &sg)
+{
+    __syncthreads();
+    const uint32_t n = min(*sg.n, STAGE);
+    if (threadIdx.x == 0) *sg.base = n ? atomicAdd(sg.count, n) : 0u;
+    __syncthreads();
+    const uint32_t b = *sg.base;
+    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
+        if (b + t < sg.cap) sg.list[b + t] = sg.items[t];
+        else atomicOr(sg.overflow, 1u);
+    }
+}
+
+#define TSVD_STAGE(name, list_, count_, cap_, overflow_)                                     \
+    __shared__ uint32_t name##_items[STAGE];                                                 \
+    __shared__ uint32_t name##_n, name##_base;                                               \
+    if (threadIdx.x == 0) name##_n = 0u;                                                     \
+    __syncthreads();                                                                         \
+    const Stage name{name##_items, &name##_n, &name##_base, (list_), (count_), (cap_), (overflow_)}
+
 template <class T>
 __global__ void k_zero_range(T *__restrict__ P, T *__restrict__ R, int32_t n, int32_t s_pad, int32_t s0, int32_t s1)
 {
@@ -110,9 +167,10 @@ __global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict
                                                 uint32_t *__restrict__ count, uint32_t cap,
                                                 uint32_t *__restrict__ overflow)
 {
+    TSVD_STAGE(sg, list, count, cap, overflow);
     const int64_t wave = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (wave >= (int64_t)ntouched * ngroups) return;
+    if (wave < (int64_t)ntouched * ngroups) {
     const int32_t t = (int32_t)(wave / ngroups);
     const int32_t g = (int32_t)(wave - (int64_t)t * ngroups);
     const int32_t i = touched[t];
@@ -123,30 +181,56 @@ __global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict
     T *Pi = P + (int64_t)i * s_pad + s;
     T *Ri = R + (int64_t)i * s_pad + s;
     T pv = active ? *Pi : T(0);
-    for (int32_t j = f0; j < d1; j++) {
-        if (j == 0) continue;                                   // ppr:2078-2079
-        const int32_t to = adj[base + j];
-        bool cross = false;
-        if (active) {
-            pv = (T)((double)pv * ((double)(j + 1) / (double)j));
-            const double pj = (double)pv / (double)(j + 1) / alpha;
-            atomicAdd(Ri, (T)(-pj));
-            const T add = (T)((1.0 - alpha) * pj);
-            T old = atomicAdd(R + (int64_t)to * s_pad + s, add);
-            cross = !beyond(old, rm, two) && beyond(old + add, rm, two);
+    // The row's own decrements are summed here and applied with one atomic at
+    // the end: nothing else lowers r[i], so its final value, and the check of
+    // it below, are the same as with one atomic per slot.  The additions to
+    // the new neighbours go out U at a time before their results are used.
+    // Slot 0 is skipped (ppr:2078-2079).
+    constexpr int U = 8;
+    double dec = 0.0;
+    const int32_t jb = max(f0, 1);
+    for (int32_t blk = jb; blk < d1; blk += 64) {
+        const int32_t cnt = min(64, d1 - blk);
+        const int32_t mine = lane < cnt ? adj[base + blk + lane] : 0;
+        for (int32_t q0 = 0; q0 < cnt; q0 += U) {
+            int32_t to[U];
+            T add[U], old[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                to[q] = __builtin_amdgcn_readlane(mine, min(q0 + q, 63));
+                add[q] = T(0);
+                const int32_t j = blk + q0 + q;
+                if (active && q0 + q < cnt) {
+                    pv = (T)((double)pv * ((double)(j + 1) / (double)j));
+                    const double pj = (double)pv / (double)(j + 1) / alpha;
+                    dec += pj;
+                    add[q] = (T)((1.0 - alpha) * pj);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                old[q] = T(0);
+                if (active && q0 + q < cnt) old[q] = atomicAdd(R + (int64_t)to[q] * s_pad + s, add[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                if (q0 + q >= cnt) break;
+                const bool cross = active & !beyond(old[q], rm, two) & beyond(old[q] + add[q], rm, two);
+                const unsigned long long c = __ballot(cross);
+                if (c && lane == 0 && deg[to[q]] > 0) stage_mark(sg, c, (uint32_t)to[q] * ngroups + g, cm);
+            }
         }
-        unsigned long long c = __ballot(cross);
-        if (c && lane == 0 && deg[to] > 0)
-            mark(c, (uint32_t)to * ngroups + g, cm, list, count, cap, overflow);
     }
     bool self = false;
     if (active) {
         *Pi = pv;
-        T x = atomicAdd(Ri, T(0));
-        self = beyond(x, rm, two) && d1 > 0;
+        const T o = atomicAdd(Ri, (T)(-dec));
+        self = beyond((T)(o - (T)dec), rm, two) && d1 > 0;
     }
     unsigned long long c2 = __ballot(self);
-    if (c2 && lane == 0) mark(c2, (uint32_t)i * ngroups + g, cm, list, count, cap, overflow);
+    if (c2 && lane == 0) stage_mark(sg, c2, (uint32_t)i * ngroups + g, cm);
+    }
+    stage_flush(sg);
 }
 
 // Relax neighbours nb[j0, j1) of a popped (v, g) item: one coalesced 64-lane
@@ -156,8 +240,7 @@ template <class T>
 __device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, int32_t j1, bool pushing, T delta,
                                    T *__restrict__ Rs, int32_t s_pad, const int32_t *__restrict__ deg, uint32_t G,
                                    int32_t g, T rm, int two, int lane, unsigned long long *__restrict__ nm,
-                                   uint32_t *__restrict__ next, uint32_t *__restrict__ next_count, uint32_t cap,
-                                   uint32_t *__restrict__ overflow)
+                                   const Stage &sg)
 {
     // The neighbour ids of 64 consecutive slots arrive in one coalesced load
     // (one id per lane) and are broadcast with readlane, so each relaxation
@@ -189,8 +272,7 @@ __device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, i
 #pragma unroll
             for (int q = 0; q < U; q++) {
                 const unsigned long long b = __ballot(c[q]);
-                if (b && lane == 0 && deg[u[q]] > 0)
-                    mark(b, (uint32_t)u[q] * G + g, nm, next, next_count, cap, overflow);
+                if (b && lane == 0 && deg[u[q]] > 0) stage_mark(sg, b, (uint32_t)u[q] * G + g, nm);
             }
         }
         if (q0 < cnt) {
@@ -213,8 +295,7 @@ __device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, i
             for (int q = 0; q < U; q++) {
                 if (q >= r) break;
                 const unsigned long long b = __ballot(c[q]);
-                if (b && lane == 0 && deg[u[q]] > 0)
-                    mark(b, (uint32_t)u[q] * G + g, nm, next, next_count, cap, overflow);
+                if (b && lane == 0 && deg[u[q]] > 0) stage_mark(sg, b, (uint32_t)u[q] * G + g, nm);
             }
         }
     }
@@ -253,7 +334,7 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
     // [4] relaxations done here (split rows hand the rest to k_push_hub)
     __shared__ unsigned long long sstat[5];
     if (threadIdx.x < 5) sstat[threadIdx.x] = 0ull;
-    __syncthreads();
+    TSVD_STAGE(sg, next, next_count, cap, overflow);
     const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (w < nitems) {
@@ -283,7 +364,7 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
             }
         }
         const unsigned long long ca = __ballot(again);
-        if (ca && lane == 0) mark(ca, item, nm, next, next_count, cap, overflow);
+        if (ca && lane == 0) stage_mark(sg, ca, item, nm);
         unsigned long long pm = __ballot(pushing);
         if (pm != 0ull && ride_min < rm && dv > 0) {
             // The row is walked anyway: lanes whose residue is positive but at or
@@ -303,7 +384,7 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                 }
             }
             const unsigned long long cb = __ballot(back);
-            if (cb && lane == 0) mark(cb, item, nm, next, next_count, cap, overflow);
+            if (cb && lane == 0) stage_mark(sg, cb, item, nm);
             pm = __ballot(pushing);
         }
         if (pm != 0ull) {
@@ -342,12 +423,11 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                 }
             }
             if (lane == 0) atomicAdd(&sstat[4], (unsigned long long)__popcll(pm) * (unsigned long long)jend);
-            relax_range(nb, 0, jend, pushing, delta, R + s, s_pad, deg, (uint32_t)ngroups, g, rm, two, lane, nm, next,
-                        next_count, cap, overflow);
+            relax_range(nb, 0, jend, pushing, delta, R + s, s_pad, deg, (uint32_t)ngroups, g, rm, two, lane, nm, sg);
         }
     }
-    __syncthreads();
-    if (threadIdx.x < 5 && sstat[threadIdx.x]) atomicAdd(stats + threadIdx.x, sstat[threadIdx.x]);
+    stage_flush(sg);
+    if (threadIdx.x < 5 && sstat[threadIdx.x]) atomicAdd(stat_slot(stats) + threadIdx.x, sstat[threadIdx.x]);
 }
 
 
@@ -363,19 +443,22 @@ __global__ void __launch_bounds__(TPB) k_push_hub(T *__restrict__ R, const uint3
                                                   const int32_t *__restrict__ adj, const int32_t *__restrict__ deg,
                                                   int32_t s_pad, int32_t ngroups, T rm, int two)
 {
+    TSVD_STAGE(sg, next, next_count, cap, overflow);
     const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (w >= nchunks) return;
-    const unsigned long long e = hub_chunks[w];
-    const uint32_t slot = (uint32_t)(e >> 32), c = (uint32_t)(e & 0xFFFFFFFFull);
-    const uint32_t item = hub_item[slot];
-    const int32_t v = (int32_t)(item / (uint32_t)ngroups);
-    const int32_t g = (int32_t)(item - (uint32_t)v * (uint32_t)ngroups);
-    const T delta = hub_delta[(int64_t)slot * 64 + lane];
-    const int32_t dv = deg[v];
-    const int32_t j0 = (int32_t)c * HUB_CHUNK, j1 = min(dv, j0 + HUB_CHUNK);
-    relax_range(adj + rowoff[v], j0, j1, delta != T(0), delta, R + (int64_t)g * 64 + lane, s_pad, deg,
-                (uint32_t)ngroups, g, rm, two, lane, nm, next, next_count, cap, overflow);
+    if (w < nchunks) {
+        const unsigned long long e = hub_chunks[w];
+        const uint32_t slot = (uint32_t)(e >> 32), c = (uint32_t)(e & 0xFFFFFFFFull);
+        const uint32_t item = hub_item[slot];
+        const int32_t v = (int32_t)(item / (uint32_t)ngroups);
+        const int32_t g = (int32_t)(item - (uint32_t)v * (uint32_t)ngroups);
+        const T delta = hub_delta[(int64_t)slot * 64 + lane];
+        const int32_t dv = deg[v];
+        const int32_t j0 = (int32_t)c * HUB_CHUNK, j1 = min(dv, j0 + HUB_CHUNK);
+        relax_range(adj + rowoff[v], j0, j1, delta != T(0), delta, R + (int64_t)g * 64 + lane, s_pad, deg,
+                    (uint32_t)ngroups, g, rm, two, lane, nm, sg);
+    }
+    stage_flush(sg);
 }
 
 // ---------------------------------------------------------------- pull rounds
@@ -437,7 +520,7 @@ __global__ void __launch_bounds__(TPB) k_pull_pop(T *__restrict__ P, T *__restri
         }
     }
     __syncthreads();
-    if (threadIdx.x < 4 && sstat[threadIdx.x]) atomicAdd(stats + threadIdx.x, sstat[threadIdx.x]);
+    if (threadIdx.x < 4 && sstat[threadIdx.x]) atomicAdd(stat_slot(stats) + threadIdx.x, sstat[threadIdx.x]);
 }
 
 template <class T>
@@ -686,13 +769,15 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
         TSVD_HIP(hipGetLastError());
         TSVD_HIP(hipMemcpyAsync(h + 2, cnt + 2, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         if (g_profile)
-            TSVD_HIP(hipMemcpyAsync(h + 16, s->stats.p, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+            TSVD_HIP(hipMemcpyAsync(s->h_stats, s->stats.p, s->stats.bytes(), hipMemcpyDeviceToHost, st));
         TSVD_HIP(hipStreamSynchronize(st));
         if (g_profile) {
             // algorithmic bytes per relaxation (column id + residue RMW) and per pop
             // (row offsets, residue read/clear, reserve RMW); relaxations of split
             // rows beyond their first chunk belong to k_push_hub
-            const unsigned long long *hs = (const unsigned long long *)(h + 16);
+            unsigned long long hs[STAT_FIELDS] = {};
+            for (int sh = 0; sh < STAT_SHARDS; sh++)
+                for (int f = 0; f < STAT_FIELDS; f++) hs[f] += s->h_stats[sh * STAT_FIELDS + f];
             const double relax = (double)(hs[1] - prof_prev[1]), pops = (double)(hs[0] - prof_prev[0]);
             const double own = pull ? relax : (double)(hs[4] - prof_prev[2]);
             const double per_relax = sizeof(T) == 8 ? 20.0 : 12.0, per_pop = sizeof(T) == 8 ? 48.0 : 32.0;
@@ -710,9 +795,11 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
     }
     tm.stop(st);
     TSVD_HIP(hipMemsetAsync(cnt + d, 0, sizeof(uint32_t), st));
-    unsigned long long hs[4] = {0, 0, 0, 0};
-    TSVD_HIP(hipMemcpyAsync(hs, s->stats.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+    TSVD_HIP(hipMemcpyAsync(s->h_stats, s->stats.p, s->stats.bytes(), hipMemcpyDeviceToHost, st));
     TSVD_HIP(hipStreamSynchronize(st));
+    unsigned long long hs[STAT_FIELDS] = {};
+    for (int sh = 0; sh < STAT_SHARDS; sh++)
+        for (int f = 0; f < STAT_FIELDS; f++) hs[f] += s->h_stats[sh * STAT_FIELDS + f];
     s->last_rounds[d] = rounds;
     s->last_pops[d] = (int64_t)hs[0];
     s->last_relax[d] = (int64_t)hs[1];
@@ -803,7 +890,8 @@ extern "C" int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t ns
             }
             TSVD_HIP(hipMemsetAsync(s->nm.p, 0, s->nm.bytes(), st));
             s->counters.alloc(8);
-            s->stats.alloc(5);
+            s->stats.alloc((size_t)STAT_SHARDS * STAT_FIELDS);
+            TSVD_HIP(hipHostMalloc((void **)&s->h_stats, s->stats.bytes(), hipHostMallocDefault));
             TSVD_HIP(hipMemsetAsync(s->counters.p, 0, s->counters.bytes(), st));
             TSVD_HIP(hipHostMalloc((void **)&s->h_counters, 128, hipHostMallocDefault));
             TSVD_HIP(hipStreamSynchronize(st));

@@ -27,6 +27,7 @@ struct tsvd_ppr {
     tsvd::DBuf<unsigned long long> hub_chunks;  //   (slot, chunk) work entries
     tsvd::DBuf<unsigned long long> stats;    // [0] pops, [1] relaxations
     uint32_t *h_counters = nullptr;          // pinned
+    unsigned long long *h_stats = nullptr;   // pinned copy of the sharded push counters
     int64_t last_rounds[2] = {0, 0}, last_pops[2] = {0, 0}, last_relax[2] = {0, 0};
     int64_t last_items[2] = {0, 0}, last_wave_relax[2] = {0, 0};
     double last_ms[2] = {0, 0};
@@ -45,6 +46,7 @@ struct tsvd_ppr {
     ~tsvd_ppr()
     {
         if (h_counters) (void)hipHostFree(h_counters);
+        if (h_stats) (void)hipHostFree(h_stats);
     }
     void *p(int d) { return P[d].p; }
     void *r(int d) { return R[d].p; }
