@@ -305,7 +305,7 @@ __device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, i
 // the first chunk and hands the rest, with its 64 pushed amounts, to
 // k_push_hub (one wave per further chunk), so a hub does not serialise a
 // round behind one wavefront.
-constexpr int HUB_CHUNK = 1024;
+constexpr int HUB_CHUNK = 4096;   // default; TSVD_HUB_CHUNK overrides it per PPR handle
 
 // One synchronous push round: one wave per pending item pops its lanes and
 // relaxes v's neighbours.  The atomic add returns the previous residue, so a
@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                                                     uint32_t *__restrict__ hub_item, T *__restrict__ hub_delta,
                                                     unsigned long long *__restrict__ hub_chunks,
                                                     uint32_t *__restrict__ hub_counts, uint32_t hub_cap,
-                                                    uint32_t chunk_cap)
+                                                    uint32_t chunk_cap, int32_t hub_chunk)
 {
     // [0] pops, [1] relaxations (lanes x degree), [2] items, [3] row visits,
     // [4] relaxations done here (split rows hand the rest to k_push_hub)
@@ -397,8 +397,8 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
             const T delta = pushing ? (T)((1.0 - alpha) * (double)x / (double)dv) : T(0);
             const int32_t *nb = adj + rowoff[v];
             int32_t jend = dv;
-            if (dv > HUB_CHUNK) {
-                const uint32_t more = (uint32_t)((dv - 1) / HUB_CHUNK);   // chunks after the first
+            if (dv > hub_chunk) {
+                const uint32_t more = (uint32_t)((dv - 1) / hub_chunk);   // chunks after the first
                 uint32_t slot = 0xFFFFFFFFu, cbase = 0;
                 if (lane == 0) {
                     slot = atomicAdd(hub_counts + 0, 1u);
@@ -419,7 +419,7 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                     hub_delta[(int64_t)slot * 64 + lane] = delta;
                     for (uint32_t c = lane; c < more; c += 64)
                         hub_chunks[cbase + c] = ((unsigned long long)slot << 32) | (unsigned long long)(c + 1);
-                    jend = HUB_CHUNK;
+                    jend = hub_chunk;
                 }
             }
             if (lane == 0) atomicAdd(&sstat[4], (unsigned long long)__popcll(pm) * (unsigned long long)jend);
@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(TPB) k_push_hub(T *__restrict__ R, const uint3
                                                   uint32_t *__restrict__ overflow,
                                                   const int64_t *__restrict__ rowoff,
                                                   const int32_t *__restrict__ adj, const int32_t *__restrict__ deg,
-                                                  int32_t s_pad, int32_t ngroups, T rm, int two)
+                                                  int32_t s_pad, int32_t ngroups, T rm, int two, int32_t hub_chunk)
 {
     TSVD_STAGE(sg, next, next_count, cap, overflow);
     const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
@@ -454,7 +454,7 @@ __global__ void __launch_bounds__(TPB) k_push_hub(T *__restrict__ R, const uint3
         const int32_t g = (int32_t)(item - (uint32_t)v * (uint32_t)ngroups);
         const T delta = hub_delta[(int64_t)slot * 64 + lane];
         const int32_t dv = deg[v];
-        const int32_t j0 = (int32_t)c * HUB_CHUNK, j1 = min(dv, j0 + HUB_CHUNK);
+        const int32_t j0 = (int32_t)c * hub_chunk, j1 = min(dv, j0 + hub_chunk);
         relax_range(adj + rowoff[v], j0, j1, delta != T(0), delta, R + (int64_t)g * 64 + lane, s_pad, deg,
                     (uint32_t)ngroups, g, rm, two, lane, nm, sg);
     }
@@ -737,9 +737,9 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
                                s->F[d].p, count, s->ngroups);
             s->last_pull_rounds[d]++;
         } else {
-            // split rows: at most one chunk list entry per (hub row, group) per HUB_CHUNK neighbours
+            // split rows: at most one chunk list entry per (hub row, group) per hub_chunk neighbours
             const uint32_t hub_cap = (uint32_t)std::min<int64_t>((int64_t)cap, (int64_t)s->ngroups *
-                                                                 (ceil_div(D.cap_total, HUB_CHUNK) + 1));
+                                                                 (ceil_div(D.cap_total, s->hub_chunk) + 1));
             const uint32_t chunk_cap = hub_cap;
             s->hub_item.ensure(std::max<uint32_t>(hub_cap, 1));
             s->hub_delta.ensure((size_t)std::max<uint32_t>(hub_cap, 1) * 64 * sizeof(T));
@@ -753,7 +753,7 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
                                    s->two_sided, s->directed, (T)(std::min(s->ride_frac, 1.0) * s->rmax), s0, s1,
                                    s->stats.p,
                                    s->hub_item.p, (T *)s->hub_delta.p, s->hub_chunks.p,
-                                   cnt + 4, hub_cap, chunk_cap);
+                                   cnt + 4, hub_cap, chunk_cap, s->hub_chunk);
             }
             TSVD_HIP(hipMemcpyAsync(h + 8, cnt + 4, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
             TSVD_HIP(hipStreamSynchronize(st));
@@ -763,7 +763,7 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
                 hipLaunchKernelGGL(k_push_hub<T>, dim3((unsigned)ceil_div(nchunks, WPB)), dim3(TPB), 0, st, R,
                                    s->hub_item.p, (const T *)s->hub_delta.p, s->hub_chunks.p, nchunks, s->nm.p,
                                    s->fnext.p, cnt + 2, cap, cnt + 3, D.rowoff.p, D.adj.p, D.deg.p, s->s_pad,
-                                   s->ngroups, (T)s->rmax, s->two_sided);
+                                   s->ngroups, (T)s->rmax, s->two_sided, s->hub_chunk);
             }
         }
         TSVD_HIP(hipGetLastError());
@@ -866,6 +866,9 @@ extern "C" int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t ns
             if (const char *e = getenv("TSVD_PULL_FRAC")) s->pull_frac = atof(e);
             // lanes above ride_frac * rmax ride along with a pushing item (>= 1: off)
             if (const char *e = getenv("TSVD_RIDE_FRAC")) s->ride_frac = atof(e);
+            // rows longer than this many neighbours are split across waves
+            s->hub_chunk = HUB_CHUNK;
+            if (const char *e = getenv("TSVD_HUB_CHUNK")) s->hub_chunk = std::max(64, atoi(e));
             s->h_sources.assign(sources, sources + nsrc);
             s->sources.alloc((size_t)nsrc);
             hipStream_t st = g->stream;

@@ -40,8 +40,9 @@ struct tsvd_ppr {
     tsvd::DBuf<int32_t> wl_cnt;              // n + 1 scratch
     int64_t wl_n[2] = {0, 0}, wl_version[2] = {-1, -1};
     int two_sided = 0;                       // 1: push |r| > rmax, 0: r > rmax (the reference)
+    int32_t hub_chunk = 4096;                // rows longer than this are split across waves
     double pull_frac = 0.0;                  // pull when items > pull_frac * n (0: never)
-    double ride_frac = 0.0;                  // sub-threshold lanes above ride_frac*rmax ride along (>= 1: off)
+    double ride_frac = 0.5;                  // sub-threshold lanes above ride_frac*rmax ride along (>= 1: off)
     std::vector<int32_t> h_sources;
     ~tsvd_ppr()
     {
