@@ -107,6 +107,17 @@ int tsvd_ppr_destroy(tsvd_ppr *s);
 /* mode TSVD_REFRESH_INIT / _DYNAMIC for sources [src_begin, src_end). */
 int tsvd_ppr_refresh(tsvd_ppr *s, int mode, int dir, int32_t src_begin, int32_t src_end);
 int tsvd_ppr_push(tsvd_ppr *s, int dir, int32_t src_begin, int32_t src_end);
+/* The insertion update of the n >= 1e7-node path (the refresh half of
+ * All_initialization_and_push_final, ppr:6504-6533 forward, :6675-6704
+ * transpose) for sources [src_begin, src_end), after tsvd_graph_append of the
+ * same batch (src, dst: host ids, or device ids with on_device).  Edge by
+ * edge in batch order the reference does: j = max(former degree, edges of
+ * `from` seen so far); j == 0 -> r[from] = 0; else p *= (j+1)/j,
+ * r[from] -= p/(j+1)/alpha, r[to] += (1-alpha) p/(j+1)/alpha (dir 1 swaps
+ * the ends and uses in-degrees).  Same result, computed edge-parallel; the
+ * vertices left with r > rmax are queued for tsvd_ppr_push.               */
+int tsvd_ppr_refresh_edges(tsvd_ppr *s, int dir, const int32_t *src, const int32_t *dst, int64_t m, int on_device,
+                           int32_t src_begin, int32_t src_end);
 /* Host copy of p or r (what = 0: reserve p, 1: residue r) as FP64, source-
  * major [nsrc][n] like the reference's float** pi / residue.              */
 int tsvd_ppr_read(tsvd_ppr *s, int dir, int what, double *out);
@@ -171,6 +182,15 @@ typedef struct {
     uint64_t seed;        /* sketch seed                                   */
     int32_t two_sided;    /* 0: push r > rmax (reference), 1: |r| > rmax   */
     int32_t right_embedding; /* LP only: keep V_j = A_j^T U of dirty blocks */
+    /* 1: the n >= 1e7-node path of Tree_d_LP.cpp:1377-2024 (directed, LP):
+     * reserves and residues kept between snapshots as positive-only pair
+     * lists (All_initialization_and_push_final, ppr:6188-6832; negative
+     * residues dropped), the insertion update in batch order (first edge of
+     * a vertex of former degree 0 clears its residue), blocks from p + p^T
+     * with rows >= 30 floor(|S|/30) left empty (ppr:6898-7229).  The state
+     * is pushed a batch of source groups at a time. */
+    int32_t sparse_state;
+    int32_t sparse_groups;   /* groups of 64 sources per batch (0: as HBM allows) */
 } tsvd_config;
 
 typedef struct {
@@ -220,13 +240,21 @@ int tsvd_pipeline_dense_gram(tsvd_pipeline *p, double *G);
  * left_matrix_U_cache / left_matrix_index_mapping). */
 /* Host copy of V rows [v0, v1) (row-major, rank columns; inactive vertices 0). */
 int tsvd_pipeline_right_embedding(tsvd_pipeline *p, int32_t v0, int32_t v1, double *V);
-/* scores[e] for pairs (src[e], dst[e]); src must be subset vertices. */
+/* scores[e] for pairs (src[e], dst[e]); src must be subset vertices.  With
+ * sparse_state the score also carries out_degree(i) * in_degree(v) of the
+ * snapshot that computed block(v)'s V, as the n >= 1e7-node path's scores do
+ * (ppr:7604-7730). */
 int tsvd_pipeline_lp_scores(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, int64_t m, double *scores);
 /* Z = A A^T Y (nsrc x l, row-major FP64 in and out) for the same matrix,
  * without forming A A^T: two FP64 products per slab of columns.  Subspace
  * iteration on it gives the exact top singular values where the Gram is too
  * large to form; no reference counterpart.                                */
 int tsvd_pipeline_gram_apply(tsvd_pipeline *p, const double *Y, int32_t l, double *Z);
+/* Reserves and residues (n values each, FP64; either pointer may be NULL) of
+ * subset source i (0 <= i < nsrc) in push direction dir after the last
+ * snapshot: the device state, or with sparse_state the stored pair lists
+ * (the reference's tripletList / residue_pair_list of that source).        */
+int tsvd_pipeline_source_state(tsvd_pipeline *p, int32_t dir, int32_t i, double *reserve, double *residue);
 
 /* ---------------------------------------------------------- served form
  * No reference counterpart: the reference's drivers run the whole update once

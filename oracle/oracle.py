@@ -53,6 +53,8 @@ def lib():
         L.orc_ppr_push.argtypes = [vp, vp, i32, i32]
         L.orc_ppr_counts.argtypes = [vp, vp, vp]
         L.orc_ppr_set_two_sided.argtypes = [vp, i32]
+        L.orc_ppr_refresh_edges.argtypes = [vp, vp, i32, vp, vp, i64, i32]
+        L.orc_ppr_drop_negative.argtypes = [vp, i32]
         _lib = L
     return _lib
 
@@ -140,6 +142,19 @@ class PPR:
     def push(self, g: Graph, nthreads=1):
         for d in self.dirs():
             lib().orc_ppr_push(self.h, g.h, d, nthreads)
+
+    def refresh_edges(self, g: Graph, src, dst, nthreads=1):
+        """The n >= 1e7-node path's insertion update, edge by edge in batch
+        order (ppr:6504-6533, 6675-6704; Q8 included)."""
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        dst = np.ascontiguousarray(dst, dtype=np.int32)
+        for d in self.dirs():
+            lib().orc_ppr_refresh_edges(self.h, g.h, d, _ptr(src), _ptr(dst), src.size, nthreads)
+
+    def drop_negative(self):
+        """Keep only positive entries, as the pair lists do (ppr:6598-6626)."""
+        for d in self.dirs():
+            lib().orc_ppr_drop_negative(self.h, d)
 
     def counts(self):
         """(pushes, relaxations) since the last call."""
@@ -357,7 +372,7 @@ class Pipeline:
 
     def __init__(self, n, directed, sources, alpha, rmax, nparts, fanout, k, q,
                  mode="NC", delta=0.65, dyn_start=2, lazy_start=2, fp32=True,
-                 seed=0, nthreads=os.cpu_count() or 1, two_sided=False):
+                 seed=0, nthreads=os.cpu_count() or 1, two_sided=False, sparse_state=False):
         self.g = Graph(n, directed)
         self.ppr = PPR(n, sources, directed, alpha, rmax, fp32=fp32, two_sided=two_sided)
         self.sources = np.asarray(sources)
@@ -380,8 +395,19 @@ class Pipeline:
         self.U_of = None
         self.it = 0
         self.last_dirty = []
+        # the n >= 1e7-node path (Tree_d_LP.cpp:1377-2024): directed LP only
+        self.sparse_state = sparse_state
+        if sparse_state and not (directed and mode == "LP" and not two_sided):
+            raise ValueError("sparse_state is the directed LP path with the positive-side threshold")
+        self.keep = 30 * (nsrc // 30)
 
     def matrix(self):
+        if self.sparse_state:
+            # ppr:6898-6960: forward and transpose reserve lists summed, rows
+            # inserted in 30 intervals of floor(|S|/30) (the rest stay empty, Q11)
+            X = self.ppr.p(0) + self.ppr.p(1)
+            X[self.keep:] = 0.0
+            return X
         return self.ppr.M()
 
     def active(self):
@@ -396,13 +422,18 @@ class Pipeline:
         self.it += 1
         it = self.it
         self.g.append(src, dst)
-        if self.lazy_start != 100 and it < self.lazy_start - 1:   # Tree_u_NC.cpp:516-518
+        # Tree_u_NC.cpp:516-518 (sentinel 200 in the n >= 1e7-node path, Tree_d_LP.cpp:1574)
+        if self.lazy_start != (200 if self.sparse_state else 100) and it < self.lazy_start - 1:
             return self.E
         if it < self.dyn_start:
             self.ppr.init_static()
+        elif self.sparse_state:
+            self.ppr.refresh_edges(self.g, src, dst, self.nthreads)
         else:
             self.ppr.refresh_dynamic(self.g, self.nthreads)
         self.ppr.push(self.g, self.nthreads)
+        if self.sparse_state:
+            self.ppr.drop_negative()
         M = self.matrix()
         act = self.active()
         dirty = []
@@ -410,7 +441,8 @@ class Pipeline:
             c0, w = self.starts[j], self.widths[j]
             A = transform(M[:, c0:c0 + w], self.rmax, self.mode)
             d = True
-            if it >= self.lazy_start:
+            # the n >= 1e7-node path tests from dyn_start on (Tree_d_LP.cpp:1658)
+            if it >= (self.dyn_start if self.sparse_state else self.lazy_start):
                 d, _, _ = dirty_test(A, self.A_old[j], self.resid[j], self.delta)
             if d:
                 self.A_old[j] = A

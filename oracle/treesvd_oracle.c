@@ -233,9 +233,13 @@ static inline void q_push(orc_ppr *s, int d, int k, int v)
     s->qt[d][k] = (s->qt[d][k] + 1) % (s->n + 1);
 }
 
-typedef struct { orc_ppr *s; const orc_graph *g; int dir, k0, k1; } job_t;
+typedef struct {
+    orc_ppr *s; const orc_graph *g; int dir, k0, k1;
+    const int *es, *ed; long long m;      /* edge batch (refresh_edges only) */
+} job_t;
 
-static void run_parallel(void *(*fn)(void *), orc_ppr *s, const orc_graph *g, int dir, int nthreads)
+static void run_parallel_edges(void *(*fn)(void *), orc_ppr *s, const orc_graph *g, int dir, int nthreads,
+                               const int *es, const int *ed, long long m)
 {
     if (nthreads < 1) nthreads = 1;
     if (nthreads > s->nsrc) nthreads = s->nsrc > 0 ? s->nsrc : 1;
@@ -245,12 +249,18 @@ static void run_parallel(void *(*fn)(void *), orc_ppr *s, const orc_graph *g, in
     /* source ranges as in the drivers' fork loop (Tree_u_NC.cpp:529-557) */
     for (int t = 0; t < nthreads; t++) {
         jobs[t].s = s; jobs[t].g = g; jobs[t].dir = dir;
+        jobs[t].es = es; jobs[t].ed = ed; jobs[t].m = m;
         jobs[t].k0 = t * (s->nsrc / nthreads);
         jobs[t].k1 = (t == nthreads - 1) ? s->nsrc : (t + 1) * (s->nsrc / nthreads);
     }
     if (nthreads == 1) { fn(&jobs[0]); return; }
     for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, fn, &jobs[t]);
     for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+}
+
+static void run_parallel(void *(*fn)(void *), orc_ppr *s, const orc_graph *g, int dir, int nthreads)
+{
+    run_parallel_edges(fn, s, g, dir, nthreads, NULL, NULL, 0);
 }
 
 /* Static branch of nodegree_DenseUndirected_Refresh_PPR_initialization
@@ -306,6 +316,65 @@ static void *refresh_job(void *arg)
 ORC_EXPORT void orc_ppr_refresh_dynamic(orc_ppr *s, const orc_graph *g, int dir, int nthreads)
 {
     run_parallel(refresh_job, s, g, dir, nthreads);
+}
+
+/* Dynamic branch of All_initialization_and_push_final, the n >= 1e7-node path
+ * (ppr_computation_store_dynamic.h:6504-6533 forward, :6675-6704 transpose):
+ * for every source, every edge of the batch in batch order, (from, to) =
+ * (src, dst) forward or (dst, src) transpose, with degrees of that direction:
+ *   j = max(former_deg[from], seen[from]);
+ *   j == 0 -> r[from] = 0, seen[from] = 1, next edge            (Q8)
+ *   seen[from] = j + 1; p[from] *= (j+1)/j; r[from] -= p[from]/(j+1)/alpha;
+ *   r[to] += (1-alpha) p[from]/(j+1)/alpha; enqueue from / to if r > rmax.
+ * No matrix accumulator: this path builds its blocks from p itself. */
+static void *refresh_edges_job(void *arg)
+{
+    job_t *J = (job_t *)arg;
+    orc_ppr *s = J->s; const orc_graph *g = J->g; int d = J->dir;
+    double a = s->alpha, rmax = s->rmax;
+    int *seen = (int *)calloc((size_t)g->n, sizeof(int));
+    for (int k = J->k0; k < J->k1; k++) {
+        double *p = s->p[d] + IDX(k, 0), *r = s->r[d] + IDX(k, 0);
+        unsigned char *fl = s->flag[d] + IDX(k, 0);
+        for (long long e = 0; e < J->m; e++) {
+            int from = d == 0 ? J->es[e] : J->ed[e];
+            int to = d == 0 ? J->ed[e] : J->es[e];
+            int f0 = g->former[d][from];
+            double j = f0 > seen[from] ? f0 : seen[from];
+            if (j == 0) {
+                r[from] = 0.0;
+                seen[from] = 1;
+                continue;
+            }
+            seen[from] = (int)j + 1;
+            p[from] = ST(p[from] * ((j + 1.0) / j));
+            r[from] = ST(r[from] - p[from] / (j + 1.0) / a);
+            r[to] = ST(r[to] + (1 - a) * p[from] / (j + 1.0) / a);
+            if (OUT(r[from]) && !fl[from]) { q_push(s, d, k, from); fl[from] = 1; }
+            if (OUT(r[to]) && !fl[to]) { q_push(s, d, k, to); fl[to] = 1; }
+        }
+        for (long long e = 0; e < J->m; e++) seen[d == 0 ? J->es[e] : J->ed[e]] = 0;
+    }
+    free(seen);
+    return NULL;
+}
+
+ORC_EXPORT void orc_ppr_refresh_edges(orc_ppr *s, const orc_graph *g, int dir, const int *src, const int *dst,
+                                      long long m, int nthreads)
+{
+    run_parallel_edges(refresh_edges_job, s, g, dir, nthreads, src, dst, m);
+}
+
+/* Storing the state as pair lists keeps only positive entries (ppr:6598-6626,
+ * :6780-6808): reserves and residues <= 0 become 0, so the negative residues
+ * left by the insertion update are dropped (Q4). */
+ORC_EXPORT void orc_ppr_drop_negative(orc_ppr *s, int dir)
+{
+    size_t N = (size_t)s->nsrc * (size_t)s->n;
+    for (size_t t = 0; t < N; t++) {
+        if (s->p[dir][t] < 0.0) s->p[dir][t] = 0.0;
+        if (s->r[dir][t] < 0.0) s->r[dir][t] = 0.0;
+    }
 }
 
 /* nodegree_DenseDynamicForwardPush (ppr_computation_store_dynamic.h:1394-1498),

@@ -27,15 +27,21 @@ def workload(directed, seed):
     return n, snaps, sub
 
 
-@pytest.mark.parametrize("directed,prec,delta", [(False, 1, 0.65), (False, 0, 0.3), (True, 1, 0.65)])
-def test_right_embedding_and_scores_follow_the_reference_bookkeeping(capi, directed, prec, delta):
+@pytest.mark.parametrize("directed,prec,delta,sparse", [(False, 1, 0.65, False), (False, 0, 0.3, False),
+                                                       (True, 1, 0.65, False), (True, 0, 0.3, True)])
+def test_right_embedding_and_scores_follow_the_reference_bookkeeping(capi, directed, prec, delta, sparse):
+    """sparse: the n >= 1e7-node path (mkl_right_matrix_multiplication_without_
+    norm_Robinhood_with_link_prediction, Tree_d_LP.cpp:1959), same bookkeeping."""
     n, snaps, sub = workload(directed, 8)
     k, nparts = 10, 8
     pipe = capi.Pipeline(n, directed, sub, 0.2, 1e-6, nparts, 2, k, 6, precision=prec,
-                         transform=capi.TSVD_TRANSFORM_LP, delta=delta, seed=4, right_embedding=True)
+                         transform=capi.TSVD_TRANSFORM_LP, delta=delta, seed=4, right_embedding=True,
+                         sparse_state=sparse)
     starts, widths = orc.block_map(n, nparts)
     V_ref = np.zeros((n, k))
     U_of = [None] * nparts
+    dout_of = [None] * nparts                     # sparse path: degrees when V_j was computed
+    din_ref = np.zeros(n)
     rng = np.random.Generator(np.random.PCG64(3))
     tol = 1e-10 if prec == 1 else 1e-5
     saw_clean = False
@@ -45,10 +51,15 @@ def test_right_embedding_and_scores_follow_the_reference_bookkeeping(capi, direc
         U = pipe.embedding()                      # LP root: E = U
         if it > 0 and (~dirty).any():
             saw_clean = True
+        dout = pipe.graph.degrees(0)[0].astype(np.float64)
+        din = pipe.graph.degrees(1)[0].astype(np.float64) if directed else None
         for j in np.flatnonzero(dirty):
             A = pipe.block_matrix(j, int(widths[j]))
             V_ref[starts[j]:starts[j] + widths[j]] = A.T @ U
             U_of[j] = U
+            if sparse:
+                dout_of[j] = dout
+                din_ref[starts[j]:starts[j] + widths[j]] = din[starts[j]:starts[j] + widths[j]]
         V = pipe.right_embedding()
         scale = max(np.abs(V_ref).max(), 1e-30)
         assert np.abs(V - V_ref).max() <= tol * scale, f"snapshot {it}: V differs"
@@ -61,6 +72,9 @@ def test_right_embedding_and_scores_follow_the_reference_bookkeeping(capi, direc
         blk = np.minimum(dst // (n // nparts), nparts - 1)
         exp = np.array([U_of[b][row[int(i)]] @ V_ref[v] if U_of[b] is not None else 0.0
                         for i, v, b in zip(src, dst, blk)])
+        if sparse:   # ppr:7707 / 7727: score *= outdegree[i] * indegree[j] of that snapshot
+            exp *= np.array([dout_of[b][int(i)] * din_ref[v] if U_of[b] is not None else 0.0
+                             for i, v, b in zip(src, dst, blk)])
         assert np.abs(sc - exp).max() <= tol * max(np.abs(exp).max(), 1e-30), f"snapshot {it}: scores differ"
     assert saw_clean or delta < 0.5, "the lazy test should keep some block clean"
 

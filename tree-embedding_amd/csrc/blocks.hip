@@ -336,6 +336,7 @@ namespace {
 __global__ void k_lp_scores(const int32_t *__restrict__ slot, int32_t n, int32_t nparts, int32_t common,
                             const double *__restrict__ V, const double *const *__restrict__ ublk, int32_t k,
                             int32_t s_pad, const int32_t *__restrict__ rows, const int32_t *__restrict__ dst, int64_t m,
+                            const double *const *__restrict__ dblk, const double *__restrict__ vdeg,
                             double *__restrict__ scores)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -351,19 +352,21 @@ __global__ void k_lp_scores(const int32_t *__restrict__ slot, int32_t n, int32_t
         const double *Vj = V + start * k + c;
         const int32_t r = rows[e];
         for (int32_t t = 0; t < k; t++) acc += U[(int64_t)t * s_pad + r] * Vj[(int64_t)t * width];
+        if (dblk) acc *= dblk[j][r] * vdeg[v];
     }
     scores[e] = acc;
 }
 
 }  // namespace
 
-void lp_scores(const Blocks &b, const double *V, const double *const *ublk, int32_t k, const int32_t *rows,
+void lp_scores(const Blocks &b, const double *V, const double *const *ublk, const double *const *dblk,
+               const double *vdeg, int32_t k, const int32_t *rows,
                const int32_t *dst, int64_t m, double *scores, hipStream_t st)
 {
     if (m <= 0) return;
     PROF("lp_scores", st);
     hipLaunchKernelGGL(k_lp_scores, dim3((unsigned)ceil_div(m, TPB)), dim3(TPB), 0, st, b.slot.p, b.n, b.nparts,
-                       b.common, V, ublk, k, b.s_pad, rows, dst, m, scores);
+                       b.common, V, ublk, k, b.s_pad, rows, dst, m, dblk, vdeg, scores);
     TSVD_HIP(hipGetLastError());
 }
 
@@ -399,6 +402,171 @@ void eig_select(double *W, const double *V, const double *lam, int l, int k, int
 }  // namespace tsvd
 
 namespace tsvd {
+
+namespace {
+
+// ---- Blocks of the n >= 1e7-node path -------------------------------------
+// Log_sparse_matrix_entries_LP_Robinhood_two_with_transpose_nparts_triplet
+// [_norm_computation] (ppr_computation_store_dynamic.h:6898-7003, 7080-7229)
+// builds block j from the forward and transpose reserve lists: entries of
+// both are inserted and duplicates summed in double (x = p + p^T of the FP32
+// values), then log10(1 + x/rmax).  Rows are inserted in 30 intervals of
+// floor(|S|/30) rows, so rows >= 30 floor(|S|/30) stay empty (Q11).  Here a
+// batch of sources is at hand as dense FP32 state (width sp), rows
+// [row0, row0 + rows) of the matrix.
+__device__ inline double xform_lp(float pf, float pt, double rm)
+{
+    const double x = (double)pf + (double)pt;
+    return x > 0.0 ? log10(1.0 + x / rm) : 0.0;
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_batch_norms(const float *__restrict__ pf, const float *__restrict__ pt,
+                                                     const T *__restrict__ A, const int32_t *__restrict__ slot,
+                                                     int32_t n, int32_t sp, int32_t row0, int32_t rows,
+                                                     int32_t s_pad, int32_t keep, int32_t nparts, int32_t common,
+                                                     double rm, double *__restrict__ norms)
+{
+    const int32_t j = blockIdx.y;
+    const int64_t start = (int64_t)j * common;
+    const int64_t width = (j == nparts - 1) ? n - start : common;
+    const int64_t v0 = start + (int64_t)blockIdx.x * VPB;
+    if (v0 >= start + width) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double a2 = 0.0, e2 = 0.0;
+    for (int vv = wave; vv < VPB; vv += TPB / 64) {
+        const int64_t v = v0 + vv;
+        if (v >= start + width) break;
+        const int32_t sl = slot[v];
+        const T *Ao = sl >= 0 ? A + (start + sl) * s_pad + row0 : nullptr;
+        for (int s = lane; s < rows; s += 64) {
+            const T a = row0 + s < keep ? (T)xform_lp(pf[v * sp + s], pt[v * sp + s], rm) : T(0);
+            const T ao = Ao ? Ao[s] : T(0);
+            a2 += (double)a * (double)a;
+            const double d = (double)a - (double)ao;
+            e2 += d * d;
+        }
+    }
+    a2 = wave_sum(a2);
+    e2 = wave_sum(e2);
+    __shared__ double red[2][TPB / 64];
+    if (lane == 0) {
+        red[0][wave] = a2;
+        red[1][wave] = e2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0, se = 0;
+        for (int w = 0; w < TPB / 64; w++) {
+            sa += red[0][w];
+            se += red[1][w];
+        }
+        atomicAdd(norms + 2 * j, sa);
+        atomicAdd(norms + 2 * j + 1, se);
+    }
+}
+
+// Rows [row0, row0 + width) of the stored columns of each listed block; rows
+// past the batch's sources (padding) and past `keep` are written as zero.
+template <class T>
+__global__ void __launch_bounds__(TPB) k_batch_gather(const int32_t *__restrict__ blocks,
+                                                      const float *__restrict__ pf, const float *__restrict__ pt,
+                                                      T *__restrict__ A, const int32_t *__restrict__ colmap,
+                                                      const int32_t *__restrict__ ncols, int32_t sp, int32_t row0,
+                                                      int32_t rows, int32_t width, int32_t s_pad, int32_t common,
+                                                      int32_t keep, double rm)
+{
+    const int32_t j = blocks[blockIdx.y];
+    const int64_t start = (int64_t)j * common;
+    const int64_t c = (int64_t)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
+    if (c >= ncols[j]) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t v = colmap[start + c];
+    T *col = A + (start + c) * s_pad + row0;
+    for (int s = lane; s < width; s += 64)
+        col[s] = (s < rows && row0 + s < keep) ? (T)xform_lp(pf[v * sp + s], pt[v * sp + s], rm) : T(0);
+}
+
+}  // namespace
+
+void blocks_active(Blocks &b, const tsvd_graph *g, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_active, dim3((unsigned)ceil_div(b.n, TPB)), dim3(TPB), 0, st, g->dir[0].deg.p,
+                       g->ndir > 1 ? g->dir[1].deg.p : nullptr, b.is_source.p, b.n, b.active.p);
+    TSVD_HIP(hipMemsetAsync(b.norms.p, 0, b.norms.bytes(), st));
+}
+
+void blocks_norms_batch(Blocks &b, const float *pf, const float *pt, int32_t sp, int32_t row0, int32_t rows,
+                        int32_t keep, hipStream_t st)
+{
+    int64_t maxw = 0;
+    for (auto w : b.width) maxw = std::max(maxw, w);
+    dim3 grid((unsigned)ceil_div(maxw, VPB), (unsigned)b.nparts);
+    if (g_profile) {
+        double stored = 0.0;
+        for (int32_t j = 0; j < b.nparts; j++) stored += (double)b.ncols[j];
+        prof_add_work("blocks_norms", (2.0 * 4.0 * b.n + stored * b.esize) * rows, 0.0);
+    }
+    PROF("blocks_norms", st);
+    if (b.precision == TSVD_FP64)
+        hipLaunchKernelGGL(k_batch_norms<double>, grid, dim3(TPB), 0, st, pf, pt, (const double *)b.A.p, b.slot.p,
+                           b.n, sp, row0, rows, b.s_pad, keep, b.nparts, b.common, b.rmax, b.norms.p);
+    else
+        hipLaunchKernelGGL(k_batch_norms<float>, grid, dim3(TPB), 0, st, pf, pt, (const float *)b.A.p, b.slot.p,
+                           b.n, sp, row0, rows, b.s_pad, keep, b.nparts, b.common, b.rmax, b.norms.p);
+    TSVD_HIP(hipGetLastError());
+}
+
+void blocks_norms_read(Blocks &b, hipStream_t st)
+{
+    std::vector<double> h((size_t)2 * b.nparts);
+    TSVD_HIP(hipMemcpyAsync(h.data(), b.norms.p, b.norms.bytes(), hipMemcpyDeviceToHost, st));
+    TSVD_HIP(hipStreamSynchronize(st));
+    for (int32_t j = 0; j < b.nparts; j++) {
+        b.a2[j] = h[2 * j];
+        b.e2[j] = h[2 * j + 1];
+    }
+}
+
+void blocks_slots(Blocks &b, const std::vector<int32_t> &dirty, hipStream_t st)
+{
+    if (dirty.empty()) return;
+    const int nd = (int)dirty.size();
+    TSVD_HIP(hipMemcpyAsync(b.d_dirty.p, dirty.data(), sizeof(int32_t) * nd, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_slots, dim3(nd), dim3(TPB), 0, st, b.d_dirty.p, b.active.p, b.n, b.nparts, b.common,
+                       b.slot.p, b.colmap.p, b.d_ncols.p);
+    TSVD_HIP(hipGetLastError());
+}
+
+void blocks_gather_batch(Blocks &b, const std::vector<int32_t> &dirty, const float *pf, const float *pt, int32_t sp,
+                         int32_t row0, int32_t rows, int32_t keep, hipStream_t st)
+{
+    if (dirty.empty()) return;
+    const int nd = (int)dirty.size();
+    int64_t maxw = 0;
+    for (int32_t j : dirty) maxw = std::max(maxw, b.width[j]);
+    const int32_t width = std::min(sp, b.s_pad - row0);
+    dim3 grid((unsigned)ceil_div(maxw, TPB / 64), (unsigned)nd);
+    if (g_profile) {
+        double w = 0.0;
+        for (int32_t j : dirty) w += (double)b.width[j];
+        prof_add_work("blocks_gather", w * width * (8.0 + (double)b.esize), 0.0);
+    }
+    PROF("blocks_gather", st);
+    if (b.precision == TSVD_FP64)
+        hipLaunchKernelGGL(k_batch_gather<double>, grid, dim3(TPB), 0, st, b.d_dirty.p, pf, pt, (double *)b.A.p,
+                           b.colmap.p, b.d_ncols.p, sp, row0, rows, width, b.s_pad, b.common, keep, b.rmax);
+    else
+        hipLaunchKernelGGL(k_batch_gather<float>, grid, dim3(TPB), 0, st, b.d_dirty.p, pf, pt, (float *)b.A.p,
+                           b.colmap.p, b.d_ncols.p, sp, row0, rows, width, b.s_pad, b.common, keep, b.rmax);
+    TSVD_HIP(hipGetLastError());
+}
+
+void blocks_ncols_read(Blocks &b, hipStream_t st)
+{
+    TSVD_HIP(hipMemcpyAsync(b.ncols.data(), b.d_ncols.p, sizeof(int32_t) * b.nparts, hipMemcpyDeviceToHost, st));
+    TSVD_HIP(hipStreamSynchronize(st));
+}
 
 namespace {
 

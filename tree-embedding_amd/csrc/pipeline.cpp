@@ -14,6 +14,7 @@
 // sqrt(||A||_F^2 - sum_k sigma^2); both terms are already at hand.
 #include "tsvd_pipeline.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 
@@ -37,6 +38,35 @@ static void validate(const tsvd_config &c)
     TSVD_REQUIRE(c.fanout * c.rank <= 4096, "fan-out * rank too large for the merge eigensolve");
     TSVD_REQUIRE(!c.right_embedding || c.transform == TSVD_TRANSFORM_LP,
                  "the right embedding is defined for the LP transform (Tree_u_LP.cpp:291-404)");
+    TSVD_REQUIRE(!c.sparse_state || (c.directed && c.transform == TSVD_TRANSFORM_LP),
+                 "sparse_state is the directed LP driver's path (Tree_d_LP.cpp:1377-2024)");
+    TSVD_REQUIRE(!c.sparse_state || !c.two_sided, "sparse_state keeps the reference's positive-side threshold");
+    TSVD_REQUIRE(c.sparse_groups >= 0, "sparse_groups must be >= 0");
+}
+
+// Batches of the sparse state: as many groups of 64 sources as half of the
+// free HBM holds (four FP32 arrays of n x 64 per group plus the push's item
+// lists), the rest left for the pair lists.
+static void sparse_setup(tsvd_pipeline *p, hipStream_t st)
+{
+    const tsvd_config &c = p->cfg;
+    SparseState &ss = p->sp;
+    const int32_t ngroups = (int32_t)ceil_div(c.nsrc, 64);
+    int32_t gpb = c.sparse_groups;
+    if (gpb <= 0) {
+        size_t fr = 0, tot = 0;
+        TSVD_HIP(hipMemGetInfo(&fr, &tot));
+        const double per = (double)c.n * (4.0 * 64 * 4 + 36.0);
+        gpb = (int32_t)std::max(1.0, std::floor(0.5 * (double)fr / per));
+    }
+    ss.gpb = std::min(gpb, ngroups);
+    ss.nbatch = (int32_t)ceil_div(ngroups, ss.gpb);
+    ss.keep = 30 * (c.nsrc / 30);
+    ss.lists.resize((size_t)ss.nbatch);
+    const int32_t first = ss.batch_sources(0, c.nsrc);
+    int rc = tsvd_ppr_create(p->g, p->sources.data(), first, c.alpha, c.rmax, TSVD_FP32, &ss.gp);
+    if (rc) throw Error{rc};
+    (void)st;
 }
 
 // V_j = A_j^T U for the listed blocks with the current root U (LP: E = U),
@@ -65,16 +95,134 @@ static void right_embedding_update(tsvd_pipeline *p, const std::vector<int32_t> 
     }
     const DT T = b.precision == TSVD_FP64 ? DT::F64 : DT::F32;
     p->gemm.run(st, gp, GemmTypes{T, DT::F64, DT::F64, DT::F64}, k > 80, "gemm_right");
-    std::vector<const double *> ptr(b.nparts, nullptr);
+    const bool degs = p->cfg.sparse_state != 0;
+    if (degs) {
+        // ppr:7707 / 7727: score *= outdegree[i] * indegree[j] of this snapshot
+        std::vector<int32_t> dout((size_t)b.n), din((size_t)b.n);
+        TSVD_HIP(hipMemcpyAsync(dout.data(), p->g->dir[0].deg.p, sizeof(int32_t) * b.n, hipMemcpyDeviceToHost, st));
+        TSVD_HIP(hipMemcpyAsync(din.data(), p->g->dir[1].deg.p, sizeof(int32_t) * b.n, hipMemcpyDeviceToHost, st));
+        TSVD_HIP(hipStreamSynchronize(st));
+        std::vector<double> hs((size_t)b.s_pad, 0.0);
+        for (int32_t i = 0; i < b.nsrc; i++) hs[(size_t)i] = (double)dout[(size_t)p->sources[(size_t)i]];
+        DBuf<double> &ds = re.dsrc[it];
+        ds.alloc((size_t)b.s_pad);
+        TSVD_HIP(hipMemcpyAsync(ds.p, hs.data(), ds.bytes(), hipMemcpyHostToDevice, st));
+        for (int32_t j : blocks) {
+            std::vector<double> hv((size_t)b.width[j]);
+            for (int64_t c = 0; c < b.width[j]; c++) hv[(size_t)c] = (double)din[(size_t)(b.start[j] + c)];
+            TSVD_HIP(hipMemcpyAsync(re.vdeg.p + b.start[j], hv.data(), sizeof(double) * hv.size(),
+                                    hipMemcpyHostToDevice, st));
+            TSVD_HIP(hipStreamSynchronize(st));
+        }
+    }
+    std::vector<const double *> ptr(b.nparts, nullptr), dptr(b.nparts, nullptr);
     for (int32_t j = 0; j < b.nparts; j++)
-        if (re.ver[j] >= 0) ptr[j] = re.U.at(re.ver[j]).p;
+        if (re.ver[j] >= 0) {
+            ptr[j] = re.U.at(re.ver[j]).p;
+            if (degs) dptr[j] = re.dsrc.at(re.ver[j]).p;
+        }
     TSVD_HIP(hipMemcpyAsync(re.ublk.p, ptr.data(), sizeof(const double *) * b.nparts, hipMemcpyHostToDevice, st));
+    if (degs)
+        TSVD_HIP(hipMemcpyAsync(re.dblk.p, dptr.data(), sizeof(const double *) * b.nparts, hipMemcpyHostToDevice, st));
     TSVD_HIP(hipStreamSynchronize(st));
     for (auto m = re.U.begin(); m != re.U.end();) {
         bool live = false;
         for (int32_t j = 0; j < b.nparts && !live; j++) live = re.ver[j] == m->first;
+        if (!live) re.dsrc.erase(m->first);
         m = live ? std::next(m) : re.U.erase(m);
     }
+}
+
+// Lazy test (Eqn 2, ppr:2699-2726 / 7204-7226): clean iff resid +
+// ||A_new - A_old|| < delta sqrt2 ||A_new||, from lazy_start on
+// (Tree_u_NC.cpp:608), or from dyn_start on in the n >= 1e7-node path
+// (Tree_d_LP.cpp:1658).  `rebuild` gets the dirty blocks, `factor` those of
+// them with a nonzero matrix (an all-zero block is skipped, Tree_u_NC.cpp:
+// 683-686, Tree_d_LP.cpp:1786-1790).
+static void decide_dirty(const tsvd_config &c, const Blocks &b, int32_t it, std::vector<int32_t> &rebuild,
+                         std::vector<int32_t> &factor)
+{
+    const double thr = c.delta * std::sqrt(2.0);
+    const int32_t from = c.sparse_state ? c.dyn_start : c.lazy_start;
+    for (int32_t j = 0; j < b.nparts; j++) {
+        bool dirty = true;
+        if (it >= from) dirty = !(b.resid[j] + std::sqrt(b.e2[j]) < thr * std::sqrt(b.a2[j]));
+        if (!dirty) continue;
+        rebuild.push_back(j);
+        if (b.a2[j] > 0.0) factor.push_back(j);
+    }
+}
+
+// Refresh, push and block test of the n >= 1e7-node path (Tree_d_LP.cpp:
+// 1561-1760, All_initialization_and_push_final at ppr:6188-6832), one batch
+// of source groups at a time: load its pair lists into the dense state (or
+// start it afresh before dyn_start), update in batch order, push both
+// directions, store the positive entries back, and add the batch's rows to
+// every block's norms.  Dirty blocks are then rebuilt from the stored
+// reserve lists (the matrix rows of a batch need both directions' p).
+static void sparse_step(tsvd_pipeline *p, int32_t it, const int32_t *dsrc, const int32_t *ddst, int64_t m,
+                        tsvd_snapshot_stats &S, std::vector<int32_t> &rebuild, std::vector<int32_t> &factor)
+{
+    const tsvd_config &c = p->cfg;
+    SparseState &ss = p->sp;
+    tsvd_ppr *gp = ss.gp;
+    Blocks &b = p->blocks;
+    hipStream_t st = p->g->stream;
+    const bool fresh = it < c.dyn_start;
+    const bool keep_r = it >= c.dyn_start - 1;   // ppr:6298-6300: residues are stored from dyn_start-1 on
+    blocks_active(b, p->g, st);
+    for (int32_t bt = 0; bt < ss.nbatch; bt++) {
+        const int32_t row0 = bt * ss.gpb * 64, cnt = ss.batch_sources(bt, c.nsrc);
+        auto &L = ss.lists[(size_t)bt];
+        ppr_bind_sources(gp, p->sources.data() + row0, cnt);
+        for (int d = 0; d < 2; d++) {
+            const double t0 = now_ms();
+            if (fresh) {
+                ppr_refresh(gp, TSVD_REFRESH_INIT, d, 0, cnt);
+            } else {
+                ppr_clear(gp, d);
+                list_load(L[(size_t)(2 * d)], gp->p(d), gp->s_pad, st);
+                list_load(L[(size_t)(2 * d + 1)], gp->r(d), gp->s_pad, st);
+                ppr_update_edges(gp, d, dsrc, ddst, m, 0, cnt);
+            }
+            const double t1 = now_ms();
+            ppr_push(gp, d, 0, cnt);
+            S.ms_refresh += t1 - t0;
+            S.ms_push += now_ms() - t1;
+            S.push_rounds += gp->last_rounds[d];
+            S.push_pops += gp->last_pops[d];
+            S.push_relax += gp->last_relax[d];
+            S.push_items += gp->last_items[d];
+            S.push_wave_relax += gp->last_wave_relax[d];
+        }
+        const double t2 = now_ms();
+        for (int d = 0; d < 2; d++) {
+            list_store(ss.work, L[(size_t)(2 * d)], gp->p(d), 4, c.n, gp->s_pad, cnt, st);
+            if (keep_r) list_store(ss.work, L[(size_t)(2 * d + 1)], gp->r(d), 4, c.n, gp->s_pad, cnt, st);
+            else list_clear(L[(size_t)(2 * d + 1)]);
+        }
+        blocks_norms_batch(b, (const float *)gp->p(0), (const float *)gp->p(1), gp->s_pad, row0, cnt, ss.keep, st);
+        S.ms_blocks += now_ms() - t2;
+    }
+    const double t3 = now_ms();
+    blocks_norms_read(b, st);
+    decide_dirty(c, b, it, rebuild, factor);
+    if (!rebuild.empty()) {
+        blocks_slots(b, rebuild, st);
+        for (int32_t bt = 0; bt < ss.nbatch; bt++) {
+            const int32_t row0 = bt * ss.gpb * 64, cnt = ss.batch_sources(bt, c.nsrc);
+            if (ss.nbatch > 1) {   // with one batch the state still holds it
+                for (int d = 0; d < 2; d++) {
+                    ppr_clear(gp, d);
+                    list_load(ss.lists[(size_t)bt][(size_t)(2 * d)], gp->p(d), gp->s_pad, st);
+                }
+            }
+            blocks_gather_batch(b, rebuild, (const float *)gp->p(0), (const float *)gp->p(1), gp->s_pad, row0, cnt,
+                                ss.keep, st);
+        }
+        blocks_ncols_read(b, st);
+    }
+    S.ms_blocks += now_ms() - t3;
 }
 
 }  // namespace tsvd
@@ -85,6 +233,7 @@ tsvd_pipeline::~tsvd_pipeline()
 {
     delete eig;
     if (ppr) tsvd_ppr_destroy(ppr);
+    if (sp.gp) tsvd_ppr_destroy(sp.gp);
     if (g) tsvd_graph_destroy(g);
 }
 
@@ -100,14 +249,25 @@ extern "C" int tsvd_pipeline_create(const tsvd_config *cfg, tsvd_pipeline **out)
             p->cfg.sources = p->sources.data();
             int rc = tsvd_graph_create(cfg->n, cfg->directed, cfg->slack > 0 ? cfg->slack : 0.1, &p->g);
             if (rc) throw Error{rc};
-            rc = tsvd_ppr_create(p->g, cfg->sources, cfg->nsrc, cfg->alpha, cfg->rmax, cfg->precision, &p->ppr);
-            if (rc) throw Error{rc};
-            p->ppr->two_sided = cfg->two_sided ? 1 : 0;
+            const int32_t s_pad = (int32_t)round_up(cfg->nsrc, 64);
+            if (!cfg->sparse_state) {
+                rc = tsvd_ppr_create(p->g, cfg->sources, cfg->nsrc, cfg->alpha, cfg->rmax, cfg->precision, &p->ppr);
+                if (rc) throw Error{rc};
+                p->ppr->two_sided = cfg->two_sided ? 1 : 0;
+            } else {
+                std::vector<int32_t> sorted(p->sources);
+                std::sort(sorted.begin(), sorted.end());
+                for (size_t i = 0; i < sorted.size(); i++) {
+                    TSVD_REQUIRE(sorted[i] >= 0 && sorted[i] < cfg->n, "source %d out of range", sorted[i]);
+                    TSVD_REQUIRE(i == 0 || sorted[i] != sorted[i - 1], "duplicate source %d", sorted[i]);
+                }
+            }
             hipStream_t st = p->g->stream;
-            blocks_setup(p->blocks, cfg->n, cfg->nparts, cfg->nsrc, p->ppr->s_pad, cfg->precision, cfg->transform,
+            blocks_setup(p->blocks, cfg->n, cfg->nparts, cfg->nsrc, s_pad, cfg->precision, cfg->transform,
                          cfg->rmax, p->sources, st);
             frpca_setup(p->fw, p->blocks, cfg->rank, cfg->q);
-            tree_setup(p->tree, cfg->nparts, cfg->fanout, cfg->rank, cfg->nsrc, p->ppr->s_pad, p->blocks.esize, st);
+            tree_setup(p->tree, cfg->nparts, cfg->fanout, cfg->rank, cfg->nsrc, s_pad, p->blocks.esize, st);
+            if (cfg->sparse_state) sparse_setup(p, st);
             p->eig = new Eig();
             for (int32_t i = 0; i < cfg->nsrc; i++) p->re.row_of[cfg->sources[i]] = i;
             if (cfg->right_embedding) {
@@ -116,6 +276,12 @@ extern "C" int tsvd_pipeline_create(const tsvd_config *cfg, tsvd_pipeline **out)
                 p->re.ver.assign(cfg->nparts, -1);
                 p->re.ublk.alloc((size_t)cfg->nparts);
                 TSVD_HIP(hipMemsetAsync(p->re.ublk.p, 0, p->re.ublk.bytes(), st));
+                if (cfg->sparse_state) {
+                    p->re.dblk.alloc((size_t)cfg->nparts);
+                    p->re.vdeg.alloc((size_t)cfg->n);
+                    TSVD_HIP(hipMemsetAsync(p->re.dblk.p, 0, p->re.dblk.bytes(), st));
+                    TSVD_HIP(hipMemsetAsync(p->re.vdeg.p, 0, p->re.vdeg.bytes(), st));
+                }
                 TSVD_HIP(hipStreamSynchronize(st));
             }
         } catch (...) {
@@ -146,46 +312,42 @@ extern "C" int tsvd_pipeline_snapshot(tsvd_pipeline *p, const int32_t *src, cons
         const double t1 = now_ms();
         S.ms_ingest = t1 - t0;
         // Tree_u_NC.cpp:516-518: snapshots before lazy_start-1 are only ingested
-        if (c.lazy_start != 100 && it < c.lazy_start - 1) {
+        // (lazy_start 100 turns that off; 200 in the n >= 1e7-node path, Tree_d_LP.cpp:1574)
+        if (c.lazy_start != (c.sparse_state ? 200 : 100) && it < c.lazy_start - 1) {
             S.reused = 1;
             S.ms_total = now_ms() - t0;
             if (stats) *stats = S;
             return;
         }
-        const int mode = it < c.dyn_start ? TSVD_REFRESH_INIT : TSVD_REFRESH_DYNAMIC;
-        for (int d = 0; d < p->ppr->ndir; d++) ppr_refresh(p->ppr, mode, d, 0, c.nsrc);
-        const double t2 = now_ms();
-        S.ms_refresh = t2 - t1;
-        for (int d = 0; d < p->ppr->ndir; d++) {
-            ppr_push(p->ppr, d, 0, c.nsrc);
-            S.push_rounds += p->ppr->last_rounds[d];
-            S.push_pops += p->ppr->last_pops[d];
-            S.push_relax += p->ppr->last_relax[d];
-            S.push_items += p->ppr->last_items[d];
-            S.push_wave_relax += p->ppr->last_wave_relax[d];
-        }
-        const double t3 = now_ms();
-        S.ms_push = t3 - t2;
-
         Blocks &b = p->blocks;
-        const void *p0 = p->ppr->p(0);
-        const void *p1 = p->ppr->ndir > 1 ? p->ppr->p(1) : nullptr;
-        blocks_norms(b, p0, p1, p->g, st);
         std::vector<int32_t> rebuild, factor;
-        const double thr = c.delta * std::sqrt(2.0);
-        for (int32_t j = 0; j < b.nparts; j++) {
-            bool dirty = true;
-            if (it >= c.lazy_start) {
-                // Eqn 2 (ppr:2699-2726): clean iff resid + ||A_new - A_old|| < delta sqrt2 ||A_new||
-                dirty = !(b.resid[j] + std::sqrt(b.e2[j]) < thr * std::sqrt(b.a2[j]));
+        if (c.sparse_state) {
+            const int32_t *dsrc = on_device ? src : p->g->e_src.p;
+            const int32_t *ddst = on_device ? dst : p->g->e_dst.p;
+            sparse_step(p, it, dsrc, ddst, m, S, rebuild, factor);
+        } else {
+            const int mode = it < c.dyn_start ? TSVD_REFRESH_INIT : TSVD_REFRESH_DYNAMIC;
+            for (int d = 0; d < p->ppr->ndir; d++) ppr_refresh(p->ppr, mode, d, 0, c.nsrc);
+            const double t2 = now_ms();
+            S.ms_refresh = t2 - t1;
+            for (int d = 0; d < p->ppr->ndir; d++) {
+                ppr_push(p->ppr, d, 0, c.nsrc);
+                S.push_rounds += p->ppr->last_rounds[d];
+                S.push_pops += p->ppr->last_pops[d];
+                S.push_relax += p->ppr->last_relax[d];
+                S.push_items += p->ppr->last_items[d];
+                S.push_wave_relax += p->ppr->last_wave_relax[d];
             }
-            if (!dirty) continue;
-            rebuild.push_back(j);
-            if (b.a2[j] > 0.0) factor.push_back(j);   // an all-zero block is skipped (Tree_u_NC.cpp:683-686)
+            const double t3 = now_ms();
+            S.ms_push = t3 - t2;
+            const void *p0 = p->ppr->p(0);
+            const void *p1 = p->ppr->ndir > 1 ? p->ppr->p(1) : nullptr;
+            blocks_norms(b, p0, p1, p->g, st);
+            decide_dirty(c, b, it, rebuild, factor);
+            blocks_rebuild(b, rebuild, p0, p1, st);
+            S.ms_blocks = now_ms() - t3;
         }
-        blocks_rebuild(b, rebuild, p0, p1, st);
         const double t4 = now_ms();
-        S.ms_blocks = t4 - t3;
 
         p->last_dirty = factor;
         S.ndirty = (int32_t)factor.size();
@@ -235,7 +397,37 @@ extern "C" int tsvd_pipeline_root_sigma(tsvd_pipeline *p, double *sigma)
 }
 
 extern "C" tsvd_graph *tsvd_pipeline_graph(tsvd_pipeline *p) { return p ? p->g : nullptr; }
-extern "C" tsvd_ppr *tsvd_pipeline_ppr(tsvd_pipeline *p) { return p ? p->ppr : nullptr; }
+extern "C" tsvd_ppr *tsvd_pipeline_ppr(tsvd_pipeline *p) { return p ? (p->ppr ? p->ppr : p->sp.gp) : nullptr; }
+
+extern "C" int tsvd_pipeline_source_state(tsvd_pipeline *p, int32_t dir, int32_t i, double *reserve,
+                                          double *residue)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p, "null pipeline");
+        const tsvd_config &c = p->cfg;
+        TSVD_REQUIRE(dir >= 0 && dir < (c.directed ? 2 : 1), "bad direction");
+        TSVD_REQUIRE(i >= 0 && i < c.nsrc, "source index out of range");
+        if (c.sparse_state) {
+            const SparseState &ss = p->sp;
+            const int32_t bt = i / (ss.gpb * 64), s = i - bt * ss.gpb * 64;
+            const auto &L = ss.lists[(size_t)bt];
+            if (reserve) list_read(L[(size_t)(2 * dir)], s, c.n, reserve);
+            if (residue) list_read(L[(size_t)(2 * dir + 1)], s, c.n, residue);
+            return;
+        }
+        tsvd_ppr *q = p->ppr;
+        double *out[2] = {reserve, residue};
+        std::vector<uint8_t> h((size_t)c.n * q->esize);
+        for (int k = 0; k < 2; k++) {
+            if (!out[k]) continue;
+            const uint8_t *base = (const uint8_t *)(k == 0 ? q->p(dir) : q->r(dir)) + (size_t)i * q->esize;
+            TSVD_HIP(hipMemcpy2D(h.data(), q->esize, base, (size_t)q->s_pad * q->esize, q->esize, (size_t)c.n,
+                                 hipMemcpyDeviceToHost));
+            for (int32_t v = 0; v < c.n; v++)
+                out[k][v] = q->esize == 8 ? ((double *)h.data())[v] : (double)((float *)h.data())[v];
+        }
+    });
+}
 
 extern "C" int tsvd_pipeline_block_matrix(tsvd_pipeline *p, int32_t j, double *A)
 {
@@ -317,6 +509,7 @@ extern "C" int tsvd_pipeline_dense_gram(tsvd_pipeline *p, double *G)
 {
     return guard([&] {
         TSVD_REQUIRE(p && G, "null argument");
+        TSVD_REQUIRE(p->ppr, "the dense Gram reads the device state (not kept with sparse_state)");
         const Blocks &b = p->blocks;
         hipStream_t st = p->g->stream;
         const int64_t chunk = 65536;
@@ -347,6 +540,7 @@ extern "C" int tsvd_pipeline_gram_apply(tsvd_pipeline *p, const double *Y, int32
 {
     return guard([&] {
         TSVD_REQUIRE(p && Y && Z, "null argument");
+        TSVD_REQUIRE(p->ppr, "gram_apply reads the device state (not kept with sparse_state)");
         const Blocks &b = p->blocks;
         TSVD_REQUIRE(l >= 1 && l <= b.nsrc, "l must be in [1, nsrc]");
         hipStream_t st = p->g->stream;
@@ -429,7 +623,8 @@ extern "C" int tsvd_pipeline_lp_scores(tsvd_pipeline *p, const int32_t *src, con
         DBuf<double> d_sc((size_t)m);
         TSVD_HIP(hipMemcpyAsync(d_rows.p, rows.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
         TSVD_HIP(hipMemcpyAsync(d_dst.p, dst, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
-        lp_scores(p->blocks, p->re.V.p, p->re.ublk.p, p->cfg.rank, d_rows.p, d_dst.p, m, d_sc.p, st);
+        lp_scores(p->blocks, p->re.V.p, p->re.ublk.p, p->cfg.sparse_state ? p->re.dblk.p : nullptr, p->re.vdeg.p,
+                  p->cfg.rank, d_rows.p, d_dst.p, m, d_sc.p, st);
         TSVD_HIP(hipMemcpyAsync(scores, d_sc.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
         TSVD_HIP(hipStreamSynchronize(st));
     });

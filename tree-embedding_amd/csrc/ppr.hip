@@ -233,6 +233,110 @@ __global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict
     stage_flush(sg);
 }
 
+// ---- Insertion update of the n >= 1e7-node path, in batch order ----------
+// All_initialization_and_push_final (ppr:6504-6533 forward, :6675-6704
+// transpose) walks the batch in order for each source: j = max(former
+// degree of `from`, edges of `from` seen so far); j == 0 sets r[from] = 0 and
+// skips the edge (Q8); otherwise p *= (j+1)/j, r[from] -= p/(j+1)/alpha,
+// r[to] += (1-alpha) p/(j+1)/alpha, in double.  For a row of former degree
+// d > 0 the ratios telescope: each of its m new edges moves p0/(d alpha),
+// with p0 the reserve before the batch, and p ends at p0 (d+m)/d.  A row of
+// former degree 0 has never been pushed along, so p0 = 0 and its edges move
+// nothing; what Q8 drops is the residue that row holds when its first edge
+// comes, which includes what earlier edges of the batch added to it.  The
+// passes below reproduce that order without walking it:
+//   rows   (phase 0): r[i] = 0 for every grown row i of former degree 0;
+//   adds            : every edge adds (1-alpha) p0[from]/(d alpha) to r[to],
+//                     except an edge that precedes the first edge of a
+//                     former-degree-0 `to` (its add was dropped with it);
+//   rows   (phase 1): rows of former degree d > 0: r -= m p0/(d alpha),
+//                     p = p0 (d+m)/d;
+//   frontier        : both ends of every edge with r > rmax and an edge to
+//                     push along join the push frontier (the reference queues
+//                     them and tests r > rmax again when it pops them).
+constexpr int32_t NO_EDGE = 0x7F7F7F7F;   // first[] of a row with no new edge
+
+__global__ void k_first_edge(const int32_t *__restrict__ efrom, int64_t m, const int32_t *__restrict__ former,
+                             int32_t *__restrict__ first, int reset)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    const int32_t i = efrom[e];
+    if (reset) first[i] = NO_EDGE;
+    else if (former[i] == 0) atomicMin(first + i, (int32_t)e);
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_edge_rows(T *__restrict__ P, T *__restrict__ R,
+                                                   const int32_t *__restrict__ touched, int32_t ntouched,
+                                                   const int32_t *__restrict__ deg, const int32_t *__restrict__ former,
+                                                   int32_t s_pad, int32_t gcount, int32_t s0, int32_t s1,
+                                                   double alpha, int phase)
+{
+    const int64_t wave = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (wave >= (int64_t)ntouched * gcount) return;
+    const int32_t t = (int32_t)(wave / gcount);
+    const int32_t s = (int32_t)(wave - (int64_t)t * gcount) * 64 + (threadIdx.x & 63);
+    if (s < s0 || s >= s1) return;
+    const int32_t i = touched[t];
+    const int32_t d = former[i];
+    const int64_t at = (int64_t)i * s_pad + s;
+    if (phase == 0) {
+        if (d == 0) R[at] = T(0);
+        return;
+    }
+    if (d == 0) return;
+    const int32_t m = deg[i] - d;
+    const double p0 = (double)P[at];
+    const double c = p0 / ((double)d * alpha);
+    R[at] = (T)((double)R[at] - (double)m * c);
+    P[at] = (T)(p0 * ((double)(d + m) / (double)d));
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_edge_adds(const T *__restrict__ P, T *__restrict__ R,
+                                                   const int32_t *__restrict__ efrom, const int32_t *__restrict__ eto,
+                                                   int64_t m, const int32_t *__restrict__ former,
+                                                   const int32_t *__restrict__ first, int32_t s_pad, int32_t gcount,
+                                                   int32_t s0, int32_t s1, double alpha)
+{
+    const int64_t wave = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (wave >= m * gcount) return;
+    const int64_t e = wave / gcount;
+    const int32_t s = (int32_t)(wave - e * gcount) * 64 + (threadIdx.x & 63);
+    if (s < s0 || s >= s1) return;
+    const int32_t i = efrom[e], to = eto[e];
+    const int32_t d = former[i];
+    if (d == 0) return;
+    if (former[to] == 0 && first[to] != NO_EDGE && e < (int64_t)first[to]) return;
+    const double c = (double)P[(int64_t)i * s_pad + s] / ((double)d * alpha);
+    atomicAdd(R + (int64_t)to * s_pad + s, (T)((1.0 - alpha) * c));
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_edge_frontier(const T *__restrict__ R, const int32_t *__restrict__ efrom,
+                                                       const int32_t *__restrict__ eto, int64_t m,
+                                                       const int32_t *__restrict__ deg, int32_t s_pad,
+                                                       int32_t ngroups, int32_t gcount, int32_t s0, int32_t s1, T rm,
+                                                       unsigned long long *__restrict__ cm, uint32_t *__restrict__ list,
+                                                       uint32_t *__restrict__ count, uint32_t cap,
+                                                       uint32_t *__restrict__ overflow)
+{
+    TSVD_STAGE(sg, list, count, cap, overflow);
+    const int64_t wave = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (wave < 2 * m * gcount) {
+        const int64_t e2 = wave / gcount;
+        const int32_t g = (int32_t)(wave - e2 * gcount);
+        const int64_t e = e2 >> 1;
+        const int32_t v = (e2 & 1) ? eto[e] : efrom[e];
+        const int32_t s = g * 64 + (threadIdx.x & 63);
+        const bool out = s >= s0 && s < s1 && deg[v] > 0 && R[(int64_t)v * s_pad + s] > rm;
+        const unsigned long long c = __ballot(out);
+        if (c && (threadIdx.x & 63) == 0) stage_mark(sg, c, (uint32_t)v * (uint32_t)ngroups + (uint32_t)g, cm);
+    }
+    stage_flush(sg);
+}
+
 // Relax neighbours nb[j0, j1) of a popped (v, g) item: one coalesced 64-lane
 // atomic add per neighbour; a lane whose residue crosses rmax upward is
 // flagged, and the wave's ballot is OR-ed into the next round's mask of (u, g).
@@ -717,6 +821,12 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
         const bool pull = !s->two_sided && s->ngroups <= 64 && s->pull_frac > 0.0 &&
                           (double)count > s->pull_frac * (double)s->n;
         if (pull) {
+            // per-vertex pushed amounts and group masks of pull rounds, made on first use
+            if (s->pm.n == 0) {
+                s->delta.alloc((size_t)s->n * s->s_pad * s->esize);
+                s->pm.alloc((size_t)s->n);
+                TSVD_HIP(hipMemsetAsync(s->pm.p, 0, s->pm.bytes(), st));
+            }
             build_worklist(s, d);
             const GraphDir &Q = pull_dir(s, d);
             {
@@ -820,6 +930,80 @@ void ppr_refresh(tsvd_ppr *s, int mode, int dir, int32_t s0, int32_t s1)
     else refresh_t<float>(s, mode, dir, s0, s1);
 }
 
+template <class T>
+static void update_edges_t(tsvd_ppr *s, int d, const int32_t *src, const int32_t *dst, int64_t m, int32_t s0,
+                           int32_t s1)
+{
+    tsvd_graph *g = s->g;
+    hipStream_t st = g->stream;
+    const GraphDir &D = g->dir[d];
+    T *P = (T *)s->p(d), *R = (T *)s->r(d);
+    const int32_t *efrom = d == 0 ? src : dst, *eto = d == 0 ? dst : src;
+    const int32_t gcount = (int32_t)ceil_div(s1, 64);
+    uint32_t *cnt = s->counters.p;
+    const uint32_t cap = (uint32_t)s->F[d].n;
+    if (s->first[d].n == 0) {
+        s->first[d].alloc((size_t)s->n);
+        TSVD_HIP(hipMemsetAsync(s->first[d].p, 0x7F, s->first[d].bytes(), st));   // NO_EDGE in every byte
+    }
+    int32_t *first = s->first[d].p;
+    if (g_profile)   // per (edge, source): p[from] read, r[to] RMW; per (grown row, source): p and r RMW
+        prof_add_work("ppr_update", (double)sizeof(T) * (s1 - s0) * (3.0 * m + 4.0 * D.ntouched), 0.0);
+    PROF("ppr_update", st);
+    if (m > 0)
+        hipLaunchKernelGGL(k_first_edge, dim3((unsigned)ceil_div(m, TPB)), dim3(TPB), 0, st, efrom, m, D.former.p,
+                           first, 0);
+    const int64_t rw = (int64_t)D.ntouched * gcount;
+    if (rw > 0)
+        hipLaunchKernelGGL(k_edge_rows<T>, dim3((unsigned)ceil_div(rw, WPB)), dim3(TPB), 0, st, P, R, D.touched.p,
+                           D.ntouched, D.deg.p, D.former.p, s->s_pad, gcount, s0, s1, s->alpha, 0);
+    if (m > 0)
+        hipLaunchKernelGGL(k_edge_adds<T>, dim3((unsigned)ceil_div(m * gcount, WPB)), dim3(TPB), 0, st, P, R, efrom,
+                           eto, m, D.former.p, first, s->s_pad, gcount, s0, s1, s->alpha);
+    if (rw > 0)
+        hipLaunchKernelGGL(k_edge_rows<T>, dim3((unsigned)ceil_div(rw, WPB)), dim3(TPB), 0, st, P, R, D.touched.p,
+                           D.ntouched, D.deg.p, D.former.p, s->s_pad, gcount, s0, s1, s->alpha, 1);
+    if (m > 0) {
+        hipLaunchKernelGGL(k_edge_frontier<T>, dim3((unsigned)ceil_div(2 * m * gcount, WPB)), dim3(TPB), 0, st, R,
+                           efrom, eto, m, D.deg.p, s->s_pad, s->ngroups, gcount, s0, s1, (T)s->rmax, s->cm[d].p,
+                           s->F[d].p, cnt + d, cap, cnt + 3);
+        hipLaunchKernelGGL(k_first_edge, dim3((unsigned)ceil_div(m, TPB)), dim3(TPB), 0, st, efrom, m, D.former.p,
+                           first, 1);
+    }
+    TSVD_HIP(hipGetLastError());
+    TSVD_HIP(hipStreamSynchronize(st));
+}
+
+void ppr_update_edges(tsvd_ppr *s, int dir, const int32_t *src, const int32_t *dst, int64_t m, int32_t s0,
+                      int32_t s1)
+{
+    TSVD_REQUIRE(dir >= 0 && dir < s->ndir, "bad direction");
+    TSVD_REQUIRE(s0 >= 0 && s0 <= s1 && s1 <= s->nsrc, "bad source range");
+    TSVD_REQUIRE(m >= 0 && m < (int64_t)NO_EDGE, "edge batch too large for the edge-order update");
+    TSVD_REQUIRE(!s->two_sided, "the edge-order update follows the positive-side threshold");
+    if (s0 == s1) return;
+    static_assert(NO_EDGE == 0x7F7F7F7F, "the marker is set with a byte fill");
+    if (s->precision == TSVD_FP64) update_edges_t<double>(s, dir, src, dst, m, s0, s1);
+    else update_edges_t<float>(s, dir, src, dst, m, s0, s1);
+}
+
+void ppr_bind_sources(tsvd_ppr *s, const int32_t *sources, int32_t count)
+{
+    TSVD_REQUIRE(count > 0 && count <= s->s_pad, "source count exceeds the state's width");
+    s->sources.ensure((size_t)s->s_pad);
+    TSVD_HIP(hipMemcpyAsync(s->sources.p, sources, sizeof(int32_t) * count, hipMemcpyHostToDevice, s->g->stream));
+    TSVD_HIP(hipStreamSynchronize(s->g->stream));
+    s->h_sources.assign(sources, sources + count);
+    s->nsrc = count;
+}
+
+void ppr_clear(tsvd_ppr *s, int dir)
+{
+    hipStream_t st = s->g->stream;
+    TSVD_HIP(hipMemsetAsync(s->P[dir].p, 0, s->P[dir].bytes(), st));
+    TSVD_HIP(hipMemsetAsync(s->R[dir].p, 0, s->R[dir].bytes(), st));
+}
+
 void ppr_push(tsvd_ppr *s, int dir, int32_t s0, int32_t s1)
 {
     TSVD_REQUIRE(dir >= 0 && dir < s->ndir, "bad direction");
@@ -886,11 +1070,6 @@ extern "C" int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t ns
             }
             s->nm.alloc(items);
             s->fnext.alloc(items);
-            if (s->ngroups <= 64) {
-                s->delta.alloc(cells * s->esize);
-                s->pm.alloc((size_t)g->n);
-                TSVD_HIP(hipMemsetAsync(s->pm.p, 0, s->pm.bytes(), st));
-            }
             TSVD_HIP(hipMemsetAsync(s->nm.p, 0, s->nm.bytes(), st));
             s->counters.alloc(8);
             s->stats.alloc((size_t)STAT_SHARDS * STAT_FIELDS);
@@ -916,6 +1095,28 @@ extern "C" int tsvd_ppr_refresh(tsvd_ppr *s, int mode, int dir, int32_t src_begi
     return guard([&] {
         TSVD_REQUIRE(s, "null ppr");
         ppr_refresh(s, mode, dir, src_begin, src_end);
+    });
+}
+
+extern "C" int tsvd_ppr_refresh_edges(tsvd_ppr *s, int dir, const int32_t *src, const int32_t *dst, int64_t m,
+                                      int on_device, int32_t src_begin, int32_t src_end)
+{
+    return guard([&] {
+        TSVD_REQUIRE(s && (m == 0 || (src && dst)) && m >= 0, "bad arguments");
+        const int32_t *ds = src, *dd = dst;
+        DBuf<int32_t> bs, bd;
+        if (m > 0 && !on_device) {
+            for (int64_t e = 0; e < m; e++)
+                TSVD_REQUIRE(src[e] >= 0 && src[e] < s->n && dst[e] >= 0 && dst[e] < s->n,
+                             "edge %lld holds a vertex id outside [0, %d)", (long long)e, s->n);
+            bs.alloc((size_t)m);
+            bd.alloc((size_t)m);
+            TSVD_HIP(hipMemcpyAsync(bs.p, src, sizeof(int32_t) * m, hipMemcpyHostToDevice, s->g->stream));
+            TSVD_HIP(hipMemcpyAsync(bd.p, dst, sizeof(int32_t) * m, hipMemcpyHostToDevice, s->g->stream));
+            ds = bs.p;
+            dd = bd.p;
+        }
+        ppr_update_edges(s, dir, ds, dd, m, src_begin, src_end);
     });
 }
 

@@ -4,6 +4,7 @@
 #include "tsvd_graph.h"
 #include "tsvd_ppr.h"
 #include "tsvd_gemm.h"
+#include "tsvd_sparse.h"
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -80,6 +81,18 @@ void blocks_norms(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g
 // Rebuild the stored matrix of the listed blocks from p0 [+ p1].
 void blocks_rebuild(Blocks &b, const std::vector<int32_t> &dirty, const void *p0, const void *p1, hipStream_t st);
 
+// The same steps for the n >= 1e7-node path, one batch of sources at a time
+// (FP32 forward/transpose reserves pf, pt of width sp = rows [row0, row0 +
+// rows) of the matrix; rows >= keep excluded):
+void blocks_active(Blocks &b, const tsvd_graph *g, hipStream_t st);   // active columns; norms zeroed
+void blocks_norms_batch(Blocks &b, const float *pf, const float *pt, int32_t sp, int32_t row0, int32_t rows,
+                        int32_t keep, hipStream_t st);
+void blocks_norms_read(Blocks &b, hipStream_t st);
+void blocks_slots(Blocks &b, const std::vector<int32_t> &dirty, hipStream_t st);
+void blocks_gather_batch(Blocks &b, const std::vector<int32_t> &dirty, const float *pf, const float *pt, int32_t sp,
+                         int32_t row0, int32_t rows, int32_t keep, hipStream_t st);
+void blocks_ncols_read(Blocks &b, hipStream_t st);
+
 void frpca_setup(FrpcaWork &w, const Blocks &b, int32_t k, int32_t q);
 // Factor the listed blocks; leaf factors U*S go to `leaves` (s_pad x k per
 // block, block j at j * s_pad * k).  Returns sum of the kept sigma^2 per block
@@ -109,8 +122,11 @@ void eig_select(double *W, const double *V, const double *lam, int l, int k, int
                 double eps, hipStream_t st);
 
 // LP link scores: scores[e] = sum_c U_j[c * s_pad + rows[e]] * V_j(slot(v), c)
-// for v = dst[e] in block j (0 when v is not a stored column or j has no V).
-void lp_scores(const Blocks &b, const double *V, const double *const *ublk, int32_t k, const int32_t *rows,
+// for v = dst[e] in block j (0 when v is not a stored column or j has no V);
+// with dblk (the n >= 1e7-node path), times dblk[j][rows[e]] * vdeg[v]: the
+// out-degree of the source and the in-degree of v when V_j was computed.
+void lp_scores(const Blocks &b, const double *V, const double *const *ublk, const double *const *dblk,
+               const double *vdeg, int32_t k, const int32_t *rows,
                const int32_t *dst, int64_t m, double *scores, hipStream_t st);
 
 // FP64 vertex-major slab [v0, v0+nv) of the transformed |S| x n matrix
@@ -145,6 +161,12 @@ struct RightEmbedding {
     std::vector<int32_t> ver;                    // per block: snapshot of its U, -1 none
     std::map<int32_t, tsvd::DBuf<double>> U;     // snapshot -> U
     tsvd::DBuf<const double *> ublk;             // per block: device pointer of its U (or null)
+    // n >= 1e7-node path: scores carry the degrees of the snapshot that
+    // computed them (ppr:7707, 7727): out-degrees of the sources per
+    // snapshot, and each vertex's in-degree when its block's V was computed
+    std::map<int32_t, tsvd::DBuf<double>> dsrc;
+    tsvd::DBuf<const double *> dblk;
+    tsvd::DBuf<double> vdeg;
     std::unordered_map<int32_t, int32_t> row_of; // subset vertex -> row
 };
 
@@ -159,6 +181,7 @@ struct tsvd_pipeline {
     tsvd::Tree tree;
     tsvd::GemmRunner gemm;
     tsvd::Eig *eig = nullptr;
+    tsvd::SparseState sp;                    // sparse_state pipelines only
     int32_t snapshot = 0;
     std::vector<int32_t> last_dirty;
     ~tsvd_pipeline();

@@ -44,6 +44,7 @@ struct tsvd_ppr {
     double pull_frac = 0.0;                  // pull when items > pull_frac * n (0: never)
     double ride_frac = 0.5;                  // sub-threshold lanes above ride_frac*rmax ride along (>= 1: off)
     std::vector<int32_t> h_sources;
+    tsvd::DBuf<int32_t> first[2];            // edge-order update: first new edge of a row (made on first use)
     ~tsvd_ppr()
     {
         if (h_counters) (void)hipHostFree(h_counters);
@@ -56,4 +57,13 @@ struct tsvd_ppr {
 namespace tsvd {
 void ppr_refresh(tsvd_ppr *s, int mode, int dir, int32_t s0, int32_t s1);
 void ppr_push(tsvd_ppr *s, int dir, int32_t s0, int32_t s1);
+// Insertion update of the n >= 1e7-node path in batch order (Q8 included);
+// src/dst: the appended batch on the device.
+void ppr_update_edges(tsvd_ppr *s, int dir, const int32_t *src, const int32_t *dst, int64_t m, int32_t s0,
+                      int32_t s1);
+// Rebind the state to `count` sources (at most its width s_pad); the caller
+// refills P and R.
+void ppr_bind_sources(tsvd_ppr *s, const int32_t *sources, int32_t count);
+// Zero P and R of a direction.
+void ppr_clear(tsvd_ppr *s, int dir);
 }

@@ -25,13 +25,13 @@ EXPORTS = [
     "tsvd_graph_degrees", "tsvd_graph_export_csr", "tsvd_read_edge_file", "tsvd_read_subset_file",
     "tsvd_read_lines", "tsvd_write_embedding_csv",
     "tsvd_ppr_create", "tsvd_ppr_destroy", "tsvd_ppr_refresh", "tsvd_ppr_push", "tsvd_ppr_read",
-    "tsvd_ppr_last_push_stats", "tsvd_ppr_set_two_sided",
+    "tsvd_ppr_last_push_stats", "tsvd_ppr_set_two_sided", "tsvd_ppr_refresh_edges",
     "frPCA", "tsvd_frpca_csr", "tsvd_frpca_csr_batch",
     "tsvd_pipeline_create", "tsvd_pipeline_destroy", "tsvd_pipeline_snapshot",
     "tsvd_pipeline_embedding", "tsvd_pipeline_root_sigma", "tsvd_pipeline_graph", "tsvd_pipeline_ppr",
     "tsvd_pipeline_block_matrix", "tsvd_pipeline_block_norms", "tsvd_pipeline_node", "tsvd_pipeline_block_columns",
     "tsvd_pipeline_dense_gram", "tsvd_pipeline_gram_apply", "tsvd_pipeline_right_embedding",
-    "tsvd_pipeline_lp_scores",
+    "tsvd_pipeline_lp_scores", "tsvd_pipeline_source_state",
     "tsvd_server_create", "tsvd_server_destroy", "tsvd_server_submit", "tsvd_server_wait",
     "tsvd_server_pipeline", "tsvd_server_edge_budget", "tsvd_probe_atomic_rate",
 ]
@@ -58,7 +58,8 @@ class Config(ctypes.Structure):
                 ("fanout", ctypes.c_int32), ("rank", ctypes.c_int32), ("q", ctypes.c_int32),
                 ("transform", ctypes.c_int32), ("delta", ctypes.c_double), ("dyn_start", ctypes.c_int32),
                 ("lazy_start", ctypes.c_int32), ("slack", ctypes.c_double), ("seed", ctypes.c_uint64),
-                ("two_sided", ctypes.c_int32), ("right_embedding", ctypes.c_int32)]
+                ("two_sided", ctypes.c_int32), ("right_embedding", ctypes.c_int32),
+                ("sparse_state", ctypes.c_int32), ("sparse_groups", ctypes.c_int32)]
 
 
 class SnapshotStats(ctypes.Structure):
@@ -124,6 +125,8 @@ def lib():
         L.tsvd_ppr_read.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
         L.tsvd_ppr_last_push_stats.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.tsvd_ppr_set_two_sided.argtypes = [vp, ctypes.c_int]
+        L.tsvd_ppr_refresh_edges.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int64, ctypes.c_int,
+                                             ctypes.c_int32, ctypes.c_int32]
         if hasattr(L, "frPCA"):
             L.frPCA.restype = None
             L.frPCA.argtypes = [ctypes.POINTER(MatCSR), vp, vp, vp, ctypes.c_int, ctypes.c_int]
@@ -150,6 +153,7 @@ def lib():
             L.tsvd_pipeline_gram_apply.argtypes = [vp, vp, i32, vp]
             L.tsvd_pipeline_right_embedding.argtypes = [vp, i32, i32, vp]
             L.tsvd_pipeline_lp_scores.argtypes = [vp, vp, vp, i64, vp]
+            L.tsvd_pipeline_source_state.argtypes = [vp, i32, i32, vp, vp]
             L.tsvd_server_create.argtypes = [vp, vp, vp]
             L.tsvd_server_destroy.argtypes = [vp]
             L.tsvd_server_submit.argtypes = [vp, vp, vp, i64, vp]
@@ -321,6 +325,14 @@ class PPR:
         except Exception:
             pass
 
+    def refresh_edges(self, src, dst, d=0, s0=0, s1=None):
+        """The n >= 1e7-node path's insertion update in batch order
+        (tsvd_ppr_refresh_edges), after the same batch was appended."""
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        dst = np.ascontiguousarray(dst, dtype=np.int32)
+        s1 = self.nsrc if s1 is None else s1
+        check(lib().tsvd_ppr_refresh_edges(self.h, int(d), _p(src), _p(dst), int(src.size), 0, int(s0), int(s1)))
+
     def set_two_sided(self, on=True):
         check(lib().tsvd_ppr_set_two_sided(self.h, int(on)))
 
@@ -410,12 +422,12 @@ class Pipeline:
 
     def __init__(self, n, directed, sources, alpha, rmax, nparts, fanout, rank, q=6, precision=TSVD_FP32,
                  transform=TSVD_TRANSFORM_NC, delta=0.65, dyn_start=2, lazy_start=2, slack=0.1, seed=0,
-                 handle=None, two_sided=False, right_embedding=False):
+                 handle=None, two_sided=False, right_embedding=False, sparse_state=False, sparse_groups=0):
         self.sources = np.ascontiguousarray(sources, dtype=np.int32)
         self.nsrc, self.rank, self.n, self.nparts = self.sources.size, rank, n, nparts
         self.cfg = Config(n, int(directed), self.nsrc, self.sources.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                           alpha, rmax, precision, nparts, fanout, rank, q, transform, delta, dyn_start, lazy_start,
-                          slack, seed, int(two_sided), int(right_embedding))
+                          slack, seed, int(two_sided), int(right_embedding), int(sparse_state), int(sparse_groups))
         self.owned = handle is None
         if handle is None:
             h = ctypes.c_void_p()
@@ -503,6 +515,13 @@ class Pipeline:
         check(lib().tsvd_pipeline_lp_scores(self.h, _p(src), _p(dst), int(src.size), _p(out)))
         return out
 
+    def source_state(self, i, d=0):
+        """(reserves, residues) of subset source i in push direction d, n
+        values each (with sparse_state: the stored positive pair lists)."""
+        p, r = np.zeros(self.n), np.zeros(self.n)
+        check(lib().tsvd_pipeline_source_state(self.h, int(d), int(i), _p(p), _p(r)))
+        return p, r
+
     def gram_apply(self, Y):
         """Z = A A^T Y in FP64 for an nsrc x l block Y, without forming A A^T."""
         Y = np.ascontiguousarray(Y, dtype=np.float64)
@@ -544,12 +563,13 @@ class Server:
 
     def __init__(self, n, directed, sources, alpha, rmax, nparts, fanout, rank, q=6, precision=TSVD_FP32,
                  transform=TSVD_TRANSFORM_NC, delta=0.65, dyn_start=2, lazy_start=2, slack=0.1, seed=0,
-                 max_step_edges=0, max_step_requests=0, two_sided=False, right_embedding=False):
+                 max_step_edges=0, max_step_requests=0, two_sided=False, right_embedding=False,
+                 sparse_state=False, sparse_groups=0):
         self.sources = np.ascontiguousarray(sources, dtype=np.int32)
         self.nsrc, self.rank = self.sources.size, rank
         self.cfg = Config(n, int(directed), self.nsrc, self.sources.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                           alpha, rmax, precision, nparts, fanout, rank, q, transform, delta, dyn_start, lazy_start,
-                          slack, seed, int(two_sided), int(right_embedding))
+                          slack, seed, int(two_sided), int(right_embedding), int(sparse_state), int(sparse_groups))
         self.scfg = ServerConfig(int(max_step_edges), int(max_step_requests))
         h = ctypes.c_void_p()
         check(lib().tsvd_server_create(ctypes.byref(self.cfg), ctypes.byref(self.scfg), ctypes.byref(h)))
@@ -557,7 +577,8 @@ class Server:
         self.pipeline = Pipeline(n, directed, self.sources, alpha, rmax, nparts, fanout, rank, q, precision,
                                  transform, delta, dyn_start, lazy_start, slack, seed,
                                  handle=ctypes.c_void_p(lib().tsvd_server_pipeline(h)), two_sided=two_sided,
-                                 right_embedding=right_embedding)
+                                 right_embedding=right_embedding, sparse_state=sparse_state,
+                                 sparse_groups=sparse_groups)
 
     @property
     def edge_budget(self):
