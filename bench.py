@@ -44,7 +44,8 @@ def pipeline_for(capi, w, seed=7):
     return capi.Pipeline(w.n, w.directed, w.subset, w.alpha, w.rmax, w.nparts, w.fanout, w.rank, w.q,
                          precision=capi.TSVD_FP64 if w.fp64_state else capi.TSVD_FP32,
                          transform=capi.TSVD_TRANSFORM_NC if w.transform == "NC" else capi.TSVD_TRANSFORM_LP,
-                         delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed)
+                         delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed,
+                         sparse_state=w.sparse_state)
 
 
 def roofline_of(prof, precision_fp64):
@@ -247,7 +248,8 @@ def served_run(capi, w, reqs, arrivals, cap, label, seed=7):
     srv = capi.Server(w.n, w.directed, w.subset, w.alpha, w.rmax, w.nparts, w.fanout, w.rank, w.q,
                       precision=capi.TSVD_FP64 if w.fp64_state else capi.TSVD_FP32,
                       transform=capi.TSVD_TRANSFORM_NC if w.transform == "NC" else capi.TSVD_TRANSFORM_LP,
-                      delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed, max_step_requests=cap)
+                      delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed, max_step_requests=cap,
+                      sparse_state=w.sparse_state)
     t = time.perf_counter()
     srv.wait(srv.submit(*w.snapshots[0]), embedding=False)
     log(f"{label}: initial snapshot {time.perf_counter() - t:.1f}s")
@@ -289,7 +291,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="batched", choices=["plain", "batched", "precision"])
+    ap.add_argument("--workload", default="batched", choices=["plain", "batched", "precision", "served"])
+    ap.add_argument("--transform", choices=["NC", "LP"], help="override the workload's block transform")
+    ap.add_argument("--shard", default=None, metavar="I/N",
+                    help="run GPU I's share of an N-GPU source-sharded job: the I-th of N contiguous "
+                         "slices of the subset")
+    ap.add_argument("--sparse-state", action="store_true",
+                    help="the n >= 1e7-node path: pair-list state, pushed a batch of source groups at a time "
+                         "(directed workloads, LP transform)")
     ap.add_argument("--scale-down", type=int, default=0, help="divide nodes/edges by 2**k (smoke runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--accuracy", action="store_true",
@@ -308,6 +317,20 @@ def main():
     from treesvd_amd import capi, gen
     t = time.time()
     w = gen.workload(args.workload, scale_down=args.scale_down)
+    if args.transform:
+        w.transform = args.transform
+    shard = None
+    if args.shard:
+        i, nsh = (int(x) for x in args.shard.split("/"))
+        if not 0 <= i < nsh:
+            raise SystemExit("--shard I/N needs 0 <= I < N")
+        w.subset = np.array_split(w.subset, nsh)[i].copy()
+        shard = {"index": i, "of": nsh, "sources": int(w.subset.size)}
+        log(f"shard {i}/{nsh}: {w.subset.size} sources")
+    if args.sparse_state:
+        if not w.directed:
+            raise SystemExit("--sparse-state is the directed LP path")
+        w.transform, w.sparse_state = "LP", True
     log(f"workload {args.workload}: n={w.n} snapshots={len(w.snapshots)} |S|={w.subset.size} "
         f"gen {time.time() - t:.1f}s")
     need = args.warmup + args.steps
@@ -335,6 +358,9 @@ def main():
     init_ms = (time.perf_counter() - t) * 1e3
     log(f"initial snapshot {init_ms:.0f} ms (push {st0['ms_push']:.0f} ms, {st0['push_rounds']} rounds, "
         f"frpca {st0['ms_frpca']:.0f} ms, {st0['ndirty']} blocks)")
+    if w.sparse_state:
+        log(f"pair-list state: {st0['state_entries']:.4e} entries ({st0['state_entries'] * 8 / 1e9:.2f} GB), "
+            f"{st0['state_batches']} source batches")
     warm = []
     for i in range(args.warmup):
         st = pipe.snapshot(*batches[1 + i])
@@ -368,6 +394,12 @@ def main():
     nmerged = float(np.mean([s["nmerged"] for s in stats]))
     phases = {k: round(float(np.mean([s[k] for s in stats])), 3)
               for k in ("ms_ingest", "ms_refresh", "ms_push", "ms_blocks", "ms_frpca", "ms_merge")}
+    state = None
+    if w.sparse_state:
+        ent = stats[-1]["state_entries"]
+        state = {"batches": stats[-1]["state_batches"], "entries": int(ent), "GB": round(ent * 8 / 1e9, 3),
+                 "dense_equivalent_GB": round(4 * w.n * (-(-w.subset.size // 64) * 64) * 4 / 1e9, 3),
+                 "note": "positive (vertex, FP32 value) pairs of p and r, both directions, all sources"}
     out = {
         "metric": "update->embedding snapshots/s (plain GPU method), p50/p99 ms per update; served vs plain",
         "value": round(value, 4), "unit": "snapshots/s", "n_gpus": 1, "steps": args.steps,
@@ -377,10 +409,11 @@ def main():
         "config": {"workload": args.workload, "n": w.n, "directed": w.directed,
                    "edges": int(sum(s.size for s, _ in w.snapshots)), "sources": int(w.subset.size),
                    "alpha": w.alpha, "rmax": w.rmax, "nparts": w.nparts, "fanout": w.fanout, "rank": w.rank,
-                   "q": w.q, "scale_down": args.scale_down, "method": "plain"},
+                   "q": w.q, "scale_down": args.scale_down, "method": "plain", "transform": w.transform,
+                   "sparse_state": w.sparse_state, "shard": shard},
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "initial_snapshot_ms": round(init_ms, 1), "phase_ms": phases,
-        "dirty_blocks_per_step": ndirty, "merged_nodes_per_step": nmerged,
+        "dirty_blocks_per_step": ndirty, "merged_nodes_per_step": nmerged, "pair_list_state": state,
         "push_atomic_roofline": {"achieved": round(achieved_rate / 1e9, 3), "ceiling": round(ceiling / 1e9, 3),
                                  "unit": "G wave-atomics/s", "frac": round(achieved_rate / ceiling, 4),
                                  "note": "plain-method push (round + hub kernels); ceiling = returning 64-lane "

@@ -202,6 +202,8 @@ typedef struct {
     int64_t push_items;       /* (vertex, 64-source group) work items popped   */
     int64_t push_wave_relax;  /* neighbour visits summed over those items      */
     double ms_total, ms_ingest, ms_refresh, ms_push, ms_blocks, ms_frpca, ms_merge;
+    int64_t state_entries;    /* sparse_state: (vertex, value) pairs stored, all lists */
+    int32_t state_batches;    /* sparse_state: source batches pushed per snapshot     */
 } tsvd_snapshot_stats;
 
 int tsvd_pipeline_create(const tsvd_config *cfg, tsvd_pipeline **out);

@@ -44,9 +44,10 @@ static void validate(const tsvd_config &c)
     TSVD_REQUIRE(c.sparse_groups >= 0, "sparse_groups must be >= 0");
 }
 
-// Batches of the sparse state: as many groups of 64 sources as half of the
+// Batches of the sparse state: as many groups of 64 sources as 30% of the
 // free HBM holds (four FP32 arrays of n x 64 per group plus the push's item
-// lists), the rest left for the pair lists.
+// lists); the rest is left for the pair lists, which on power-law graphs
+// can outgrow the batch state (a hub's residue reaches millions of vertices).
 static void sparse_setup(tsvd_pipeline *p, hipStream_t st)
 {
     const tsvd_config &c = p->cfg;
@@ -57,7 +58,7 @@ static void sparse_setup(tsvd_pipeline *p, hipStream_t st)
         size_t fr = 0, tot = 0;
         TSVD_HIP(hipMemGetInfo(&fr, &tot));
         const double per = (double)c.n * (4.0 * 64 * 4 + 36.0);
-        gpb = (int32_t)std::max(1.0, std::floor(0.5 * (double)fr / per));
+        gpb = (int32_t)std::max(1.0, std::floor(0.3 * (double)fr / per));
     }
     ss.gpb = std::min(gpb, ngroups);
     ss.nbatch = (int32_t)ceil_div(ngroups, ss.gpb);
@@ -223,6 +224,9 @@ static void sparse_step(tsvd_pipeline *p, int32_t it, const int32_t *dsrc, const
         blocks_ncols_read(b, st);
     }
     S.ms_blocks += now_ms() - t3;
+    S.state_batches = ss.nbatch;
+    for (const auto &L : ss.lists)
+        for (const auto &x : L) S.state_entries += x.nnz;
 }
 
 }  // namespace tsvd

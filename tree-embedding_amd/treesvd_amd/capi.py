@@ -68,7 +68,8 @@ class SnapshotStats(ctypes.Structure):
                 ("push_relax", ctypes.c_int64), ("push_items", ctypes.c_int64),
                 ("push_wave_relax", ctypes.c_int64), ("ms_total", ctypes.c_double), ("ms_ingest", ctypes.c_double),
                 ("ms_refresh", ctypes.c_double), ("ms_push", ctypes.c_double), ("ms_blocks", ctypes.c_double),
-                ("ms_frpca", ctypes.c_double), ("ms_merge", ctypes.c_double)]
+                ("ms_frpca", ctypes.c_double), ("ms_merge", ctypes.c_double),
+                ("state_entries", ctypes.c_int64), ("state_batches", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -76,6 +76,21 @@ def chung_lu_edges(n: int, m: int, seed: int, out_exp: float, in_exp: float,
     c_out = np.cumsum(w_out); c_out /= c_out[-1]
     c_in = np.cumsum(w_in); c_in /= c_in[-1]
     keys = np.empty(0, dtype=np.int64)
+    if m > 4 * chunk:
+        # Large graphs: draws in one go, looked up as sorted queries (random
+        # lookups in the CDF miss cache on every draw), the u sample randomly
+        # matched to the v sample (still independent pairs); distinct keys,
+        # then a random m of them.
+        while keys.size < m:
+            batch = int((m - keys.size) * 1.3) + 4096
+            u = np.searchsorted(c_out, np.sort(rng.random(batch)), side="right").astype(np.int64)
+            u = u[rng.permutation(batch)]
+            v = np.searchsorted(c_in, np.sort(rng.random(batch)), side="right").astype(np.int64)
+            u = np.minimum(u, n - 1); v = np.minimum(v, n - 1)
+            keep = u != v
+            keys = np.unique(np.concatenate([keys, u[keep] * n + v[keep]]))
+        keys = keys[rng.permutation(keys.size)[:m]]
+        return (keys // n).astype(np.int32), (keys % n).astype(np.int32)
     while keys.size < m:
         need = m - keys.size
         batch = min(chunk, int(need * 1.3) + 4096)
@@ -143,6 +158,7 @@ class Workload:
     fp64_state: bool           # PPR reserves/residues kept in FP64
     transform: str = "NC"      # "NC": log10(x/rmax) for x>rmax ; "LP": log10(1+x/rmax)
     delta: float = 0.65
+    sparse_state: bool = False  # the n >= 1e7-node path's pair-list state (directed LP)
 
 
 def workload(name: str, scale_down: int = 0, nsub: int | None = None,
@@ -157,6 +173,10 @@ def workload(name: str, scale_down: int = 0, nsub: int | None = None,
     precision  directed Chung-Lu, 2,738,013 nodes, 13,960,811 edges, exponents
                out 2.1 / in 2.3, 80% + 8 equal batches, |S|=3000, rmax 1e-7,
                512 blocks (ragged fan-out-4 tree), rank 128.
+    served     directed Chung-Lu, 6,216,199 nodes, 177,862,656 edges, exponents
+               in 2.0 / out 2.3, 80% initial + 60 lognormal batches (median
+               10k, sigma 1, clipped to [1k, 1M]), |S|=3000, rmax 1e-7, 1024
+               blocks, rank 128, fan-out 4, LP transform.
     `scale_down` divides node and edge counts by 2**scale_down (tests only).
     """
     sd = 1 << scale_down
@@ -184,6 +204,16 @@ def workload(name: str, scale_down: int = 0, nsub: int | None = None,
         snaps = shuffle_split(s, d, 2, 0.8, sizes)
         sub = sample_subset(snaps[0][0], nsub or 3000, 2)
         return Workload(name, n, True, snaps, sub, 0.2, 1e-7, 512, 4, 128, 6, False)
+    if name == "served":
+        n = 6_216_199 // sd
+        m = 177_862_656 // sd
+        s, d = chung_lu_edges(n, m, seed=3, out_exp=2.3, in_exp=2.0)
+        # 60 batches: the plain steps take the first few, the served stream the rest
+        k = n_snap if n_snap is not None else 60
+        sizes = lognormal_sizes(k, 10_000 / sd, 1.0, 3, max(1, 1_000 // sd), 1_000_000 // sd)
+        snaps = shuffle_split(s, d, 3, 0.8, sizes)
+        sub = sample_subset(snaps[0][0], nsub or 3000, 3)
+        return Workload(name, n, True, snaps, sub, 0.2, 1e-7, 1024, 4, 128, 6, False, transform="LP")
     raise ValueError(f"unknown workload {name!r}")
 
 
