@@ -242,14 +242,15 @@ def accuracy_report(pipe, w, seed=7):
 def served_run(capi, w, reqs, arrivals, cap, label, seed=7):
     """Replays a request stream against a server on its own clock: a feeder
     thread submits request i at arrivals[i] seconds, this thread collects the
-    results in order.  cap = 1 is the plain per-snapshot method, cap = 0 the
-    served form (every queued request admitted into the next step)."""
+    results in order.  cap = 1 is the plain per-snapshot method (steps one at
+    a time), cap = 0 the served form (every queued request admitted into the
+    next step, the back part of each step overlapping the next step's push)."""
     import threading
     srv = capi.Server(w.n, w.directed, w.subset, w.alpha, w.rmax, w.nparts, w.fanout, w.rank, w.q,
                       precision=capi.TSVD_FP64 if w.fp64_state else capi.TSVD_FP32,
                       transform=capi.TSVD_TRANSFORM_NC if w.transform == "NC" else capi.TSVD_TRANSFORM_LP,
                       delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed, max_step_requests=cap,
-                      sparse_state=w.sparse_state)
+                      sparse_state=w.sparse_state, overlap=cap != 1)
     t = time.perf_counter()
     srv.wait(srv.submit(*w.snapshots[0]), embedding=False)
     log(f"{label}: initial snapshot {time.perf_counter() - t:.1f}s")

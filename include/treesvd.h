@@ -266,12 +266,16 @@ int tsvd_pipeline_source_state(tsvd_pipeline *p, int32_t dir, int32_t i, double 
  * up to max_step_edges (0: half of the free HBM at 64 B per edge) and
  * max_step_requests (0: no cap; 1: the plain per-snapshot method), appends
  * them as one snapshot and completes them with that snapshot's embedding.
- * Requests arriving during a step wait for the next one.                   */
+ * Requests arriving during a step wait for the next one.  Steps overlap on
+ * the device: the frPCA, merges and root of step t run on a second stream
+ * while step t+1 appends, updates and pushes; step t+1's dirty test waits for
+ * step t (no_overlap = 1, or right_embedding, runs steps one at a time).   */
 typedef struct tsvd_server tsvd_server;
 
 typedef struct {
     int64_t max_step_edges;
     int32_t max_step_requests;
+    int32_t no_overlap;
 } tsvd_server_config;
 
 typedef struct {

@@ -154,3 +154,22 @@ def test_served_rejects_bad_ids_and_unknown_tickets(capi):
     with pytest.raises(capi.TreeSVDError):
         srv.wait(t)                      # collected once
     srv.close()
+
+
+def test_served_overlap_matches_steps_one_at_a_time(capi):
+    """Back parts of step t (frPCA, merges, root) overlap the front part of
+    step t+1 on another stream; with the same steps (cap 1) the results equal
+    those of a server that runs steps one at a time, within the push-order
+    tolerance, and requests complete in order."""
+    n, snaps, sub = workload(seed=11)
+    outs = {}
+    for overlap in (True, False):
+        srv = capi.Server(n, False, sub, max_step_requests=1, overlap=overlap, **ARGS)
+        tickets = [srv.submit(s, d) for s, d in snaps]
+        res = [srv.wait(t) for t in tickets]
+        assert [r["snapshot"] for _, r in res] == list(range(1, len(snaps) + 1))
+        assert all(r["done_ms"] >= r["start_ms"] for _, r in res)
+        outs[overlap] = [E for E, _ in res]
+        srv.close()
+    for t, (Ea, Eb) in enumerate(zip(outs[True], outs[False])):
+        same_embedding(Eb, Ea, ARGS["rank"], True, f"ticket {t}, overlapped vs one at a time")

@@ -81,7 +81,10 @@ __global__ void k_check_cap(const int64_t *__restrict__ rowoff, const int32_t *_
     if ((int64_t)deg[v] + cnt[v] > cap) atomicOr(overflow, 1);
 }
 
-// New capacity of every row: need*(1+slack) rounded up, at least need+2.
+// New capacity of every row: need*(1+slack) rounded up, at least need+2 --
+// empty rows included, so that a vertex's first edges (common on power-law
+// graphs, where many vertices have no in- or out-edge yet) do not force a
+// rebuild of the whole CSR.
 __global__ void k_new_cap(const int32_t *__restrict__ deg, const int32_t *__restrict__ cnt, int32_t n, double slack,
                           int64_t *__restrict__ cap)
 {
@@ -89,7 +92,7 @@ __global__ void k_new_cap(const int32_t *__restrict__ deg, const int32_t *__rest
     if (v >= n) return;
     int64_t need = (int64_t)deg[v] + cnt[v];
     int64_t c = need + (int64_t)ceil((double)need * slack);
-    if (need > 0 && c < need + 2) c = need + 2;
+    if (c < need + 2) c = need + 2;
     cap[v] = c;
 }
 

@@ -159,10 +159,11 @@ static void decide_dirty(const tsvd_config &c, const Blocks &b, int32_t it, std:
 // of source groups at a time: load its pair lists into the dense state (or
 // start it afresh before dyn_start), update in batch order, push both
 // directions, store the positive entries back, and add the batch's rows to
-// every block's norms.  Dirty blocks are then rebuilt from the stored
-// reserve lists (the matrix rows of a batch need both directions' p).
-static void sparse_step(tsvd_pipeline *p, int32_t it, const int32_t *dsrc, const int32_t *ddst, int64_t m,
-                        tsvd_snapshot_stats &S, std::vector<int32_t> &rebuild, std::vector<int32_t> &factor)
+// every block's norms.  sparse_rebuild then rebuilds the dirty blocks from
+// the stored reserve lists (the matrix rows of a batch need both directions'
+// p).
+static void sparse_front(tsvd_pipeline *p, int32_t it, const int32_t *dsrc, const int32_t *ddst, int64_t m,
+                         tsvd_snapshot_stats &S)
 {
     const tsvd_config &c = p->cfg;
     SparseState &ss = p->sp;
@@ -207,7 +208,19 @@ static void sparse_step(tsvd_pipeline *p, int32_t it, const int32_t *dsrc, const
     }
     const double t3 = now_ms();
     blocks_norms_read(b, st);
-    decide_dirty(c, b, it, rebuild, factor);
+    S.ms_blocks += now_ms() - t3;
+    S.state_batches = ss.nbatch;
+    for (const auto &L : ss.lists)
+        for (const auto &x : L) S.state_entries += x.nnz;
+}
+
+static void sparse_rebuild(tsvd_pipeline *p, const std::vector<int32_t> &rebuild)
+{
+    const tsvd_config &c = p->cfg;
+    SparseState &ss = p->sp;
+    tsvd_ppr *gp = ss.gp;
+    Blocks &b = p->blocks;
+    hipStream_t st = p->g->stream;
     if (!rebuild.empty()) {
         blocks_slots(b, rebuild, st);
         for (int32_t bt = 0; bt < ss.nbatch; bt++) {
@@ -223,10 +236,97 @@ static void sparse_step(tsvd_pipeline *p, int32_t it, const int32_t *dsrc, const
         }
         blocks_ncols_read(b, st);
     }
-    S.ms_blocks += now_ms() - t3;
-    S.state_batches = ss.nbatch;
-    for (const auto &L : ss.lists)
-        for (const auto &x : L) S.state_entries += x.nnz;
+}
+
+bool pipeline_front(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, int64_t m, int on_device,
+                    StepPlan &plan)
+{
+    const tsvd_config &c = p->cfg;
+    hipStream_t st = p->g->stream;
+    plan = StepPlan{};
+    tsvd_snapshot_stats &S = plan.S;
+    plan.t0 = now_ms();
+    graph_append(p->g, src, dst, m, on_device);
+    plan.it = ++p->snapshot;
+    S.snapshot = plan.it;
+    const double t1 = now_ms();
+    S.ms_ingest = t1 - plan.t0;
+    // Tree_u_NC.cpp:516-518: snapshots before lazy_start-1 are only ingested
+    // (lazy_start 100 turns that off; 200 in the n >= 1e7-node path, Tree_d_LP.cpp:1574)
+    if (c.lazy_start != (c.sparse_state ? 200 : 100) && plan.it < c.lazy_start - 1) {
+        plan.skipped = true;
+        S.reused = 1;
+        return false;
+    }
+    Blocks &b = p->blocks;
+    if (c.sparse_state) {
+        const int32_t *dsrc = on_device ? src : p->g->e_src.p;
+        const int32_t *ddst = on_device ? dst : p->g->e_dst.p;
+        sparse_front(p, plan.it, dsrc, ddst, m, S);
+    } else {
+        const int mode = plan.it < c.dyn_start ? TSVD_REFRESH_INIT : TSVD_REFRESH_DYNAMIC;
+        for (int d = 0; d < p->ppr->ndir; d++) ppr_refresh(p->ppr, mode, d, 0, c.nsrc);
+        const double t2 = now_ms();
+        S.ms_refresh = t2 - t1;
+        for (int d = 0; d < p->ppr->ndir; d++) {
+            ppr_push(p->ppr, d, 0, c.nsrc);
+            S.push_rounds += p->ppr->last_rounds[d];
+            S.push_pops += p->ppr->last_pops[d];
+            S.push_relax += p->ppr->last_relax[d];
+            S.push_items += p->ppr->last_items[d];
+            S.push_wave_relax += p->ppr->last_wave_relax[d];
+        }
+        const double t3 = now_ms();
+        S.ms_push = t3 - t2;
+        blocks_norms(b, p->ppr->p(0), p->ppr->ndir > 1 ? p->ppr->p(1) : nullptr, p->g, st);
+        S.ms_blocks = now_ms() - t3;
+    }
+    plan.a2 = b.a2;
+    return true;
+}
+
+void pipeline_decide(tsvd_pipeline *p, StepPlan &plan)
+{
+    const double t0 = now_ms();
+    Blocks &b = p->blocks;
+    decide_dirty(p->cfg, b, plan.it, plan.rebuild, plan.factor);
+    if (p->cfg.sparse_state)
+        sparse_rebuild(p, plan.rebuild);
+    else
+        blocks_rebuild(b, plan.rebuild, p->ppr->p(0), p->ppr->ndir > 1 ? p->ppr->p(1) : nullptr, p->g->stream);
+    p->last_dirty = plan.factor;
+    plan.S.ndirty = (int32_t)plan.factor.size();
+    plan.S.ms_blocks += now_ms() - t0;
+}
+
+void pipeline_back(tsvd_pipeline *p, StepPlan &plan, hipStream_t st)
+{
+    const tsvd_config &c = p->cfg;
+    tsvd_snapshot_stats &S = plan.S;
+    Blocks &b = p->blocks;
+    const double t4 = now_ms();
+    if (!plan.skipped) {
+        const std::vector<int32_t> &factor = plan.factor;
+        if (!factor.empty()) {
+            std::vector<double> sig2;
+            frpca_blocks(p->fw, b, factor, p->tree.level[0].p, p->gemm, *p->eig, c.seed, st, sig2);
+            if (plan.it >= c.lazy_start - 1)
+                for (size_t i = 0; i < factor.size(); i++) {
+                    const int32_t j = factor[i];
+                    b.resid[j] = std::sqrt(std::max(0.0, plan.a2[j] - sig2[i]));
+                }
+        }
+        const double t5 = now_ms();
+        S.ms_frpca = t5 - t4;
+        if (factor.empty() && p->tree.have_root) {
+            S.reused = 1;
+        } else if (!factor.empty()) {
+            S.nmerged = tree_update(p->tree, factor, c.transform, p->gemm, *p->eig, st);
+        }
+        if (c.right_embedding) right_embedding_update(p, plan.rebuild, plan.it, st);
+        S.ms_merge = now_ms() - t5;
+    }
+    S.ms_total = now_ms() - plan.t0;
 }
 
 }  // namespace tsvd
@@ -306,77 +406,10 @@ extern "C" int tsvd_pipeline_snapshot(tsvd_pipeline *p, const int32_t *src, cons
 {
     return guard([&] {
         TSVD_REQUIRE(p, "null pipeline");
-        const tsvd_config &c = p->cfg;
-        hipStream_t st = p->g->stream;
-        tsvd_snapshot_stats S{};
-        const double t0 = now_ms();
-        graph_append(p->g, src, dst, m, on_device);
-        const int32_t it = ++p->snapshot;
-        S.snapshot = it;
-        const double t1 = now_ms();
-        S.ms_ingest = t1 - t0;
-        // Tree_u_NC.cpp:516-518: snapshots before lazy_start-1 are only ingested
-        // (lazy_start 100 turns that off; 200 in the n >= 1e7-node path, Tree_d_LP.cpp:1574)
-        if (c.lazy_start != (c.sparse_state ? 200 : 100) && it < c.lazy_start - 1) {
-            S.reused = 1;
-            S.ms_total = now_ms() - t0;
-            if (stats) *stats = S;
-            return;
-        }
-        Blocks &b = p->blocks;
-        std::vector<int32_t> rebuild, factor;
-        if (c.sparse_state) {
-            const int32_t *dsrc = on_device ? src : p->g->e_src.p;
-            const int32_t *ddst = on_device ? dst : p->g->e_dst.p;
-            sparse_step(p, it, dsrc, ddst, m, S, rebuild, factor);
-        } else {
-            const int mode = it < c.dyn_start ? TSVD_REFRESH_INIT : TSVD_REFRESH_DYNAMIC;
-            for (int d = 0; d < p->ppr->ndir; d++) ppr_refresh(p->ppr, mode, d, 0, c.nsrc);
-            const double t2 = now_ms();
-            S.ms_refresh = t2 - t1;
-            for (int d = 0; d < p->ppr->ndir; d++) {
-                ppr_push(p->ppr, d, 0, c.nsrc);
-                S.push_rounds += p->ppr->last_rounds[d];
-                S.push_pops += p->ppr->last_pops[d];
-                S.push_relax += p->ppr->last_relax[d];
-                S.push_items += p->ppr->last_items[d];
-                S.push_wave_relax += p->ppr->last_wave_relax[d];
-            }
-            const double t3 = now_ms();
-            S.ms_push = t3 - t2;
-            const void *p0 = p->ppr->p(0);
-            const void *p1 = p->ppr->ndir > 1 ? p->ppr->p(1) : nullptr;
-            blocks_norms(b, p0, p1, p->g, st);
-            decide_dirty(c, b, it, rebuild, factor);
-            blocks_rebuild(b, rebuild, p0, p1, st);
-            S.ms_blocks = now_ms() - t3;
-        }
-        const double t4 = now_ms();
-
-        p->last_dirty = factor;
-        S.ndirty = (int32_t)factor.size();
-        if (!factor.empty()) {
-            std::vector<double> sig2;
-            frpca_blocks(p->fw, b, factor, p->tree.level[0].p, p->gemm, *p->eig,
-                         c.seed, st, sig2);
-            if (it >= c.lazy_start - 1)
-                for (size_t i = 0; i < factor.size(); i++) {
-                    const int32_t j = factor[i];
-                    b.resid[j] = std::sqrt(std::max(0.0, b.a2[j] - sig2[i]));
-                }
-        }
-        const double t5 = now_ms();
-        S.ms_frpca = t5 - t4;
-        if (factor.empty() && p->tree.have_root) {
-            S.reused = 1;
-        } else if (!factor.empty()) {
-            S.nmerged = tree_update(p->tree, factor, c.transform, p->gemm, *p->eig, st);
-        }
-        if (c.right_embedding) right_embedding_update(p, rebuild, it, st);
-        const double t6 = now_ms();
-        S.ms_merge = t6 - t5;
-        S.ms_total = t6 - t0;
-        if (stats) *stats = S;
+        StepPlan plan;
+        if (pipeline_front(p, src, dst, m, on_device, plan)) pipeline_decide(p, plan);
+        pipeline_back(p, plan, p->g->stream);
+        if (stats) *stats = plan.S;
     });
 }
 

@@ -14,6 +14,15 @@
 // per request.  A request cap of 1 is the plain per-snapshot method under
 // the same arrival stream.
 //
+// Steps overlap on the device: a second thread runs the back part of step t
+// (frPCA of the dirty blocks, merges, root embedding) on its own stream while
+// the worker runs the front part of step t+1 (append, insertion update,
+// push, block norms) on the graph's stream.  The worker then waits for step
+// t to finish before the dirty test and block rebuild of step t+1, which
+// read the residuals and rewrite the blocks step t uses.  With the LP right
+// embedding the steps run one after the other (its scores read the degrees
+// of their own snapshot).
+//
 // Appending batches b1..bk as one snapshot gives the same CSR rows as
 // appending them one by one (each row gains its neighbours in batch order),
 // and the insertion update walks every new slot in order, so the push
@@ -45,6 +54,15 @@ struct Done {
     std::shared_ptr<std::vector<double>> E;
 };
 
+struct Job {
+    std::vector<Request> step;
+    int64_t edges = 0;
+    double t_start = 0.0;
+    StepPlan plan;
+    int rc = TSVD_OK;
+    std::string err;
+};
+
 }  // namespace tsvd
 
 using namespace tsvd;
@@ -62,6 +80,14 @@ struct tsvd_server {
     bool stop = false;
     std::thread worker;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    // back parts: one job at a time, on their own stream, by the finisher thread
+    bool overlap = false;
+    hipStream_t back_stream = nullptr;
+    std::mutex fmu;
+    std::condition_variable fcv;
+    std::unique_ptr<Job> pending;
+    bool back_busy = false, fstop = false;
+    std::thread finisher;
 
     double now_ms() const
     {
@@ -69,66 +95,117 @@ struct tsvd_server {
     }
 
     void run();
+    void finish_loop();
+    void finish(Job &job, hipStream_t st);
+    void wait_back_idle()
+    {
+        std::unique_lock<std::mutex> lk(fmu);
+        fcv.wait(lk, [&] { return !back_busy; });
+    }
+    ~tsvd_server()
+    {
+        if (back_stream) (void)hipStreamDestroy(back_stream);
+    }
 };
 
 void tsvd_server::run()
 {
-    const int32_t ns = pipe->cfg.nsrc, k = pipe->cfg.rank;
     for (;;) {
-        std::vector<Request> step;
-        int64_t edges = 0;
+        std::unique_ptr<Job> job(new Job());
         {
             std::unique_lock<std::mutex> lk(mu);
             cv_work.wait(lk, [&] { return stop || !queue.empty(); });
-            if (queue.empty()) return;   // stop requested and the queue is drained
+            if (queue.empty()) break;   // stop requested and the queue is drained
             while (!queue.empty()) {
                 const Request &r = queue.front();
                 const int64_t m = (int64_t)r.src.size();
-                if (!step.empty()) {
-                    if (scfg.max_step_requests > 0 && (int32_t)step.size() >= scfg.max_step_requests) break;
-                    if (edges + m > edge_budget) break;
+                if (!job->step.empty()) {
+                    if (scfg.max_step_requests > 0 && (int32_t)job->step.size() >= scfg.max_step_requests) break;
+                    if (job->edges + m > edge_budget) break;
                 }
-                edges += m;
-                step.push_back(std::move(queue.front()));
+                job->edges += m;
+                job->step.push_back(std::move(queue.front()));
                 queue.pop_front();
             }
         }
-        const double t_start = now_ms();
+        job->t_start = now_ms();
         std::vector<int32_t> src, dst;
-        src.reserve((size_t)edges);
-        dst.reserve((size_t)edges);
-        for (const Request &r : step) {
+        src.reserve((size_t)job->edges);
+        dst.reserve((size_t)job->edges);
+        for (const Request &r : job->step) {
             src.insert(src.end(), r.src.begin(), r.src.end());
             dst.insert(dst.end(), r.dst.begin(), r.dst.end());
         }
-        tsvd_snapshot_stats st{};
-        int rc = tsvd_pipeline_snapshot(pipe, src.data(), dst.data(), edges, 0, &st);
-        auto E = std::make_shared<std::vector<double>>((size_t)ns * k, 0.0);
-        if (rc == TSVD_OK) rc = tsvd_pipeline_embedding(pipe, E->data());
-        const std::string err = rc == TSVD_OK ? std::string() : std::string(tsvd_last_error());
-        const double t_done = now_ms();
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            for (const Request &r : step) {
-                Done &d = done[r.ticket];
-                d.rc = rc;
-                d.err = err;
-                d.E = E;
-                d.res.ticket = r.ticket;
-                d.res.snapshot = st.snapshot;
-                d.res.step_requests = (int32_t)step.size();
-                d.res.step_edges = edges;
-                d.res.last_ticket = step.back().ticket;
-                d.res.submit_ms = r.submit_ms;
-                d.res.start_ms = t_start;
-                d.res.done_ms = t_done;
-                d.res.latency_ms = t_done - r.submit_ms;
-                d.res.step = st;
-            }
-            completed_below = step.back().ticket + 1;
+        bool front = false;
+        int rc = guard([&] { front = pipeline_front(pipe, src.data(), dst.data(), job->edges, 0, job->plan); });
+        if (overlap) wait_back_idle();   // the previous step's back part is done
+        if (rc == TSVD_OK && front) rc = guard([&] { pipeline_decide(pipe, job->plan); });
+        job->rc = rc;
+        if (rc != TSVD_OK) job->err = tsvd_last_error();
+        if (overlap) {
+            std::lock_guard<std::mutex> lk(fmu);
+            pending = std::move(job);
+            back_busy = true;
+            fcv.notify_all();
+        } else {
+            finish(*job, pipe->g->stream);
         }
-        cv_done.notify_all();
     }
+    if (overlap) wait_back_idle();
+}
+
+void tsvd_server::finish_loop()
+{
+    for (;;) {
+        std::unique_ptr<Job> job;
+        {
+            std::unique_lock<std::mutex> lk(fmu);
+            fcv.wait(lk, [&] { return fstop || pending; });
+            if (!pending) return;
+            job = std::move(pending);
+        }
+        finish(*job, back_stream);
+        {
+            std::lock_guard<std::mutex> lk(fmu);
+            back_busy = false;
+        }
+        fcv.notify_all();
+    }
+}
+
+// Back part of a step, the embedding, and the completion of its requests.
+void tsvd_server::finish(Job &job, hipStream_t st)
+{
+    const int32_t ns = pipe->cfg.nsrc, k = pipe->cfg.rank;
+    int rc = job.rc;
+    std::string err = job.err;
+    if (rc == TSVD_OK) rc = guard([&] { pipeline_back(pipe, job.plan, st); });
+    auto E = std::make_shared<std::vector<double>>((size_t)ns * k, 0.0);
+    if (rc == TSVD_OK) rc = tsvd_pipeline_embedding(pipe, E->data());
+    if (rc != TSVD_OK && err.empty()) err = tsvd_last_error();
+    const double t_done = now_ms();
+    const tsvd_snapshot_stats &st_ = job.plan.S;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (const Request &r : job.step) {
+            Done &d = done[r.ticket];
+            d.rc = rc;
+            d.err = err;
+            d.E = E;
+            d.res.ticket = r.ticket;
+            d.res.snapshot = st_.snapshot;
+            d.res.step_requests = (int32_t)job.step.size();
+            d.res.step_edges = job.edges;
+            d.res.last_ticket = job.step.back().ticket;
+            d.res.submit_ms = r.submit_ms;
+            d.res.start_ms = job.t_start;
+            d.res.done_ms = t_done;
+            d.res.latency_ms = t_done - r.submit_ms;
+            d.res.step = st_;
+        }
+        completed_below = job.step.back().ticket + 1;
+    }
+    cv_done.notify_all();
 }
 
 extern "C" int tsvd_server_create(const tsvd_config *cfg, const tsvd_server_config *scfg, tsvd_server **out)
@@ -142,6 +219,11 @@ extern "C" int tsvd_server_create(const tsvd_config *cfg, const tsvd_server_conf
         s->scfg = sc;
         int rc = tsvd_pipeline_create(cfg, &s->pipe);
         if (rc) throw Error{rc};
+        s->overlap = !sc.no_overlap && !cfg->right_embedding;
+        if (s->overlap) {
+            TSVD_HIP(hipStreamCreateWithFlags(&s->back_stream, hipStreamNonBlocking));
+            s->finisher = std::thread([p = s.get()] { p->finish_loop(); });
+        }
         if (sc.max_step_edges > 0) {
             s->edge_budget = sc.max_step_edges;
         } else {
@@ -167,6 +249,12 @@ extern "C" int tsvd_server_destroy(tsvd_server *s)
         }
         s->cv_work.notify_all();
         if (s->worker.joinable()) s->worker.join();
+        {
+            std::lock_guard<std::mutex> lk(s->fmu);
+            s->fstop = true;
+        }
+        s->fcv.notify_all();
+        if (s->finisher.joinable()) s->finisher.join();
         if (s->pipe) tsvd_pipeline_destroy(s->pipe);
         delete s;
     });

@@ -170,6 +170,27 @@ struct RightEmbedding {
     std::unordered_map<int32_t, int32_t> row_of; // subset vertex -> row
 };
 
+namespace tsvd {
+// One snapshot in three parts, so that a server can run the back part of
+// step t (frPCA, merges, root: stream `st`) while the front part of step t+1
+// (ingest, refresh, push, block norms) runs on the graph's stream.  The
+// decide part (dirty test, rebuild of the dirty blocks) must follow the
+// previous back part: it reads the residuals that part writes and rewrites
+// the blocks it factors.
+struct StepPlan {
+    int32_t it = 0;
+    bool skipped = false;                  // only ingested (the lazy_start rule)
+    std::vector<int32_t> rebuild, factor;  // dirty blocks, and those with a nonzero matrix
+    std::vector<double> a2;                // ||A_new||^2 per block at this step
+    tsvd_snapshot_stats S{};
+    double t0 = 0.0;
+};
+bool pipeline_front(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, int64_t m, int on_device,
+                    StepPlan &plan);
+void pipeline_decide(tsvd_pipeline *p, StepPlan &plan);
+void pipeline_back(tsvd_pipeline *p, StepPlan &plan, hipStream_t st);
+}  // namespace tsvd
+
 struct tsvd_pipeline {
     tsvd_config cfg{};
     std::vector<int32_t> sources;

@@ -76,7 +76,8 @@ class SnapshotStats(ctypes.Structure):
 
 
 class ServerConfig(ctypes.Structure):
-    _fields_ = [("max_step_edges", ctypes.c_int64), ("max_step_requests", ctypes.c_int32)]
+    _fields_ = [("max_step_edges", ctypes.c_int64), ("max_step_requests", ctypes.c_int32),
+                ("no_overlap", ctypes.c_int32)]
 
 
 class RequestResult(ctypes.Structure):
@@ -565,13 +566,13 @@ class Server:
     def __init__(self, n, directed, sources, alpha, rmax, nparts, fanout, rank, q=6, precision=TSVD_FP32,
                  transform=TSVD_TRANSFORM_NC, delta=0.65, dyn_start=2, lazy_start=2, slack=0.1, seed=0,
                  max_step_edges=0, max_step_requests=0, two_sided=False, right_embedding=False,
-                 sparse_state=False, sparse_groups=0):
+                 sparse_state=False, sparse_groups=0, overlap=True):
         self.sources = np.ascontiguousarray(sources, dtype=np.int32)
         self.nsrc, self.rank = self.sources.size, rank
         self.cfg = Config(n, int(directed), self.nsrc, self.sources.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                           alpha, rmax, precision, nparts, fanout, rank, q, transform, delta, dyn_start, lazy_start,
                           slack, seed, int(two_sided), int(right_embedding), int(sparse_state), int(sparse_groups))
-        self.scfg = ServerConfig(int(max_step_edges), int(max_step_requests))
+        self.scfg = ServerConfig(int(max_step_edges), int(max_step_requests), int(not overlap))
         h = ctypes.c_void_p()
         check(lib().tsvd_server_create(ctypes.byref(self.cfg), ctypes.byref(self.scfg), ctypes.byref(h)))
         self.h = h
