@@ -239,7 +239,7 @@ def accuracy_report(pipe, w, seed=7):
             "seconds": {"oracle": round(orac.seconds, 1), "reference_algorithm": round(t_ref, 1)}}
 
 
-def served_run(capi, w, reqs, arrivals, cap, label, seed=7):
+def served_run(capi, w, reqs, arrivals, cap, label, seed=7, overlap=True):
     """Replays a request stream against a server on its own clock: a feeder
     thread submits request i at arrivals[i] seconds, this thread collects the
     results in order.  cap = 1 is the plain per-snapshot method (steps one at
@@ -250,7 +250,7 @@ def served_run(capi, w, reqs, arrivals, cap, label, seed=7):
                       precision=capi.TSVD_FP64 if w.fp64_state else capi.TSVD_FP32,
                       transform=capi.TSVD_TRANSFORM_NC if w.transform == "NC" else capi.TSVD_TRANSFORM_LP,
                       delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed, max_step_requests=cap,
-                      sparse_state=w.sparse_state, overlap=cap != 1)
+                      sparse_state=w.sparse_state, overlap=overlap and cap != 1)
     t = time.perf_counter()
     srv.wait(srv.submit(*w.snapshots[0]), embedding=False)
     log(f"{label}: initial snapshot {time.perf_counter() - t:.1f}s")
@@ -307,6 +307,8 @@ def main():
     ap.add_argument("--served-requests", type=int, default=48,
                     help="requests in the served-vs-plain stream (0: skip that comparison)")
     ap.add_argument("--served-rate", type=float, default=12.0, help="Poisson arrival rate of that stream (1/s)")
+    ap.add_argument("--served-no-overlap", action="store_true",
+                    help="run the served form's steps one at a time (no back/front overlap)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="first run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -432,12 +434,13 @@ def main():
         log(f"served vs plain: {len(reqs)} requests ({sum(s.size for s, _ in reqs)} edges, lognormal median 10k) "
             f"at {args.served_rate}/s Poisson")
         plain = served_run(capi, w, reqs, arr, 1, "plain (one snapshot per request)")
-        served = served_run(capi, w, reqs, arr, 0, "served (continuous batching)")
+        served = served_run(capi, w, reqs, arr, 0, "served (continuous batching)",
+                            overlap=not args.served_no_overlap)
         out["served_vs_plain"] = {
             "stream": {"requests": len(reqs), "edges": int(sum(s.size for s, _ in reqs)),
                        "rate_per_s": args.served_rate, "sizes": "lognormal median 10k, sigma 1, clip [1k, 1M]",
                        "graph": f"{args.workload} workload after its initial snapshot"},
-            "plain": plain, "served": served,
+            "plain": plain, "served": served, "served_overlap": not args.served_no_overlap,
             "throughput_ratio": round(served["snapshots_per_s"] / plain["snapshots_per_s"], 2),
             "p99_ratio": round(served["p99_ms"] / plain["p99_ms"], 4)}
     # kernel events cover every launch of this run, as a rocprofv3 trace of the
