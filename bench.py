@@ -48,6 +48,35 @@ def pipeline_for(capi, w, seed=7):
                          sparse_state=w.sparse_state)
 
 
+def atscale_workload(capi, gen, scale_down=0):
+    """Served at scale: directed R-MAT (0.57, 0.19, 0.19, 0.05) over 2^26 ids
+    taken modulo 41,652,230, seed 4, 1,468,365,182 distinct edges without
+    self-loops, drawn in HBM (tsvd_gen_rmat: host generation would outlast
+    the run); 90% initial, then LogNormal(ln 100k, 1) batches; 3000 sources
+    drawn from the initial snapshot's edge sources; rmax 1e-7; 4096 blocks,
+    rank 128, fan-out 4; LP transform (the directed LP driver).  Snapshots are
+    device slices of the generated arrays; returns (workload, owned arrays)."""
+    sd = 1 << scale_down
+    n, m = 41_652_230 // sd, 1_468_365_182 // sd
+    scale = int(np.ceil(np.log2(n)))
+    src, dst = capi.gen_rmat(scale, n, m, 0.57, 0.19, 0.19, seed=4)
+    n0 = int(round(0.9 * m))
+    snaps = [(capi.DeviceSlice(src, 0, n0), capi.DeviceSlice(dst, 0, n0))]
+    pos = n0
+    for b in gen.lognormal_sizes(60, 100_000 / sd, 1.0, 4):
+        b = int(min(b, m - pos))
+        if b <= 0:
+            break
+        snaps.append((capi.DeviceSlice(src, pos, b), capi.DeviceSlice(dst, pos, b)))
+        pos += b
+    # edge order is random, so the first 16M initial edges sample its sources uniformly
+    head = capi.DeviceSlice(src, 0, min(n0, 1 << 24)).numpy()
+    sub = gen.sample_subset(head, 3000, 4)
+    w = gen.Workload("atscale", n, True, snaps, sub, 0.2, 1e-7, max(16, 4096 // sd), 4, 128, 6, False,
+                     transform="LP")
+    return w, (src, dst)
+
+
 def roofline_of(prof, precision_fp64):
     """Dominant kernel of this library by total time (library solver calls,
     which carry no algorithmic count, are skipped); HBM-bound kernels report
@@ -252,7 +281,11 @@ def served_run(capi, w, reqs, arrivals, cap, label, seed=7, overlap=True):
                       delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed, max_step_requests=cap,
                       sparse_state=w.sparse_state, overlap=overlap and cap != 1)
     t = time.perf_counter()
-    srv.wait(srv.submit(*w.snapshots[0]), embedding=False)
+    s0, d0 = w.snapshots[0]
+    if isinstance(s0, capi.DeviceSlice):
+        srv.pipeline.snapshot(s0, d0)     # device-resident initial graph; the worker is idle until a submit
+    else:
+        srv.wait(srv.submit(s0, d0), embedding=False)
     log(f"{label}: initial snapshot {time.perf_counter() - t:.1f}s")
     tickets = [None] * len(reqs)
     ready = [threading.Event() for _ in reqs]
@@ -292,7 +325,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="batched", choices=["plain", "batched", "precision", "served"])
+    ap.add_argument("--workload", default="batched", choices=["plain", "batched", "precision", "served", "atscale"])
     ap.add_argument("--transform", choices=["NC", "LP"], help="override the workload's block transform")
     ap.add_argument("--shard", default=None, metavar="I/N",
                     help="run GPU I's share of an N-GPU source-sharded job: the I-th of N contiguous "
@@ -319,7 +352,12 @@ def main():
 
     from treesvd_amd import capi, gen
     t = time.time()
-    w = gen.workload(args.workload, scale_down=args.scale_down)
+    owned = None
+    if args.workload == "atscale":
+        w, owned = atscale_workload(capi, gen, args.scale_down)
+        args.no_cpu_baseline = True   # the CPU leg would need the 1.47e9-edge graph on the host
+    else:
+        w = gen.workload(args.workload, scale_down=args.scale_down)
     if args.transform:
         w.transform = args.transform
     shard = None
@@ -350,8 +388,10 @@ def main():
         log(f"calibration copy of {args.calibrate_copy} MiB done")
         a.free(); b.free()
     pipe = pipeline_for(capi, w)
-    batches = [(capi.DeviceArray(s.astype(np.int32)), capi.DeviceArray(d.astype(np.int32)))
-               for s, d in w.snapshots[:1 + need]]
+    def on_device(x):
+        return x if isinstance(x, capi.DeviceSlice) else capi.DeviceArray(x.astype(np.int32))
+
+    batches = [(on_device(s), on_device(d)) for s, d in w.snapshots[:1 + need]]
     # kernel events cover every launch of the run (as a rocprofv3 trace of this
     # command does); the snapshot rate comes from the timed steps only
     capi.profile_reset()
@@ -430,15 +470,23 @@ def main():
         for x in b:
             x.free()
     if args.served_requests > 0:
-        reqs, arr = gen.request_stream(w, args.served_requests, args.served_rate, seed=11)
-        log(f"served vs plain: {len(reqs)} requests ({sum(s.size for s, _ in reqs)} edges, lognormal median 10k) "
-            f"at {args.served_rate}/s Poisson")
+        if owned is not None:   # the stream's batches on the host; the initial snapshot stays on the device
+            import dataclasses
+            w = dataclasses.replace(w, snapshots=[w.snapshots[0]] + [(s.numpy(), d.numpy())
+                                                                     for s, d in w.snapshots[1:]])
+        # request sizes of the workload's stream: median 100k at the 1.47e9-edge shape, 10k otherwise
+        med = 100_000 if args.workload == "atscale" else 10_000
+        reqs, arr = gen.request_stream(w, args.served_requests, args.served_rate, seed=11,
+                                       median=med / (1 << args.scale_down))
+        log(f"served vs plain: {len(reqs)} requests ({sum(s.size for s, _ in reqs)} edges, lognormal median "
+            f"{med / (1 << args.scale_down):.0f}) at {args.served_rate}/s Poisson")
         plain = served_run(capi, w, reqs, arr, 1, "plain (one snapshot per request)")
         served = served_run(capi, w, reqs, arr, 0, "served (continuous batching)",
                             overlap=not args.served_no_overlap)
         out["served_vs_plain"] = {
             "stream": {"requests": len(reqs), "edges": int(sum(s.size for s, _ in reqs)),
-                       "rate_per_s": args.served_rate, "sizes": "lognormal median 10k, sigma 1, clip [1k, 1M]",
+                       "rate_per_s": args.served_rate,
+                       "sizes": f"lognormal median {med // 1000}k, sigma 1, clip [1k, 1M]",
                        "graph": f"{args.workload} workload after its initial snapshot"},
             "plain": plain, "served": served, "served_overlap": not args.served_no_overlap,
             "throughput_ratio": round(served["snapshots_per_s"] / plain["snapshots_per_s"], 2),

@@ -258,6 +258,15 @@ int tsvd_pipeline_gram_apply(tsvd_pipeline *p, const double *Y, int32_t l, doubl
  * (the reference's tripletList / residue_pair_list of that source).        */
 int tsvd_pipeline_source_state(tsvd_pipeline *p, int32_t dir, int32_t i, double *reserve, double *residue);
 
+/* Seeded synthetic directed R-MAT graph drawn in HBM (no reference
+ * counterpart; for the measured configurations, whose host generation would
+ * outlast the run).  Quadrant probabilities a, b, c, 1-a-b-c per level over
+ * `scale` levels, ids modulo n, no self-loops, m distinct edges in a seeded
+ * random order.  *src and *dst are device arrays of m ids, released with
+ * tsvd_device_free. */
+int tsvd_gen_rmat(int32_t scale, int32_t n, int64_t m, double a, double b, double c, uint64_t seed, int32_t **src,
+                  int32_t **dst);
+
 /* ---------------------------------------------------------- served form
  * No reference counterpart: the reference's drivers run the whole update once
  * per snapshot file (Tree_u_NC.cpp:507-866).  A server owns one pipeline and a

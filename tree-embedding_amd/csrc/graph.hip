@@ -295,6 +295,14 @@ void graph_append(tsvd_graph *g, const int32_t *src, const int32_t *dst, int64_t
     g->m += m;
     g->version++;
     TSVD_HIP(hipStreamSynchronize(st));
+    if (m > ((int64_t)1 << 26)) {
+        // an initial load of a large graph: its staging (keys, values, sort
+        // and scan buffers, about 40 B per edge) is not kept for the small
+        // batches that follow
+        g->k_in.release(); g->v_in.release(); g->k_out.release(); g->v_out.release();
+        g->scan_flag.release(); g->scan_pos.release(); g->cub_tmp.release();
+        g->e_src.release(); g->e_dst.release();
+    }
 }
 
 }  // namespace tsvd

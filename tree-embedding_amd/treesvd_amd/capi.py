@@ -33,7 +33,7 @@ EXPORTS = [
     "tsvd_pipeline_dense_gram", "tsvd_pipeline_gram_apply", "tsvd_pipeline_right_embedding",
     "tsvd_pipeline_lp_scores", "tsvd_pipeline_source_state",
     "tsvd_server_create", "tsvd_server_destroy", "tsvd_server_submit", "tsvd_server_wait",
-    "tsvd_server_pipeline", "tsvd_server_edge_budget", "tsvd_probe_atomic_rate",
+    "tsvd_server_pipeline", "tsvd_server_edge_budget", "tsvd_probe_atomic_rate", "tsvd_gen_rmat",
 ]
 
 
@@ -165,6 +165,8 @@ def lib():
             L.tsvd_server_edge_budget.restype = i64
             L.tsvd_server_edge_budget.argtypes = [vp]
             L.tsvd_probe_atomic_rate.argtypes = [i64, i32, vp]
+            L.tsvd_gen_rmat.argtypes = [i32, i32, i64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_uint64, vp, vp]
         _lib = L
     return _lib
 
@@ -191,6 +193,15 @@ class DeviceArray:
         if host.nbytes:
             check(lib().tsvd_memcpy(self.ptr, ctypes.c_void_p(host.ctypes.data), host.nbytes, 0))
 
+    @classmethod
+    def adopt(cls, ptr, dtype, shape):
+        """Take ownership of a device allocation made by the library."""
+        self = cls.__new__(cls)
+        self.dtype, self.shape = np.dtype(dtype), tuple(shape)
+        self.nbytes = max(int(np.prod(shape)) * self.dtype.itemsize, 1)
+        self.ptr = ptr
+        return self
+
     def numpy(self):
         out = np.empty(self.shape, self.dtype)
         if out.nbytes:
@@ -207,6 +218,34 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class DeviceSlice:
+    """A view [offset, offset+count) of a device array of int32 (no ownership)."""
+
+    def __init__(self, base, offset, count):
+        self.base, self.offset, self.count = base, int(offset), int(count)
+        self.ptr = ctypes.c_void_p(base.ptr.value + 4 * self.offset)
+        self.shape, self.size, self.dtype = (self.count,), self.count, np.dtype(np.int32)
+
+    def numpy(self):
+        out = np.empty(self.count, np.int32)
+        if self.count:
+            check(lib().tsvd_memcpy(ctypes.c_void_p(out.ctypes.data), self.ptr, 4 * self.count, 1))
+        return out
+
+    def free(self):
+        pass
+
+
+def gen_rmat(scale, n, m, a=0.57, b=0.19, c=0.19, seed=0):
+    """Seeded directed R-MAT edges drawn in HBM (tsvd_gen_rmat): m distinct
+    (src, dst) pairs, ids mod n, no self-loops, in a seeded random order.
+    Returns two DeviceArray objects of m int32 each."""
+    ps, pd = ctypes.c_void_p(), ctypes.c_void_p()
+    check(lib().tsvd_gen_rmat(int(scale), int(n), int(m), float(a), float(b), float(c), ctypes.c_uint64(seed),
+                              ctypes.byref(ps), ctypes.byref(pd)))
+    return DeviceArray.adopt(ps, np.int32, (int(m),)), DeviceArray.adopt(pd, np.int32, (int(m),))
 
 
 def _dptr(t):
