@@ -189,6 +189,7 @@ static void append_dir(tsvd_graph *g, int d, const int32_t *dsrc, const int32_t 
     g->scan_flag.ensure((size_t)ne);
     g->scan_pos.ensure((size_t)ne);
 
+    D.appended = ne;
     // former <- degree (Graph_dynamic.h:545-547 / :135-138)
     TSVD_HIP(hipMemcpyAsync(D.former.p, D.deg.p, D.deg.bytes(), hipMemcpyDeviceToDevice, st));
     TSVD_HIP(hipMemsetAsync(D.cnt.p, 0, D.cnt.bytes(), st));

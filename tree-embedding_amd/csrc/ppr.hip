@@ -740,14 +740,8 @@ void refresh_t(tsvd_ppr *s, int mode, int d, int32_t s0, int32_t s1)
             const int64_t waves = (int64_t)D.ntouched * s->ngroups;
             // algorithmic bytes per (new slot, source): p RMW, r[from] and r[to] RMW,
             // and the matrix entry of the reference (8 B each in FP32 terms)
-            int64_t slots = 0;
-            if (g_profile) {
-                std::vector<int32_t> c((size_t)g->n);
-                TSVD_HIP(hipMemcpyAsync(c.data(), D.cnt.p, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, st));
-                TSVD_HIP(hipStreamSynchronize(st));
-                for (int32_t v = 0; v < g->n; v++) slots += c[v];
-                prof_add_work("ppr_update", 4.0 * sizeof(T) * 2.0 * (double)slots * (s1 - s0), 0.0);
-            }
+            if (g_profile)
+                prof_add_work("ppr_update", 4.0 * sizeof(T) * 2.0 * (double)D.appended * (s1 - s0), 0.0);
             PROF("ppr_update", st);
             hipLaunchKernelGGL(k_update<T>, dim3((unsigned)ceil_div(waves, WPB)), dim3(TPB), 0, st, P, R,
                                D.touched.p, D.ntouched, D.rowoff.p, D.adj.p, D.deg.p, D.former.p, s->s_pad,

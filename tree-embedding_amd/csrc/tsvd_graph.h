@@ -12,6 +12,7 @@ struct GraphDir {
     DBuf<int32_t> touched;  // rows grown by the last append, ascending
     DBuf<int32_t> adj;      // neighbour ids
     int64_t cap_total = 0;
+    int64_t appended = 0;   // entries added by the last append (new slots)
     int32_t ntouched = 0;
 };
 
