@@ -13,3 +13,18 @@ def test_more_than_one_gpu_is_refused_with_a_message():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in msg["error"]
+
+
+def run_bench(*args):
+    return subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *args], capture_output=True, text=True,
+                          timeout=300)
+
+
+def test_sparse_state_needs_a_directed_workload():
+    r = run_bench("--workload", "plain", "--scale-down", "6", "--sparse-state")
+    assert r.returncode != 0 and "directed LP" in r.stderr
+
+
+def test_shard_index_is_checked():
+    r = run_bench("--workload", "plain", "--scale-down", "6", "--shard", "3/2")
+    assert r.returncode != 0 and "0 <= I < N" in r.stderr
