@@ -38,8 +38,9 @@ __device__ inline double wave_sum(double v)
 template <class T>
 __global__ void __launch_bounds__(TPB) k_block_norms(const T *__restrict__ p0, const T *__restrict__ p1,
                                                      const T *__restrict__ A, const int32_t *__restrict__ slot,
-                                                     int32_t n, int32_t nsrc, int32_t s_pad, int32_t nparts,
-                                                     int32_t common, int transform, T rm, double *__restrict__ norms)
+                                                     const int32_t *__restrict__ row_of, int32_t n, int32_t nsrc,
+                                                     int32_t s_pad, int32_t nparts, int32_t common, int transform,
+                                                     T rm, double *__restrict__ norms)
 {
     const int32_t j = blockIdx.y;
     const int64_t start = (int64_t)j * common;
@@ -59,7 +60,7 @@ __global__ void __launch_bounds__(TPB) k_block_norms(const T *__restrict__ p0, c
             T x = x0[s];
             if (x1) x += x1[s];
             const T a = xform<T>(x, rm, transform);
-            const T ao = Ao ? Ao[s] : T(0);
+            const T ao = Ao ? Ao[row_of[s]] : T(0);
             a2 += (double)a * (double)a;
             const double d = (double)a - (double)ao;
             e2 += d * d;
@@ -129,7 +130,8 @@ template <class T>
 __global__ void __launch_bounds__(TPB) k_gather(const int32_t *__restrict__ blocks, const T *__restrict__ p0,
                                                 const T *__restrict__ p1, T *__restrict__ A,
                                                 const int32_t *__restrict__ colmap, const int32_t *__restrict__ ncols,
-                                                int32_t nsrc, int32_t s_pad, int32_t common, int transform, T rm)
+                                                const int32_t *__restrict__ row_of, int32_t nsrc, int32_t s_pad,
+                                                int32_t common, int transform, T rm)
 {
     const int32_t j = blocks[blockIdx.y];
     const int64_t start = (int64_t)j * common;
@@ -147,7 +149,7 @@ __global__ void __launch_bounds__(TPB) k_gather(const int32_t *__restrict__ bloc
             if (x1) x += x1[s];
             a = xform<T>(x, rm, transform);
         }
-        col[s] = a;
+        col[row_of[s]] = a;
     }
 }
 
@@ -255,6 +257,13 @@ void blocks_setup(Blocks &b, int32_t n, int32_t nparts, int32_t nsrc, int32_t s_
     b.is_source.alloc((size_t)n);
     b.norms.alloc((size_t)2 * nparts);
     b.d_dirty.alloc((size_t)nparts);
+    b.row_of.alloc((size_t)s_pad);
+    {
+        std::vector<int32_t> id((size_t)s_pad);
+        for (int32_t s = 0; s < s_pad; s++) id[(size_t)s] = s;
+        TSVD_HIP(hipMemcpyAsync(b.row_of.p, id.data(), sizeof(int32_t) * s_pad, hipMemcpyHostToDevice, st));
+        TSVD_HIP(hipStreamSynchronize(st));
+    }
     TSVD_HIP(hipMemsetAsync(b.A.p, 0, b.A.bytes(), st));
     TSVD_HIP(hipMemsetAsync(b.slot.p, 0xFF, b.slot.bytes(), st));   // -1: no stored column
     TSVD_HIP(hipMemsetAsync(b.d_ncols.p, 0, b.d_ncols.bytes(), st));
@@ -284,12 +293,12 @@ void blocks_norms(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g
     PROF("blocks_norms", st);
     if (b.precision == TSVD_FP64)
         hipLaunchKernelGGL(k_block_norms<double>, grid, dim3(TPB), 0, st, (const double *)p0, (const double *)p1,
-                           (const double *)b.A.p, b.slot.p, b.n, b.nsrc, b.s_pad, b.nparts, b.common, b.transform,
-                           b.rmax, b.norms.p);
+                           (const double *)b.A.p, b.slot.p, b.row_of.p, b.n, b.nsrc, b.s_pad, b.nparts, b.common,
+                           b.transform, b.rmax, b.norms.p);
     else
         hipLaunchKernelGGL(k_block_norms<float>, grid, dim3(TPB), 0, st, (const float *)p0, (const float *)p1,
-                           (const float *)b.A.p, b.slot.p, b.n, b.nsrc, b.s_pad, b.nparts, b.common, b.transform,
-                           (float)b.rmax, b.norms.p);
+                           (const float *)b.A.p, b.slot.p, b.row_of.p, b.n, b.nsrc, b.s_pad, b.nparts, b.common,
+                           b.transform, (float)b.rmax, b.norms.p);
     TSVD_HIP(hipGetLastError());
     std::vector<double> h((size_t)2 * b.nparts);
     TSVD_HIP(hipMemcpyAsync(h.data(), b.norms.p, b.norms.bytes(), hipMemcpyDeviceToHost, st));
@@ -318,12 +327,12 @@ void blocks_rebuild(Blocks &b, const std::vector<int32_t> &dirty, const void *p0
     }
     if (b.precision == TSVD_FP64)
         hipLaunchKernelGGL(k_gather<double>, grid, dim3(TPB), 0, st, b.d_dirty.p, (const double *)p0,
-                           (const double *)p1, (double *)b.A.p, b.colmap.p, b.d_ncols.p, b.nsrc, b.s_pad, b.common,
-                           b.transform, b.rmax);
+                           (const double *)p1, (double *)b.A.p, b.colmap.p, b.d_ncols.p, b.row_of.p, b.nsrc,
+                           b.s_pad, b.common, b.transform, b.rmax);
     else
         hipLaunchKernelGGL(k_gather<float>, grid, dim3(TPB), 0, st, b.d_dirty.p, (const float *)p0,
-                           (const float *)p1, (float *)b.A.p, b.colmap.p, b.d_ncols.p, b.nsrc, b.s_pad, b.common,
-                           b.transform, (float)b.rmax);
+                           (const float *)p1, (float *)b.A.p, b.colmap.p, b.d_ncols.p, b.row_of.p, b.nsrc,
+                           b.s_pad, b.common, b.transform, (float)b.rmax);
     TSVD_HIP(hipGetLastError());
     TSVD_HIP(hipMemcpyAsync(b.ncols.data(), b.d_ncols.p, sizeof(int32_t) * b.nparts, hipMemcpyDeviceToHost, st));
     TSVD_HIP(hipStreamSynchronize(st));
@@ -574,7 +583,8 @@ namespace {
 // padding rows zero: a vertex-major slab of the full |S| x n matrix.
 template <class T>
 __global__ void k_transform_chunk(const T *__restrict__ p0, const T *__restrict__ p1, double *__restrict__ X,
-                                  int64_t v0, int64_t nv, int32_t nsrc, int32_t s_pad, int transform, T rm)
+                                  const int32_t *__restrict__ row_of, int64_t v0, int64_t nv, int32_t nsrc,
+                                  int32_t s_pad, int transform, T rm)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nv * s_pad) return;
@@ -587,7 +597,7 @@ __global__ void k_transform_chunk(const T *__restrict__ p0, const T *__restrict_
         if (p1) x += p1[at];
         a = (double)xform<T>(x, rm, transform);
     }
-    X[t] = a;
+    X[vv * s_pad + row_of[s]] = a;
 }
 
 }  // namespace
@@ -599,11 +609,12 @@ void transform_chunk(const Blocks &b, const void *p0, const void *p1, double *X,
     if (tot <= 0) return;
     if (b.precision == TSVD_FP64)
         hipLaunchKernelGGL(k_transform_chunk<double>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st,
-                           (const double *)p0, (const double *)p1, X, v0, nv, b.nsrc, b.s_pad, b.transform, b.rmax);
+                           (const double *)p0, (const double *)p1, X, b.row_of.p, v0, nv, b.nsrc, b.s_pad,
+                           b.transform, b.rmax);
     else
         hipLaunchKernelGGL(k_transform_chunk<float>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st,
-                           (const float *)p0, (const float *)p1, X, v0, nv, b.nsrc, b.s_pad, b.transform,
-                           (float)b.rmax);
+                           (const float *)p0, (const float *)p1, X, b.row_of.p, v0, nv, b.nsrc, b.s_pad,
+                           b.transform, (float)b.rmax);
     TSVD_HIP(hipGetLastError());
 }
 

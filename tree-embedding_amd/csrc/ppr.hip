@@ -1132,12 +1132,14 @@ extern "C" int tsvd_ppr_read(tsvd_ppr *s, int dir, int what, double *out)
         const void *src = what == 0 ? s->p(dir) : s->r(dir);
         TSVD_HIP(hipMemcpyAsync(h.data(), src, h.size(), hipMemcpyDeviceToHost, s->g->stream));
         TSVD_HIP(hipStreamSynchronize(s->g->stream));
-        for (int32_t k = 0; k < s->nsrc; k++)
+        for (int32_t k = 0; k < s->nsrc; k++) {
+            const size_t row = s->out_row.empty() ? (size_t)k : (size_t)s->out_row[(size_t)k];
             for (int32_t v = 0; v < s->n; v++) {
                 const size_t c = (size_t)v * s->s_pad + k;
-                out[(size_t)k * s->n + v] =
+                out[row * s->n + v] =
                     s->esize == 8 ? ((const double *)h.data())[c] : (double)((const float *)h.data())[c];
             }
+        }
     });
 }
 

@@ -46,6 +46,7 @@ struct Blocks {
     DBuf<uint8_t> active, is_source;        // n
     DBuf<double> norms;                     // 2 * nparts: ||A_new||^2, ||A_new - A_old||^2
     DBuf<int32_t> d_dirty;                  // dirty block ids
+    DBuf<int32_t> row_of;                   // s_pad: matrix row of push lane s (a permutation of the sources)
     std::vector<double> a2, e2, resid;
 };
 
@@ -203,6 +204,8 @@ struct tsvd_pipeline {
     tsvd::GemmRunner gemm;
     tsvd::Eig *eig = nullptr;
     tsvd::SparseState sp;                    // sparse_state pipelines only
+    bool order_set = false;                  // push lanes ordered (at the first static snapshot)
+    std::vector<int32_t> lane_of;            // subset row -> push lane (empty: the same)
     int32_t snapshot = 0;
     std::vector<int32_t> last_dirty;
     ~tsvd_pipeline();

@@ -44,6 +44,7 @@ struct tsvd_ppr {
     double pull_frac = 0.0;                  // pull when items > pull_frac * n (0: never)
     double ride_frac = 0.5;                  // sub-threshold lanes above ride_frac*rmax ride along (>= 1: off)
     std::vector<int32_t> h_sources;
+    std::vector<int32_t> out_row;            // tsvd_ppr_read row of lane s (empty: row s)
     tsvd::DBuf<int32_t> first[2];            // edge-order update: first new edge of a row (made on first use)
     ~tsvd_ppr()
     {
