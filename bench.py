@@ -268,7 +268,7 @@ def accuracy_report(pipe, w, seed=7):
             "seconds": {"oracle": round(orac.seconds, 1), "reference_algorithm": round(t_ref, 1)}}
 
 
-def served_run(capi, w, reqs, arrivals, cap, label, seed=7, overlap=True):
+def served_run(capi, w, reqs, arrivals, cap, label, seed=7, overlap=True, push_setting=None):
     """Replays a request stream against a server on its own clock: a feeder
     thread submits request i at arrivals[i] seconds, this thread collects the
     results in order.  cap = 1 is the plain per-snapshot method (steps one at
@@ -281,6 +281,8 @@ def served_run(capi, w, reqs, arrivals, cap, label, seed=7, overlap=True):
                       delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed, max_step_requests=cap,
                       sparse_state=w.sparse_state, overlap=overlap and cap != 1)
     t = time.perf_counter()
+    if push_setting is not None:          # the plain run's autotuned push setting, for both replays
+        srv.pipeline.ppr.push_setting = push_setting
     s0, d0 = w.snapshots[0]
     if isinstance(s0, capi.DeviceSlice):
         srv.pipeline.snapshot(s0, d0)     # device-resident initial graph; the worker is idle until a submit
@@ -340,6 +342,8 @@ def main():
     ap.add_argument("--served-requests", type=int, default=48,
                     help="requests in the served-vs-plain stream (0: skip that comparison)")
     ap.add_argument("--served-rate", type=float, default=12.0, help="Poisson arrival rate of that stream (1/s)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the push autotuner on the first warm-up snapshot")
     ap.add_argument("--served-no-overlap", action="store_true",
                     help="run the served form's steps one at a time (no back/front overlap)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -405,8 +409,19 @@ def main():
         log(f"pair-list state: {st0['state_entries']:.4e} entries ({st0['state_entries'] * 8 / 1e9:.2f} GB), "
             f"{st0['state_batches']} source batches")
     warm = []
+    tuning = None
     for i in range(args.warmup):
-        st = pipe.snapshot(*batches[1 + i])
+        if i == 0 and not args.no_tune:
+            # the push autotuner: this snapshot's push timed under each setting from the same state
+            cands = [(h, r) for h in (2048, 4096, 8192) for r in (0.25, 0.5, 1.0)]
+            st, chosen, ms = pipe.snapshot_tuned(*batches[1 + i], cands)
+            tuning = {"candidates": [{"hub_chunk": h, "ride_frac": r, "push_ms": round(float(t), 1)}
+                                     for (h, r), t in zip(cands, ms)],
+                      "chosen": {"hub_chunk": chosen[0], "ride_frac": chosen[1]}}
+            log(f"push autotuner: chose hub_chunk {chosen[0]}, ride_frac {chosen[1]} "
+                f"({', '.join(f'{h}/{r}: {t:.0f}' for (h, r), t in zip(cands, ms))} ms)")
+        else:
+            st = pipe.snapshot(*batches[1 + i])
         warm.append(st)
         log(f"warmup {i}: {st['ms_total']:.1f} ms dirty={st['ndirty']}")
 
@@ -457,6 +472,7 @@ def main():
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "initial_snapshot_ms": round(init_ms, 1), "phase_ms": phases,
         "dirty_blocks_per_step": ndirty, "merged_nodes_per_step": nmerged, "pair_list_state": state,
+        "push_tuning": tuning,
         "push_atomic_roofline": {"achieved": round(achieved_rate / 1e9, 3), "ceiling": round(ceiling / 1e9, 3),
                                  "unit": "G wave-atomics/s", "frac": round(achieved_rate / ceiling, 4),
                                  "note": "plain-method push (round + hub kernels); ceiling = returning 64-lane "
@@ -480,9 +496,10 @@ def main():
                                        median=med / (1 << args.scale_down))
         log(f"served vs plain: {len(reqs)} requests ({sum(s.size for s, _ in reqs)} edges, lognormal median "
             f"{med / (1 << args.scale_down):.0f}) at {args.served_rate}/s Poisson")
-        plain = served_run(capi, w, reqs, arr, 1, "plain (one snapshot per request)")
+        ps = (tuning["chosen"]["hub_chunk"], tuning["chosen"]["ride_frac"]) if tuning else None
+        plain = served_run(capi, w, reqs, arr, 1, "plain (one snapshot per request)", push_setting=ps)
         served = served_run(capi, w, reqs, arr, 0, "served (continuous batching)",
-                            overlap=not args.served_no_overlap)
+                            overlap=not args.served_no_overlap, push_setting=ps)
         out["served_vs_plain"] = {
             "stream": {"requests": len(reqs), "edges": int(sum(s.size for s, _ in reqs)),
                        "rate_per_s": args.served_rate,

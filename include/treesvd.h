@@ -218,6 +218,26 @@ int tsvd_pipeline_root_sigma(tsvd_pipeline *p, double *sigma);
 /* Handles owned by the pipeline (valid until it is destroyed). */
 tsvd_graph *tsvd_pipeline_graph(tsvd_pipeline *p);
 tsvd_ppr *tsvd_pipeline_ppr(tsvd_pipeline *p);
+/* The project's autotuner for the push's launch shape (no reference
+ * counterpart).  Setting: rows longer than hub_chunk neighbours are split
+ * across wavefronts; lanes whose residue is above ride_frac * rmax (but not
+ * above rmax) push along with a popped item (>= 1: off).  A tuned snapshot
+ * is tsvd_pipeline_snapshot, except that its push is first timed under each
+ * of the ncand candidates from the same saved state (p, r and the pending
+ * lists), then run for real with the fastest, which stays for later
+ * snapshots.  cand_ms (may be NULL) gets each candidate's push time.  A
+ * snapshot that cannot be timed so (static, sparse_state, or too little free
+ * HBM to save the state) runs untuned: settings unchanged, cand_ms -1. */
+typedef struct {
+    int32_t hub_chunk;
+    double ride_frac;
+} tsvd_push_setting;
+/* Read or set the push setting of a push state (e.g. tsvd_pipeline_ppr's). */
+int tsvd_ppr_get_push_setting(const tsvd_ppr *s, tsvd_push_setting *out);
+int tsvd_ppr_set_push_setting(tsvd_ppr *s, const tsvd_push_setting *in);
+int tsvd_pipeline_snapshot_tuned(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, int64_t m, int on_device,
+                                 const tsvd_push_setting *cand, int32_t ncand, tsvd_push_setting *chosen,
+                                 double *cand_ms, tsvd_snapshot_stats *stats);
 /* Host copy of leaf block j's factored matrix (nsrc x width, col-major FP64)
  * and of the per-block norms ||A||, ||A_new - A_old||, residual. */
 int tsvd_pipeline_block_matrix(tsvd_pipeline *p, int32_t j, double *A);

@@ -280,8 +280,42 @@ static void order_sources(tsvd_pipeline *p)
     TSVD_HIP(hipStreamSynchronize(st));
 }
 
+// Time this snapshot's push under each candidate setting from the same saved
+// state, keep the fastest, and put the state back for the real push.
+static void tune_push(tsvd_pipeline *p, const PushTune &t)
+{
+    tsvd_ppr *q = p->ppr;
+    for (int32_t i = 0; i < t.ncand; i++) {
+        if (t.cand_ms) t.cand_ms[i] = -1.0;
+        TSVD_REQUIRE(t.cand[i].hub_chunk >= 64 && t.cand[i].ride_frac >= 0.0, "bad push setting %d", i);
+    }
+    size_t fr = 0, tot = 0;
+    TSVD_HIP(hipMemGetInfo(&fr, &tot));
+    if (ppr_save_bytes(q) + ((size_t)2 << 30) > fr) return;   // no room to save the state: untuned
+    PushSave sv;
+    ppr_save(q, sv);
+    double best = 0.0;
+    tsvd_push_setting pick{q->hub_chunk, q->ride_frac};
+    for (int32_t i = 0; i < t.ncand; i++) {
+        ppr_restore(q, sv);
+        q->hub_chunk = t.cand[i].hub_chunk;
+        q->ride_frac = t.cand[i].ride_frac;
+        const double t0 = now_ms();
+        for (int d = 0; d < q->ndir; d++) ppr_push(q, d, 0, p->cfg.nsrc);
+        const double ms = now_ms() - t0;
+        if (t.cand_ms) t.cand_ms[i] = ms;
+        if (i == 0 || ms < best) {
+            best = ms;
+            pick = t.cand[i];
+        }
+    }
+    q->hub_chunk = pick.hub_chunk;
+    q->ride_frac = pick.ride_frac;
+    ppr_restore(q, sv);
+}
+
 bool pipeline_front(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, int64_t m, int on_device,
-                    StepPlan &plan)
+                    StepPlan &plan, const PushTune *tune)
 {
     const tsvd_config &c = p->cfg;
     hipStream_t st = p->g->stream;
@@ -309,6 +343,7 @@ bool pipeline_front(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, in
         const int mode = plan.it < c.dyn_start ? TSVD_REFRESH_INIT : TSVD_REFRESH_DYNAMIC;
         if (mode == TSVD_REFRESH_INIT && !p->order_set) order_sources(p);
         for (int d = 0; d < p->ppr->ndir; d++) ppr_refresh(p->ppr, mode, d, 0, c.nsrc);
+        if (tune && tune->ncand > 0 && mode == TSVD_REFRESH_DYNAMIC) tune_push(p, *tune);
         const double t2 = now_ms();
         S.ms_refresh = t2 - t1;
         for (int d = 0; d < p->ppr->ndir; d++) {
@@ -452,6 +487,30 @@ extern "C" int tsvd_pipeline_snapshot(tsvd_pipeline *p, const int32_t *src, cons
         StepPlan plan;
         if (pipeline_front(p, src, dst, m, on_device, plan)) pipeline_decide(p, plan);
         pipeline_back(p, plan, p->g->stream);
+        if (stats) *stats = plan.S;
+    });
+}
+
+extern "C" int tsvd_pipeline_snapshot_tuned(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, int64_t m,
+                                            int on_device, const tsvd_push_setting *cand, int32_t ncand,
+                                            tsvd_push_setting *chosen, double *cand_ms, tsvd_snapshot_stats *stats)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p, "null pipeline");
+        TSVD_REQUIRE(ncand >= 0 && (ncand == 0 || cand), "bad candidate list");
+        PushTune t;
+        t.cand = cand;
+        t.ncand = p->cfg.sparse_state ? 0 : ncand;
+        t.cand_ms = cand_ms;
+        for (int32_t i = 0; i < ncand && cand_ms; i++) cand_ms[i] = -1.0;
+        StepPlan plan;
+        if (pipeline_front(p, src, dst, m, on_device, plan, &t)) pipeline_decide(p, plan);
+        pipeline_back(p, plan, p->g->stream);
+        if (chosen) {
+            tsvd_ppr *q = p->ppr ? p->ppr : p->sp.gp;
+            chosen->hub_chunk = q->hub_chunk;
+            chosen->ride_frac = q->ride_frac;
+        }
         if (stats) *stats = plan.S;
     });
 }

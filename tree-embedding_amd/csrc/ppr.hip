@@ -981,6 +981,48 @@ void ppr_update_edges(tsvd_ppr *s, int dir, const int32_t *src, const int32_t *d
     else update_edges_t<float>(s, dir, src, dst, m, s0, s1);
 }
 
+void ppr_save(tsvd_ppr *s, PushSave &v)
+{
+    hipStream_t st = s->g->stream;
+    TSVD_HIP(hipMemcpyAsync(s->h_counters, s->counters.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    TSVD_HIP(hipStreamSynchronize(st));
+    for (int d = 0; d < s->ndir; d++) {
+        v.count[d] = s->h_counters[d];
+        v.P[d].alloc(s->P[d].bytes());
+        v.R[d].alloc(s->R[d].bytes());
+        v.cm[d].alloc(s->cm[d].n);
+        v.F[d].alloc(std::max<uint32_t>(v.count[d], 1));
+        TSVD_HIP(hipMemcpyAsync(v.P[d].p, s->P[d].p, s->P[d].bytes(), hipMemcpyDeviceToDevice, st));
+        TSVD_HIP(hipMemcpyAsync(v.R[d].p, s->R[d].p, s->R[d].bytes(), hipMemcpyDeviceToDevice, st));
+        TSVD_HIP(hipMemcpyAsync(v.cm[d].p, s->cm[d].p, s->cm[d].bytes(), hipMemcpyDeviceToDevice, st));
+        if (v.count[d])
+            TSVD_HIP(hipMemcpyAsync(v.F[d].p, s->F[d].p, sizeof(uint32_t) * v.count[d], hipMemcpyDeviceToDevice, st));
+    }
+    TSVD_HIP(hipStreamSynchronize(st));
+}
+
+void ppr_restore(tsvd_ppr *s, const PushSave &v)
+{
+    hipStream_t st = s->g->stream;
+    uint32_t cnt[2] = {v.count[0], v.count[1]};
+    for (int d = 0; d < s->ndir; d++) {
+        TSVD_HIP(hipMemcpyAsync(s->P[d].p, v.P[d].p, s->P[d].bytes(), hipMemcpyDeviceToDevice, st));
+        TSVD_HIP(hipMemcpyAsync(s->R[d].p, v.R[d].p, s->R[d].bytes(), hipMemcpyDeviceToDevice, st));
+        TSVD_HIP(hipMemcpyAsync(s->cm[d].p, v.cm[d].p, s->cm[d].bytes(), hipMemcpyDeviceToDevice, st));
+        if (v.count[d])
+            TSVD_HIP(hipMemcpyAsync(s->F[d].p, v.F[d].p, sizeof(uint32_t) * v.count[d], hipMemcpyDeviceToDevice, st));
+        TSVD_HIP(hipMemcpyAsync(s->counters.p + d, &cnt[d], sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    }
+    TSVD_HIP(hipStreamSynchronize(st));
+}
+
+size_t ppr_save_bytes(const tsvd_ppr *s)
+{
+    size_t b = 0;
+    for (int d = 0; d < s->ndir; d++) b += s->P[d].bytes() + s->R[d].bytes() + s->cm[d].bytes() + s->F[d].bytes();
+    return b;
+}
+
 void ppr_bind_sources(tsvd_ppr *s, const int32_t *sources, int32_t count)
 {
     TSVD_REQUIRE(count > 0 && count <= s->s_pad, "source count exceeds the state's width");
@@ -1111,6 +1153,25 @@ extern "C" int tsvd_ppr_refresh_edges(tsvd_ppr *s, int dir, const int32_t *src, 
             dd = bd.p;
         }
         ppr_update_edges(s, dir, ds, dd, m, src_begin, src_end);
+    });
+}
+
+extern "C" int tsvd_ppr_get_push_setting(const tsvd_ppr *s, tsvd_push_setting *out)
+{
+    return guard([&] {
+        TSVD_REQUIRE(s && out, "null argument");
+        out->hub_chunk = s->hub_chunk;
+        out->ride_frac = s->ride_frac;
+    });
+}
+
+extern "C" int tsvd_ppr_set_push_setting(tsvd_ppr *s, const tsvd_push_setting *in)
+{
+    return guard([&] {
+        TSVD_REQUIRE(s && in, "null argument");
+        TSVD_REQUIRE(in->hub_chunk >= 64 && in->ride_frac >= 0.0, "bad push setting");
+        s->hub_chunk = in->hub_chunk;
+        s->ride_frac = in->ride_frac;
     });
 }
 

@@ -186,8 +186,13 @@ struct StepPlan {
     tsvd_snapshot_stats S{};
     double t0 = 0.0;
 };
+struct PushTune {
+    const tsvd_push_setting *cand = nullptr;
+    int32_t ncand = 0;
+    double *cand_ms = nullptr;   // per candidate, -1 when not timed
+};
 bool pipeline_front(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, int64_t m, int on_device,
-                    StepPlan &plan);
+                    StepPlan &plan, const PushTune *tune = nullptr);
 void pipeline_decide(tsvd_pipeline *p, StepPlan &plan);
 void pipeline_back(tsvd_pipeline *p, StepPlan &plan, hipStream_t st);
 }  // namespace tsvd

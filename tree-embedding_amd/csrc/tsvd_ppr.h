@@ -67,4 +67,16 @@ void ppr_update_edges(tsvd_ppr *s, int dir, const int32_t *src, const int32_t *d
 void ppr_bind_sources(tsvd_ppr *s, const int32_t *sources, int32_t count);
 // Zero P and R of a direction.
 void ppr_clear(tsvd_ppr *s, int dir);
+// The push's input (p, r, pending masks and lists of every direction), saved
+// and put back, so that one push can be timed under several settings from the
+// same start.  A push leaves the next-round masks zero, so these suffice.
+struct PushSave {
+    tsvd::DBuf<uint8_t> P[2], R[2];
+    tsvd::DBuf<unsigned long long> cm[2];
+    tsvd::DBuf<uint32_t> F[2];
+    uint32_t count[2] = {0, 0};
+};
+void ppr_save(tsvd_ppr *s, PushSave &v);
+void ppr_restore(tsvd_ppr *s, const PushSave &v);
+size_t ppr_save_bytes(const tsvd_ppr *s);   // an upper bound of what ppr_save allocates
 }

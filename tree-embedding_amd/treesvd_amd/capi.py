@@ -26,12 +26,13 @@ EXPORTS = [
     "tsvd_read_lines", "tsvd_write_embedding_csv",
     "tsvd_ppr_create", "tsvd_ppr_destroy", "tsvd_ppr_refresh", "tsvd_ppr_push", "tsvd_ppr_read",
     "tsvd_ppr_last_push_stats", "tsvd_ppr_set_two_sided", "tsvd_ppr_refresh_edges",
+    "tsvd_ppr_get_push_setting", "tsvd_ppr_set_push_setting",
     "frPCA", "tsvd_frpca_csr", "tsvd_frpca_csr_batch",
     "tsvd_pipeline_create", "tsvd_pipeline_destroy", "tsvd_pipeline_snapshot",
     "tsvd_pipeline_embedding", "tsvd_pipeline_root_sigma", "tsvd_pipeline_graph", "tsvd_pipeline_ppr",
     "tsvd_pipeline_block_matrix", "tsvd_pipeline_block_norms", "tsvd_pipeline_node", "tsvd_pipeline_block_columns",
     "tsvd_pipeline_dense_gram", "tsvd_pipeline_gram_apply", "tsvd_pipeline_right_embedding",
-    "tsvd_pipeline_lp_scores", "tsvd_pipeline_source_state",
+    "tsvd_pipeline_lp_scores", "tsvd_pipeline_source_state", "tsvd_pipeline_snapshot_tuned",
     "tsvd_server_create", "tsvd_server_destroy", "tsvd_server_submit", "tsvd_server_wait",
     "tsvd_server_pipeline", "tsvd_server_edge_budget", "tsvd_probe_atomic_rate", "tsvd_gen_rmat",
 ]
@@ -73,6 +74,10 @@ class SnapshotStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PushSetting(ctypes.Structure):
+    _fields_ = [("hub_chunk", ctypes.c_int32), ("ride_frac", ctypes.c_double)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -129,6 +134,8 @@ def lib():
         L.tsvd_ppr_set_two_sided.argtypes = [vp, ctypes.c_int]
         L.tsvd_ppr_refresh_edges.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int64, ctypes.c_int,
                                              ctypes.c_int32, ctypes.c_int32]
+        L.tsvd_ppr_get_push_setting.argtypes = [vp, vp]
+        L.tsvd_ppr_set_push_setting.argtypes = [vp, vp]
         if hasattr(L, "frPCA"):
             L.frPCA.restype = None
             L.frPCA.argtypes = [ctypes.POINTER(MatCSR), vp, vp, vp, ctypes.c_int, ctypes.c_int]
@@ -156,6 +163,7 @@ def lib():
             L.tsvd_pipeline_right_embedding.argtypes = [vp, i32, i32, vp]
             L.tsvd_pipeline_lp_scores.argtypes = [vp, vp, vp, i64, vp]
             L.tsvd_pipeline_source_state.argtypes = [vp, i32, i32, vp, vp]
+            L.tsvd_pipeline_snapshot_tuned.argtypes = [vp, vp, vp, i64, ctypes.c_int, vp, i32, vp, vp, vp]
             L.tsvd_server_create.argtypes = [vp, vp, vp]
             L.tsvd_server_destroy.argtypes = [vp]
             L.tsvd_server_submit.argtypes = [vp, vp, vp, i64, vp]
@@ -366,6 +374,18 @@ class PPR:
         except Exception:
             pass
 
+    @property
+    def push_setting(self):
+        """(hub_chunk, ride_frac) of the push (tsvd_ppr_get_push_setting)."""
+        x = PushSetting()
+        check(lib().tsvd_ppr_get_push_setting(self.h, ctypes.byref(x)))
+        return x.hub_chunk, x.ride_frac
+
+    @push_setting.setter
+    def push_setting(self, hr):
+        x = PushSetting(int(hr[0]), float(hr[1]))
+        check(lib().tsvd_ppr_set_push_setting(self.h, ctypes.byref(x)))
+
     def refresh_edges(self, src, dst, d=0, s0=0, s1=None):
         """The n >= 1e7-node path's insertion update in batch order
         (tsvd_ppr_refresh_edges), after the same batch was appended."""
@@ -500,6 +520,23 @@ class Pipeline:
         st = SnapshotStats()
         check(lib().tsvd_pipeline_snapshot(self.h, ps, pd, int(src.shape[0]), dev, ctypes.byref(st)))
         return st.as_dict()
+
+    def snapshot_tuned(self, src, dst, candidates):
+        """One snapshot whose push is first timed under each (hub_chunk,
+        ride_frac) candidate (tsvd_pipeline_snapshot_tuned).  Returns (stats,
+        chosen setting, per-candidate push ms; -1 where not timed)."""
+        if isinstance(src, np.ndarray):
+            src = np.ascontiguousarray(src, dtype=np.int32)
+            dst = np.ascontiguousarray(dst, dtype=np.int32)
+        ps, dev = _dptr(src)
+        pd, _ = _dptr(dst)
+        cand = (PushSetting * max(len(candidates), 1))(*[PushSetting(int(h), float(r)) for h, r in candidates])
+        ms = np.zeros(max(len(candidates), 1))
+        chosen = PushSetting()
+        st = SnapshotStats()
+        check(lib().tsvd_pipeline_snapshot_tuned(self.h, ps, pd, int(src.shape[0]), dev, cand, len(candidates),
+                                                 ctypes.byref(chosen), _p(ms), ctypes.byref(st)))
+        return st.as_dict(), (chosen.hub_chunk, chosen.ride_frac), ms[:len(candidates)]
 
     def embedding(self):
         E = np.zeros((self.nsrc, self.rank))
