@@ -350,8 +350,8 @@ def main():
                     help="first run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is not launched by this build)"}))
+        print(json.dumps({"error": "the library runs on one GPU; --gpus must be 1 "
+                          "(source sharding over several GPUs is designed, not launched; see --shard)"}))
         sys.exit(2)
 
     from treesvd_amd import capi, gen
@@ -406,7 +406,8 @@ def main():
     log(f"initial snapshot {init_ms:.0f} ms (push {st0['ms_push']:.0f} ms, {st0['push_rounds']} rounds, "
         f"frpca {st0['ms_frpca']:.0f} ms, {st0['ndirty']} blocks)")
     if w.sparse_state:
-        log(f"pair-list state: {st0['state_entries']:.4e} entries ({st0['state_entries'] * 8 / 1e9:.2f} GB), "
+        log("sparse state resident (no pair lists)" if st0["state_batches"] == 0 else
+            f"pair-list state: {st0['state_entries']:.4e} entries ({st0['state_entries'] * 8 / 1e9:.2f} GB), "
             f"{st0['state_batches']} source batches")
     warm = []
     tuning = None
@@ -455,9 +456,14 @@ def main():
     state = None
     if w.sparse_state:
         ent = stats[-1]["state_entries"]
-        state = {"batches": stats[-1]["state_batches"], "entries": int(ent), "GB": round(ent * 8 / 1e9, 3),
-                 "dense_equivalent_GB": round(4 * w.n * (-(-w.subset.size // 64) * 64) * 4 / 1e9, 3),
-                 "note": "positive (vertex, FP32 value) pairs of p and r, both directions, all sources"}
+        dense_gb = round(4 * w.n * (-(-w.subset.size // 64) * 64) * 4 / 1e9, 3)
+        if stats[-1]["state_batches"] == 0:
+            state = {"resident": True, "dense_GB": dense_gb,
+                     "note": "the whole state fits in HBM: one resident batch, no pair lists"}
+        else:
+            state = {"batches": stats[-1]["state_batches"], "entries": int(ent), "GB": round(ent * 8 / 1e9, 3),
+                     "dense_equivalent_GB": dense_gb,
+                     "note": "positive (vertex, FP32 value) pairs of p and r, both directions, all sources"}
     out = {
         "metric": "update->embedding snapshots/s (plain GPU method), p50/p99 ms per update; served vs plain",
         "value": round(value, 4), "unit": "snapshots/s", "n_gpus": 1, "steps": args.steps,

@@ -190,7 +190,8 @@ typedef struct {
      * with rows >= 30 floor(|S|/30) left empty (ppr:6898-7229).  The state
      * is pushed a batch of source groups at a time. */
     int32_t sparse_state;
-    int32_t sparse_groups;   /* groups of 64 sources per batch (0: as HBM allows) */
+    int32_t sparse_groups;   /* groups of 64 sources per batch (0: all of them resident
+                              * when they fit in HBM, else as HBM allows)          */
 } tsvd_config;
 
 typedef struct {
@@ -203,7 +204,8 @@ typedef struct {
     int64_t push_wave_relax;  /* neighbour visits summed over those items      */
     double ms_total, ms_ingest, ms_refresh, ms_push, ms_blocks, ms_frpca, ms_merge;
     int64_t state_entries;    /* sparse_state: (vertex, value) pairs stored, all lists */
-    int32_t state_batches;    /* sparse_state: source batches pushed per snapshot     */
+    int32_t state_batches;    /* sparse_state: source batches pushed per snapshot (0: one
+                               * resident batch, the state kept in HBM without lists)  */
 } tsvd_snapshot_stats;
 
 int tsvd_pipeline_create(const tsvd_config *cfg, tsvd_pipeline **out);

@@ -45,10 +45,15 @@ static void validate(const tsvd_config &c)
     TSVD_REQUIRE(c.sparse_groups >= 0, "sparse_groups must be >= 0");
 }
 
-// Batches of the sparse state: as many groups of 64 sources as 30% of the
-// free HBM holds (four FP32 arrays of n x 64 per group plus the push's item
-// lists); the rest is left for the pair lists, which on power-law graphs
-// can outgrow the batch state (a hub's residue reaches millions of vertices).
+// Batches of the sparse state.  When the state of every source fits in 60%
+// of the free HBM it stays resident as one batch: no pair lists are written,
+// and the negative residues the lists would drop are cleared at the start of
+// the next update instead (the same state, since nothing reads them in
+// between).  Otherwise, as many groups of 64 sources as 30% of the free HBM
+// holds (four FP32 arrays of n x 64 per group plus the push's item lists) are
+// pushed at a time, and the rest of the memory is left for the pair lists,
+// which on power-law graphs can outgrow the batch state (a hub's residue
+// reaches millions of vertices).  sparse_groups > 0 always uses lists.
 static void sparse_setup(tsvd_pipeline *p, hipStream_t st)
 {
     const tsvd_config &c = p->cfg;
@@ -59,7 +64,12 @@ static void sparse_setup(tsvd_pipeline *p, hipStream_t st)
         size_t fr = 0, tot = 0;
         TSVD_HIP(hipMemGetInfo(&fr, &tot));
         const double per = (double)c.n * (4.0 * 64 * 4 + 36.0);
-        gpb = (int32_t)std::max(1.0, std::floor(0.3 * (double)fr / per));
+        if (per * ngroups <= 0.6 * (double)fr) {
+            ss.resident = true;
+            gpb = ngroups;
+        } else {
+            gpb = (int32_t)std::max(1.0, std::floor(0.3 * (double)fr / per));
+        }
     }
     ss.gpb = std::min(gpb, ngroups);
     ss.nbatch = (int32_t)ceil_div(ngroups, ss.gpb);
@@ -177,15 +187,19 @@ static void sparse_front(tsvd_pipeline *p, int32_t it, const int32_t *dsrc, cons
     for (int32_t bt = 0; bt < ss.nbatch; bt++) {
         const int32_t row0 = bt * ss.gpb * 64, cnt = ss.batch_sources(bt, c.nsrc);
         auto &L = ss.lists[(size_t)bt];
-        ppr_bind_sources(gp, p->sources.data() + row0, cnt);
+        if (!ss.resident) ppr_bind_sources(gp, p->sources.data() + row0, cnt);
         for (int d = 0; d < 2; d++) {
             const double t0 = now_ms();
             if (fresh) {
                 ppr_refresh(gp, TSVD_REFRESH_INIT, d, 0, cnt);
             } else {
-                ppr_clear(gp, d);
-                list_load(L[(size_t)(2 * d)], gp->p(d), gp->s_pad, st);
-                list_load(L[(size_t)(2 * d + 1)], gp->r(d), gp->s_pad, st);
+                if (ss.resident) {
+                    ppr_clamp_negative(gp, d);   // what storing the positive entries drops
+                } else {
+                    ppr_clear(gp, d);
+                    list_load(L[(size_t)(2 * d)], gp->p(d), gp->s_pad, st);
+                    list_load(L[(size_t)(2 * d + 1)], gp->r(d), gp->s_pad, st);
+                }
                 ppr_update_edges(gp, d, dsrc, ddst, m, 0, cnt);
             }
             const double t1 = now_ms();
@@ -199,7 +213,7 @@ static void sparse_front(tsvd_pipeline *p, int32_t it, const int32_t *dsrc, cons
             S.push_wave_relax += gp->last_wave_relax[d];
         }
         const double t2 = now_ms();
-        for (int d = 0; d < 2; d++) {
+        for (int d = 0; d < 2 && !ss.resident; d++) {
             list_store(ss.work, L[(size_t)(2 * d)], gp->p(d), 4, c.n, gp->s_pad, cnt, st);
             if (keep_r) list_store(ss.work, L[(size_t)(2 * d + 1)], gp->r(d), 4, c.n, gp->s_pad, cnt, st);
             else list_clear(L[(size_t)(2 * d + 1)]);
@@ -210,7 +224,7 @@ static void sparse_front(tsvd_pipeline *p, int32_t it, const int32_t *dsrc, cons
     const double t3 = now_ms();
     blocks_norms_read(b, st);
     S.ms_blocks += now_ms() - t3;
-    S.state_batches = ss.nbatch;
+    S.state_batches = ss.resident ? 0 : ss.nbatch;
     for (const auto &L : ss.lists)
         for (const auto &x : L) S.state_entries += x.nnz;
 }
@@ -546,7 +560,7 @@ extern "C" int tsvd_pipeline_source_state(tsvd_pipeline *p, int32_t dir, int32_t
         const tsvd_config &c = p->cfg;
         TSVD_REQUIRE(dir >= 0 && dir < (c.directed ? 2 : 1), "bad direction");
         TSVD_REQUIRE(i >= 0 && i < c.nsrc, "source index out of range");
-        if (c.sparse_state) {
+        if (c.sparse_state && !p->sp.resident) {
             const SparseState &ss = p->sp;
             const int32_t bt = i / (ss.gpb * 64), s = i - bt * ss.gpb * 64;
             const auto &L = ss.lists[(size_t)bt];
@@ -554,17 +568,14 @@ extern "C" int tsvd_pipeline_source_state(tsvd_pipeline *p, int32_t dir, int32_t
             if (residue) list_read(L[(size_t)(2 * dir + 1)], s, c.n, residue);
             return;
         }
-        tsvd_ppr *q = p->ppr;
+        tsvd_ppr *q = p->ppr ? p->ppr : p->sp.gp;   // the dense state, or the resident sparse one
         const int32_t lane = p->lane_of.empty() ? i : p->lane_of[(size_t)i];
         double *out[2] = {reserve, residue};
-        std::vector<uint8_t> h((size_t)c.n * q->esize);
         for (int k = 0; k < 2; k++) {
             if (!out[k]) continue;
-            const uint8_t *base = (const uint8_t *)(k == 0 ? q->p(dir) : q->r(dir)) + (size_t)lane * q->esize;
-            TSVD_HIP(hipMemcpy2D(h.data(), q->esize, base, (size_t)q->s_pad * q->esize, q->esize, (size_t)c.n,
-                                 hipMemcpyDeviceToHost));
-            for (int32_t v = 0; v < c.n; v++)
-                out[k][v] = q->esize == 8 ? ((double *)h.data())[v] : (double)((float *)h.data())[v];
+            ppr_read_lane(q, dir, k, lane, out[k]);
+            if (c.sparse_state)   // the stored state: positive entries only
+                for (int32_t v = 0; v < c.n; v++) out[k][v] = std::max(out[k][v], 0.0);
         }
     });
 }

@@ -256,6 +256,20 @@ __global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict
 //                     them and tests r > rmax again when it pops them).
 constexpr int32_t NO_EDGE = 0x7F7F7F7F;   // first[] of a row with no new edge
 
+template <class T>
+__global__ void k_lane_column(const T *__restrict__ X, int32_t n, int32_t s_pad, int32_t lane, double *__restrict__ out)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) out[v] = (double)X[v * s_pad + lane];
+}
+
+template <class T>
+__global__ void k_clamp_negative(T *__restrict__ R, int64_t cells)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cells && R[i] < T(0)) R[i] = T(0);
+}
+
 __global__ void k_first_edge(const int32_t *__restrict__ efrom, int64_t m, const int32_t *__restrict__ former,
                              int32_t *__restrict__ first, int reset)
 {
@@ -1031,6 +1045,36 @@ void ppr_bind_sources(tsvd_ppr *s, const int32_t *sources, int32_t count)
     TSVD_HIP(hipStreamSynchronize(s->g->stream));
     s->h_sources.assign(sources, sources + count);
     s->nsrc = count;
+}
+
+void ppr_read_lane(tsvd_ppr *s, int dir, int what, int32_t lane, double *out)
+{
+    TSVD_REQUIRE(dir >= 0 && dir < s->ndir && lane >= 0 && lane < s->s_pad, "bad direction or lane");
+    hipStream_t st = s->g->stream;
+    DBuf<double> col((size_t)s->n);
+    const void *X = what == 0 ? s->p(dir) : s->r(dir);
+    if (s->precision == TSVD_FP64)
+        hipLaunchKernelGGL(k_lane_column<double>, dim3((unsigned)ceil_div(s->n, TPB)), dim3(TPB), 0, st,
+                           (const double *)X, s->n, s->s_pad, lane, col.p);
+    else
+        hipLaunchKernelGGL(k_lane_column<float>, dim3((unsigned)ceil_div(s->n, TPB)), dim3(TPB), 0, st,
+                           (const float *)X, s->n, s->s_pad, lane, col.p);
+    TSVD_HIP(hipGetLastError());
+    TSVD_HIP(hipMemcpyAsync(out, col.p, sizeof(double) * s->n, hipMemcpyDeviceToHost, st));
+    TSVD_HIP(hipStreamSynchronize(st));
+}
+
+void ppr_clamp_negative(tsvd_ppr *s, int dir)
+{
+    hipStream_t st = s->g->stream;
+    const int64_t cells = (int64_t)s->n * s->s_pad;
+    if (s->precision == TSVD_FP64)
+        hipLaunchKernelGGL(k_clamp_negative<double>, dim3((unsigned)ceil_div(cells, TPB)), dim3(TPB), 0, st,
+                           (double *)s->R[dir].p, cells);
+    else
+        hipLaunchKernelGGL(k_clamp_negative<float>, dim3((unsigned)ceil_div(cells, TPB)), dim3(TPB), 0, st,
+                           (float *)s->R[dir].p, cells);
+    TSVD_HIP(hipGetLastError());
 }
 
 void ppr_clear(tsvd_ppr *s, int dir)

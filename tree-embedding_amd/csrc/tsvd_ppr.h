@@ -67,6 +67,10 @@ void ppr_update_edges(tsvd_ppr *s, int dir, const int32_t *src, const int32_t *d
 void ppr_bind_sources(tsvd_ppr *s, const int32_t *sources, int32_t count);
 // Zero P and R of a direction.
 void ppr_clear(tsvd_ppr *s, int dir);
+// Host FP64 copy of one lane's reserves (what 0) or residues (1), n values.
+void ppr_read_lane(tsvd_ppr *s, int dir, int what, int32_t lane, double *out);
+// r = max(r, 0) over a direction's residues.
+void ppr_clamp_negative(tsvd_ppr *s, int dir);
 // The push's input (p, r, pending masks and lists of every direction), saved
 // and put back, so that one push can be timed under several settings from the
 // same start.  A push leaves the next-round masks zero, so these suffice.

@@ -32,6 +32,7 @@ struct SparseWork {
 // push state of one batch (width gpb * 64), rebound to each batch in turn.
 struct SparseState {
     int32_t gpb = 0, nbatch = 0;
+    bool resident = false;                         // one batch kept in HBM, no lists
     int32_t keep = 0;                              // rows >= keep are left out of the blocks (Q11)
     tsvd_ppr *gp = nullptr;
     std::vector<std::array<PairList, 4>> lists;   // per batch: p, r of dir 0, p, r of dir 1
