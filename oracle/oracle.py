@@ -8,6 +8,9 @@ product library never imports it.
   oracle/treesvd_oracle.c (ctypes below), built by oracle/Makefile.
 * Block map, elementwise transform, lazy dirty test, frPCA, tree merge and the
   root embedding: numpy FP64 below, each citing the reference lines.
+* The n >= 1e7-node path (Pipeline(sparse_state=True)): the edge-order update
+  and the positive-only storage in C, blocks from p + p^T with the
+  30-interval row cut here.
 
 Pinning of each function (the tests that check it) is listed in its docstring;
 `exact_ppr` and the textbook numpy SVD are the closed forms used there.

@@ -18,7 +18,12 @@
  *   refresh+push  -- the PPR invariant  pi_s = p_s + sum_v r_s(v) pi_v  against
  *                    an exact FP64 linear solve, the termination condition,
  *                    and hand-computed single-step vectors;
- *   file reader   -- the reference's duplicate-last-edge behaviour (oracle/_ref).
+ *   file reader   -- the reference's duplicate-last-edge behaviour (oracle/_ref);
+ *   edge-order update (the n >= 1e7-node path) -- equal to the row update and
+ *                    the exact invariant where no vertex gets its first edge;
+ *                    with a first edge, the invariant's defect is exactly the
+ *                    dropped residue times pi_w, in batch order;
+ *   negative drop -- keeps exactly the positive entries.
  */
 #include <stdint.h>
 #include <stdio.h>
