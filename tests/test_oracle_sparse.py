@@ -178,3 +178,23 @@ def test_first_snapshot_equals_the_dense_path():
         assert np.abs(x - y).max() <= 1e-12 * max(1.0, np.abs(y).max())
     # frPCA's LU-orthonormalised power passes round at about 1e-7 relative
     assert np.allclose(a.root_sigma, b.root_sigma, rtol=1e-6)
+
+
+def test_edge_order_update_is_the_same_on_any_thread_count():
+    """Sources are independent; each thread keeps its own per-vertex edge count."""
+    n = 120
+    u, v = random_graph(n, 900, 21, True)
+    cut = 700
+    src = np.arange(0, 40, 3, dtype=np.int32)
+    out = []
+    for threads in (1, 4):
+        g = orc.Graph(n, True)
+        g.append(u[:cut], v[:cut])
+        pp = orc.PPR(n, src, True, ALPHA, 1e-7, fp32=True)
+        pp.init_static()
+        pp.push(g, threads)
+        g.append(u[cut:], v[cut:])
+        pp.refresh_edges(g, u[cut:], v[cut:], threads)
+        out.append([(pp.p(d).copy(), pp.r(d).copy()) for d in (0, 1)])
+    for d in (0, 1):
+        assert np.array_equal(out[0][d][0], out[1][d][0]) and np.array_equal(out[0][d][1], out[1][d][1])
