@@ -102,8 +102,24 @@ struct tsvd_server {
         std::unique_lock<std::mutex> lk(fmu);
         fcv.wait(lk, [&] { return !back_busy; });
     }
+    // Drains the queue, then stops the worker, the finisher, the pipeline and
+    // the back stream, in that order; also reached when creation fails part
+    // way, with whichever of them had been started.
     ~tsvd_server()
     {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv_work.notify_all();
+        if (worker.joinable()) worker.join();
+        {
+            std::lock_guard<std::mutex> lk(fmu);
+            fstop = true;
+        }
+        fcv.notify_all();
+        if (finisher.joinable()) finisher.join();
+        if (pipe) tsvd_pipeline_destroy(pipe);
         if (back_stream) (void)hipStreamDestroy(back_stream);
     }
 };
@@ -241,23 +257,7 @@ extern "C" int tsvd_server_create(const tsvd_config *cfg, const tsvd_server_conf
 
 extern "C" int tsvd_server_destroy(tsvd_server *s)
 {
-    return guard([&] {
-        if (!s) return;
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->stop = true;
-        }
-        s->cv_work.notify_all();
-        if (s->worker.joinable()) s->worker.join();
-        {
-            std::lock_guard<std::mutex> lk(s->fmu);
-            s->fstop = true;
-        }
-        s->fcv.notify_all();
-        if (s->finisher.joinable()) s->finisher.join();
-        if (s->pipe) tsvd_pipeline_destroy(s->pipe);
-        delete s;
-    });
+    return guard([&] { delete s; });
 }
 
 extern "C" int tsvd_server_submit(tsvd_server *s, const int32_t *src, const int32_t *dst, int64_t m, int64_t *ticket)
