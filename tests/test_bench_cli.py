@@ -28,3 +28,9 @@ def test_sparse_state_needs_a_directed_workload():
 def test_shard_index_is_checked():
     r = run_bench("--workload", "plain", "--scale-down", "6", "--shard", "3/2")
     assert r.returncode != 0 and "0 <= I < N" in r.stderr
+
+
+def test_more_steps_than_insertion_batches_is_refused_before_any_device_call():
+    r = run_bench("--workload", "plain", "--scale-down", "6", "--steps", "500", "--warmup", "1")
+    assert r.returncode != 0 and "insertion batches; warmup+steps=501" in r.stderr
+    assert "device " not in r.stderr
