@@ -72,6 +72,12 @@ int tsvd_graph_destroy(tsvd_graph *g);
 /* src/dst: `m` int32 ids; on_device != 0 means they are device pointers. */
 int tsvd_graph_append(tsvd_graph *g, const int32_t *src, const int32_t *dst, int64_t m, int on_device);
 int64_t tsvd_graph_num_edges(const tsvd_graph *g);
+/* A host batch is staged on the device; a batch of more than `edges` edges
+ * (default 2^26) frees that staging once the append and the caller's
+ * insertion update have read it, so an initial load of a large graph does
+ * not keep ~40 B per edge for the small batches that follow.  No reference
+ * counterpart (its reader holds the whole edge vector on the host). */
+int tsvd_graph_set_staging_limit(tsvd_graph *g, int64_t edges);
 /* Host copies of degree / former degree (n int32 each; either may be NULL). */
 int tsvd_graph_degrees(const tsvd_graph *g, int dir, int32_t *deg, int32_t *former);
 /* Compact host CSR of one direction: rowptr[n+1], col[rowptr[n]]; col may be

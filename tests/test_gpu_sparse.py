@@ -212,3 +212,33 @@ def test_sparse_pipeline_vs_oracle_and_batching(capi):
         assert worst <= 4.0 * it and batch_gap <= 4.0 * it, it
         assert s12 < 1e-3, it
         assert diff[:k // 2].max() < 1e-2 and diff.max() < 3e-2, it
+
+
+def test_sparse_host_batches_above_the_staging_limit(capi):
+    """Host batches larger than the graph's staging limit free their device
+    copy after use; the sparse-state insertion update reads that copy, so it
+    must outlive the append.  With the limit at 100 edges every batch of this
+    stream is above it: the run must match one with the default limit within
+    the bound the batching test uses (two runs differ only in the order of
+    their atomic adds, and so in which negative residues they drop)."""
+    n, snaps, sub = workload(seed=7, nsrc=64)
+    rmax = 1e-6
+    kw = dict(transform=capi.TSVD_TRANSFORM_LP, delta=0.3, dyn_start=2, lazy_start=2, seed=5, precision=1,
+              sparse_state=True)
+    low = capi.Pipeline(n, True, sub, ALPHA, rmax, 8, 2, 10, 6, **kw)
+    ref = capi.Pipeline(n, True, sub, ALPHA, rmax, 8, 2, 10, 6, **kw)
+    low.graph.set_staging_limit(100)
+    assert all(s.size > 100 for s, _ in snaps)
+    for it, (s, d) in enumerate(snaps, start=1):
+        low.snapshot(s, d)
+        ref.snapshot(s, d)
+        for dd in (0, 1):
+            rp, col = ref.graph.csr(dd)
+            c = column_mass(n, rp, col, ALPHA)
+            for i in (0, 17, 63):
+                a, b = low.source_state(i, dd)[0], ref.source_state(i, dd)[0]
+                # the lists store FP32 values; each snapshot drops a slightly
+                # different set of negative residues (as in the batching test)
+                assert (np.abs(a - b) <= 4.0 * it * rmax * c + 4e-7 * np.abs(b) + 1e-12).all(), (it, dd, i)
+        s1, s2 = low.root_sigma(), ref.root_sigma()
+        assert np.abs(s1 - s2).max() / s2.max() < 1e-3

@@ -296,13 +296,23 @@ void graph_append(tsvd_graph *g, const int32_t *src, const int32_t *dst, int64_t
     g->m += m;
     g->version++;
     TSVD_HIP(hipStreamSynchronize(st));
-    if (m > ((int64_t)1 << 26)) {
-        // an initial load of a large graph: its staging (keys, values, sort
-        // and scan buffers, about 40 B per edge) is not kept for the small
-        // batches that follow
+    if (m > g->release_above) {
+        // an initial load of a large graph: its sort staging (keys, values,
+        // sort and scan buffers, about 32 B per edge) is not kept for the
+        // small batches that follow.  The staged edge ids stay: the caller's
+        // insertion update may still read them (graph_release_edges frees
+        // them once it has).
         g->k_in.release(); g->v_in.release(); g->k_out.release(); g->v_out.release();
         g->scan_flag.release(); g->scan_pos.release(); g->cub_tmp.release();
-        g->e_src.release(); g->e_dst.release();
+    }
+}
+
+void graph_release_edges(tsvd_graph *g)
+{
+    if ((int64_t)g->e_src.n > g->release_above) {
+        TSVD_HIP(hipStreamSynchronize(g->stream));
+        g->e_src.release();
+        g->e_dst.release();
     }
 }
 
@@ -343,6 +353,16 @@ extern "C" int tsvd_graph_append(tsvd_graph *g, const int32_t *src, const int32_
     return guard([&] {
         TSVD_REQUIRE(g, "null graph");
         graph_append(g, src, dst, m, on_device);
+        graph_release_edges(g);
+    });
+}
+
+extern "C" int tsvd_graph_set_staging_limit(tsvd_graph *g, int64_t edges)
+{
+    return guard([&] {
+        TSVD_REQUIRE(g, "null graph");
+        TSVD_REQUIRE(edges >= 0, "negative edge count");
+        g->release_above = edges;
     });
 }
 

@@ -43,6 +43,7 @@ static void validate(const tsvd_config &c)
                  "sparse_state is the directed LP driver's path (Tree_d_LP.cpp:1377-2024)");
     TSVD_REQUIRE(!c.sparse_state || !c.two_sided, "sparse_state keeps the reference's positive-side threshold");
     TSVD_REQUIRE(c.sparse_groups >= 0, "sparse_groups must be >= 0");
+    TSVD_REQUIRE(c.dyn_start >= 1 && c.lazy_start >= 1, "dyn_start and lazy_start are 1-based snapshot numbers");
 }
 
 // Batches of the sparse state.  When the state of every source fits in 60%
@@ -346,6 +347,7 @@ bool pipeline_front(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, in
     if (c.lazy_start != (c.sparse_state ? 200 : 100) && plan.it < c.lazy_start - 1) {
         plan.skipped = true;
         S.reused = 1;
+        graph_release_edges(p->g);
         return false;
     }
     Blocks &b = p->blocks;
@@ -373,6 +375,7 @@ bool pipeline_front(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, in
         blocks_norms(b, p->ppr->p(0), p->ppr->ndir > 1 ? p->ppr->p(1) : nullptr, p->g, st);
         S.ms_blocks = now_ms() - t3;
     }
+    graph_release_edges(p->g);
     plan.a2 = b.a2;
     return true;
 }

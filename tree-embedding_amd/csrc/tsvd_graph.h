@@ -25,6 +25,7 @@ struct tsvd_graph {
     int64_t m = 0;
     int64_t rebuilds = 0;
     int64_t version = 0;      // bumped by every append
+    int64_t release_above = (int64_t)1 << 26;   // batches above this many edges free their staging
     hipStream_t stream = nullptr;
     tsvd::GraphDir dir[2];
     tsvd::DBuf<int32_t> e_src, e_dst, k_in, v_in, k_out, v_out, scan_flag, scan_pos, segstart;
@@ -41,4 +42,7 @@ struct tsvd_graph {
 
 namespace tsvd {
 void graph_append(tsvd_graph *g, const int32_t *src, const int32_t *dst, int64_t m, int on_device);
+// frees the device copy of the last host batch when it exceeded release_above
+// edges; callers that read the staged ids (e_src/e_dst) call it afterwards
+void graph_release_edges(tsvd_graph *g);
 }

@@ -22,7 +22,7 @@ EXPORTS = [
     "tsvd_last_error", "tsvd_device_info", "tsvd_device_alloc", "tsvd_device_free", "tsvd_memcpy",
     "tsvd_profile_enable", "tsvd_profile_reset", "tsvd_profile_read",
     "tsvd_graph_create", "tsvd_graph_destroy", "tsvd_graph_append", "tsvd_graph_num_edges",
-    "tsvd_graph_degrees", "tsvd_graph_export_csr", "tsvd_read_edge_file", "tsvd_read_subset_file",
+    "tsvd_graph_degrees", "tsvd_graph_export_csr", "tsvd_graph_set_staging_limit", "tsvd_read_edge_file", "tsvd_read_subset_file",
     "tsvd_read_lines", "tsvd_write_embedding_csv",
     "tsvd_ppr_create", "tsvd_ppr_destroy", "tsvd_ppr_refresh", "tsvd_ppr_push", "tsvd_ppr_read",
     "tsvd_ppr_last_push_stats", "tsvd_ppr_set_two_sided", "tsvd_ppr_refresh_edges",
@@ -118,6 +118,7 @@ def lib():
         L.tsvd_graph_destroy.argtypes = [vp]
         L.tsvd_graph_append.argtypes = [vp, vp, vp, i64, ctypes.c_int]
         L.tsvd_graph_num_edges.restype = i64
+        L.tsvd_graph_set_staging_limit.argtypes = [vp, i64]
         L.tsvd_graph_num_edges.argtypes = [vp]
         L.tsvd_graph_degrees.argtypes = [vp, ctypes.c_int, vp, vp]
         L.tsvd_graph_export_csr.argtypes = [vp, ctypes.c_int, vp, vp]
@@ -301,6 +302,10 @@ class Graph:
 
     def num_edges(self):
         return lib().tsvd_graph_num_edges(self.h)
+
+    def set_staging_limit(self, edges):
+        """Host batches above `edges` edges free their device staging after use."""
+        check(lib().tsvd_graph_set_staging_limit(self.h, int(edges)))
 
     def degrees(self, d=0):
         deg = np.zeros(self.n, np.int32); former = np.zeros(self.n, np.int32)
