@@ -48,35 +48,6 @@ def pipeline_for(capi, w, seed=7):
                          sparse_state=w.sparse_state)
 
 
-def atscale_workload(capi, gen, scale_down=0):
-    """Served at scale: directed R-MAT (0.57, 0.19, 0.19, 0.05) over 2^26 ids
-    taken modulo 41,652,230, seed 4, 1,468,365,182 distinct edges without
-    self-loops, drawn in HBM (tsvd_gen_rmat: host generation would outlast
-    the run); 90% initial, then LogNormal(ln 100k, 1) batches; 3000 sources
-    drawn from the initial snapshot's edge sources; rmax 1e-7; 4096 blocks,
-    rank 128, fan-out 4; LP transform (the directed LP driver).  Snapshots are
-    device slices of the generated arrays; returns (workload, owned arrays)."""
-    sd = 1 << scale_down
-    n, m = 41_652_230 // sd, 1_468_365_182 // sd
-    scale = int(np.ceil(np.log2(n)))
-    src, dst = capi.gen_rmat(scale, n, m, 0.57, 0.19, 0.19, seed=4)
-    n0 = int(round(0.9 * m))
-    snaps = [(capi.DeviceSlice(src, 0, n0), capi.DeviceSlice(dst, 0, n0))]
-    pos = n0
-    for b in gen.lognormal_sizes(60, 100_000 / sd, 1.0, 4):
-        b = int(min(b, m - pos))
-        if b <= 0:
-            break
-        snaps.append((capi.DeviceSlice(src, pos, b), capi.DeviceSlice(dst, pos, b)))
-        pos += b
-    # edge order is random, so the first 16M initial edges sample its sources uniformly
-    head = capi.DeviceSlice(src, 0, min(n0, 1 << 24)).numpy()
-    sub = gen.sample_subset(head, 3000, 4)
-    w = gen.Workload("atscale", n, True, snaps, sub, 0.2, 1e-7, max(16, 4096 // sd), 4, 128, 6, False,
-                     transform="LP")
-    return w, (src, dst)
-
-
 def roofline_of(prof, precision_fp64):
     """Dominant kernel of this library by total time (library solver calls,
     which carry no algorithmic count, are skipped); HBM-bound kernels report
@@ -358,9 +329,9 @@ def main():
     from treesvd_amd import capi, gen
     t = time.time()
     owned = None
-    if args.workload == "atscale":
-        w, owned = atscale_workload(capi, gen, args.scale_down)
-        args.no_cpu_baseline = True   # the CPU leg would need the 1.47e9-edge graph on the host
+    if args.workload in ("served", "atscale"):
+        w, owned = gen.device_workload(capi, args.workload, args.scale_down)
+        args.no_cpu_baseline = True   # the CPU leg would need the 1.8e8 / 1.47e9-edge graph on the host
     else:
         w = gen.workload(args.workload, scale_down=args.scale_down)
     if args.transform:

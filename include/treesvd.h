@@ -294,6 +294,13 @@ int tsvd_pipeline_source_state(tsvd_pipeline *p, int32_t dir, int32_t i, double 
  * tsvd_device_free. */
 int tsvd_gen_rmat(int32_t scale, int32_t n, int64_t m, double a, double b, double c, uint64_t seed, int32_t **src,
                   int32_t **dst);
+/* Seeded synthetic directed Chung-Lu graph drawn in HBM (no reference
+ * counterpart; same purpose).  u is drawn with probability proportional to
+ * (u+1)^(-1/(out_exp-1)), v proportional to (pi(v)+1)^(-1/(in_exp-1)) for a
+ * seeded random permutation pi; no self-loops; m distinct edges in a seeded
+ * random order.  Outputs as tsvd_gen_rmat. */
+int tsvd_gen_chung_lu(int32_t n, int64_t m, double out_exp, double in_exp, uint64_t seed, int32_t **src,
+                      int32_t **dst);
 
 /* ---------------------------------------------------------- served form
  * No reference counterpart: the reference's drivers run the whole update once

@@ -35,6 +35,7 @@ EXPORTS = [
     "tsvd_pipeline_lp_scores", "tsvd_pipeline_source_state", "tsvd_pipeline_snapshot_tuned",
     "tsvd_server_create", "tsvd_server_destroy", "tsvd_server_submit", "tsvd_server_wait",
     "tsvd_server_pipeline", "tsvd_server_edge_budget", "tsvd_probe_atomic_rate", "tsvd_gen_rmat",
+    "tsvd_gen_chung_lu",
 ]
 
 
@@ -176,6 +177,7 @@ def lib():
             L.tsvd_probe_atomic_rate.argtypes = [i64, i32, vp]
             L.tsvd_gen_rmat.argtypes = [i32, i32, i64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_uint64, vp, vp]
+            L.tsvd_gen_chung_lu.argtypes = [i32, i64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, vp, vp]
         _lib = L
     return _lib
 
@@ -254,6 +256,17 @@ def gen_rmat(scale, n, m, a=0.57, b=0.19, c=0.19, seed=0):
     ps, pd = ctypes.c_void_p(), ctypes.c_void_p()
     check(lib().tsvd_gen_rmat(int(scale), int(n), int(m), float(a), float(b), float(c), ctypes.c_uint64(seed),
                               ctypes.byref(ps), ctypes.byref(pd)))
+    return DeviceArray.adopt(ps, np.int32, (int(m),)), DeviceArray.adopt(pd, np.int32, (int(m),))
+
+
+def gen_chung_lu(n, m, out_exp, in_exp, seed=0):
+    """Seeded directed Chung-Lu edges drawn in HBM (tsvd_gen_chung_lu): m
+    distinct (src, dst) pairs, out-weight (u+1)^(-1/(out_exp-1)), in-weight
+    (pi(v)+1)^(-1/(in_exp-1)) for a seeded permutation pi, no self-loops, in
+    a seeded random order.  Returns two DeviceArray objects of m int32 each."""
+    ps, pd = ctypes.c_void_p(), ctypes.c_void_p()
+    check(lib().tsvd_gen_chung_lu(int(n), int(m), float(out_exp), float(in_exp), ctypes.c_uint64(seed),
+                                  ctypes.byref(ps), ctypes.byref(pd)))
     return DeviceArray.adopt(ps, np.int32, (int(m),)), DeviceArray.adopt(pd, np.int32, (int(m),))
 
 

@@ -217,6 +217,50 @@ def workload(name: str, scale_down: int = 0, nsub: int | None = None,
     raise ValueError(f"unknown workload {name!r}")
 
 
+def device_workload(capi, name: str, scale_down: int = 0, nsub: int | None = None):
+    """The two largest configurations, drawn in HBM (their host generation
+    takes minutes): returns (Workload whose snapshots are device slices of
+    the generated edge arrays, the owned device arrays).
+
+    served   directed Chung-Lu (tsvd_gen_chung_lu), 6,216,199 nodes,
+             177,862,656 edges, out-exponent 2.3, in-exponent 2.0, seed 3;
+             80% initial + 60 lognormal batches (median 10k, sigma 1, clipped
+             to [1k, 1M]); |S| = 3000, rmax 1e-7, 1024 blocks, rank 128,
+             fan-out 4, LP transform.
+    atscale  directed R-MAT (0.57, 0.19, 0.19, 0.05) over 2^26 ids taken
+             modulo 41,652,230 (tsvd_gen_rmat), 1,468,365,182 edges, seed 4;
+             90% initial + 60 lognormal batches (median 100k, sigma 1);
+             |S| = 3000, rmax 1e-7, 4096 blocks, rank 128, fan-out 4, LP.
+    Sources are drawn among the edge sources of the first 16M initial edges
+    (the edge order is random, so they sample the initial snapshot's sources
+    uniformly)."""
+    sd = 1 << scale_down
+    if name == "served":
+        n, m = 6_216_199 // sd, 177_862_656 // sd
+        src, dst = capi.gen_chung_lu(n, m, 2.3, 2.0, seed=3)
+        init, sizes, seed, nparts = 0.8, lognormal_sizes(60, 10_000 / sd, 1.0, 3, max(1, 1_000 // sd),
+                                                          1_000_000 // sd), 3, 1024
+    elif name == "atscale":
+        n, m = 41_652_230 // sd, 1_468_365_182 // sd
+        src, dst = capi.gen_rmat(int(np.ceil(np.log2(n))), n, m, 0.57, 0.19, 0.19, seed=4)
+        init, sizes, seed, nparts = 0.9, lognormal_sizes(60, 100_000 / sd, 1.0, 4), 4, 4096
+    else:
+        raise ValueError(f"no device workload {name!r}")
+    n0 = int(round(init * m))
+    snaps = [(capi.DeviceSlice(src, 0, n0), capi.DeviceSlice(dst, 0, n0))]
+    pos = n0
+    for b in sizes:
+        b = int(min(b, m - pos))
+        if b <= 0:
+            break
+        snaps.append((capi.DeviceSlice(src, pos, b), capi.DeviceSlice(dst, pos, b)))
+        pos += b
+    head = capi.DeviceSlice(src, 0, min(n0, 1 << 24)).numpy()
+    sub = sample_subset(head, nsub or 3000, seed)
+    w = Workload(name, n, True, snaps, sub, 0.2, 1e-7, max(16, nparts // sd), 4, 128, 6, False, transform="LP")
+    return w, (src, dst)
+
+
 def request_stream(w: Workload, count: int, rate: float, seed: int, median: float = 10_000, sigma: float = 1.0,
                    lo: int = 1_000, hi: int = 1_000_000):
     """A served-mode stream over a workload's graph: after its initial
