@@ -161,6 +161,23 @@ int tsvd_frpca_csr(const mat_csr *A, mat *U, mat *S, mat *V, int k, int q, uint6
 int tsvd_frpca_csr_batch(const mat_csr *const *A, mat *const *U, mat *const *S, mat *const *V, int count, int k,
                          int q, uint64_t seed);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The computation of tsvd_frpca_csr on a device-resident matrix:
+ * values/cols/pointerB/pointerE are device arrays in mat_csr's 1-based
+ * layout, U (m x k), S (k, ascending) and V (n x k) device arrays
+ * (column-major) that the caller owns.  It calls no hipMalloc, hipFree or
+ * allocating library routine: every temporary lives in one static device
+ * array per GPU (about 45 MB, held until the process ends), so m <= 4096,
+ * n <= 32768 and k + 5 <= 136; larger sizes are rejected with TSVD_EINVAL.
+ * The matrix is checked on the device before use.  Calls in one process are
+ * serialised; each call leaves the array's contents behind, and nothing in
+ * it is assumed zero on entry.  With the same seed it draws the same sketch
+ * as tsvd_frpca_csr, so the two agree to rounding. */
+int tsvd_frpca_csr_noalloc(int32_t m, int32_t n, int32_t nnz, const double *values, const int32_t *cols,
+                           const int32_t *pointerB, const int32_t *pointerE, double *U, double *S, double *V,
+                           int32_t k, int32_t q, uint64_t seed);
+
 /* ------------------------------------------------------------- pipeline
  * The per-snapshot driver of Tree_u_NC.cpp:507-866, Tree_d_NC.cpp:611-1132,
  * Tree_u_LP.cpp:664-925 (the "plain GPU method"): ingest -> refresh -> push

@@ -25,3 +25,17 @@ def test_library_exports_every_symbol():
     L = ctypes.CDLL(capi.LIB_PATH)
     missing = [s for s in capi.EXPORTS if not hasattr(L, s)]
     assert not missing, missing
+
+
+def test_noalloc_frpca_rejects_sizes_beyond_its_workspace_before_any_device_call():
+    """tsvd_frpca_csr_noalloc checks its size limits (m <= 4096, n <= 32768,
+    k + 5 <= 136) before touching the device, so this runs without a GPU."""
+    L = ctypes.CDLL(capi.LIB_PATH)
+    L.tsvd_last_error.restype = ctypes.c_char_p
+    f = L.tsvd_frpca_csr_noalloc
+    f.argtypes = [ctypes.c_int32] * 3 + [ctypes.c_void_p] * 7 + [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64]
+    for m, n, k, what in [(4097, 100, 8, "rows"), (100, 32769, 8, "columns"), (100, 100, 132, "rank"),
+                          (0, 100, 8, "rows")]:
+        rc = f(m, n, 0, None, None, None, None, None, None, None, k, 6, 1)
+        assert rc == -1
+        assert what in L.tsvd_last_error().decode()

@@ -177,11 +177,22 @@ void launch_t(hipStream_t st, const GemmProb *d, int nprob, int64_t tiles, bool 
 
 void GemmRunner::run(hipStream_t st, std::vector<GemmProb> &probs, GemmTypes t, bool wide, const char *label)
 {
+    int nlive = 0;
+    for (auto &p : probs) nlive += (p.M > 0 && p.N > 0);
+    dprobs_.ensure((size_t)std::max(nlive, 1));
+    run_into(st, probs.data(), (int)probs.size(), t, wide, dprobs_.p, nlive, label);
+}
+
+void GemmRunner::run_into(hipStream_t st, GemmProb *probs, int nprobs, GemmTypes t, bool wide, GemmProb *dev,
+                          int cap, const char *label)
+{
     const int BN = wide ? 144 : 80;
     int64_t tiles = 0;
-    std::vector<GemmProb> live;
-    live.reserve(probs.size());
-    for (auto &p : probs) {
+    GemmProb live[64];
+    int np = 0;
+    TSVD_REQUIRE(cap >= 0, "bad descriptor capacity");
+    for (int i = 0; i < nprobs; i++) {
+        GemmProb &p = probs[i];
         if (p.M <= 0 || p.N <= 0) continue;
         TSVD_REQUIRE(p.K >= 0, "negative K");
         p.tiles_m = (int32_t)ceil_div(p.M, BM);
@@ -193,19 +204,28 @@ void GemmRunner::run(hipStream_t st, std::vector<GemmProb> &probs, GemmTypes t, 
         }
         p.tile_begin = tiles;
         tiles += (int64_t)p.tiles_m * p.tiles_n * p.tiles_k;
-        live.push_back(p);
+        TSVD_REQUIRE(np < cap, "more GEMM problems (%d) than descriptor slots (%d)", np + 1, cap);
+        // copied in chunks of 64 problems
+        live[np % 64] = p;
+        np++;
+        if (np % 64 == 0) {
+            TSVD_HIP(hipMemcpyAsync(dev + np - 64, live, sizeof(GemmProb) * 64, hipMemcpyHostToDevice, st));
+            // the staging array is reused for the next chunk
+            TSVD_HIP(hipStreamSynchronize(st));
+        }
     }
-    if (live.empty() || tiles == 0) return;
+    if (np == 0 || tiles == 0) return;
     TSVD_REQUIRE(tiles < (int64_t)INT32_MAX, "too many GEMM tiles");
-    dprobs_.ensure(live.size());
-    TSVD_HIP(hipMemcpyAsync(dprobs_.p, live.data(), sizeof(GemmProb) * live.size(), hipMemcpyHostToDevice, st));
-    const int np = (int)live.size();
-    // the staging vector must outlive the copy
+    if (np % 64 != 0)
+        TSVD_HIP(hipMemcpyAsync(dev + (np / 64) * 64, live, sizeof(GemmProb) * (np % 64), hipMemcpyHostToDevice, st));
+    // the staging array must outlive the copy
     TSVD_HIP(hipStreamSynchronize(st));
     if (g_profile) {
         auto sz = [](DT d) { return d == DT::F64 ? 8.0 : 4.0; };
         double flops = 0.0, bytes = 0.0;
-        for (const auto &p : live) {
+        for (int i = 0; i < nprobs; i++) {
+            const GemmProb &p = probs[i];
+            if (p.M <= 0 || p.N <= 0) continue;
             flops += 2.0 * p.M * p.N * p.K;
             bytes += (double)p.M * p.K * sz(t.a) + (double)p.K * p.N * sz(t.b) + (double)p.M * p.N * sz(t.c);
         }
@@ -214,13 +234,13 @@ void GemmRunner::run(hipStream_t st, std::vector<GemmProb> &probs, GemmTypes t, 
     PROF(label, st);
     if (t.acc == DT::F32) {
         TSVD_REQUIRE(t.a == DT::F32 && t.b == DT::F32 && t.c == DT::F32, "FP32 accumulate takes FP32 operands");
-        launch_t<float, float, float, float>(st, dprobs_.p, np, tiles, wide);
+        launch_t<float, float, float, float>(st, dev, np, tiles, wide);
     } else {
-        if (t.a == DT::F32 && t.b == DT::F32 && t.c == DT::F64) launch_t<float, float, double, double>(st, dprobs_.p, np, tiles, wide);
-        else if (t.a == DT::F32 && t.b == DT::F64 && t.c == DT::F32) launch_t<float, double, float, double>(st, dprobs_.p, np, tiles, wide);
-        else if (t.a == DT::F64 && t.b == DT::F64 && t.c == DT::F64) launch_t<double, double, double, double>(st, dprobs_.p, np, tiles, wide);
-        else if (t.a == DT::F32 && t.b == DT::F64 && t.c == DT::F64) launch_t<float, double, double, double>(st, dprobs_.p, np, tiles, wide);
-        else if (t.a == DT::F64 && t.b == DT::F64 && t.c == DT::F32) launch_t<double, double, float, double>(st, dprobs_.p, np, tiles, wide);
+        if (t.a == DT::F32 && t.b == DT::F32 && t.c == DT::F64) launch_t<float, float, double, double>(st, dev, np, tiles, wide);
+        else if (t.a == DT::F32 && t.b == DT::F64 && t.c == DT::F32) launch_t<float, double, float, double>(st, dev, np, tiles, wide);
+        else if (t.a == DT::F64 && t.b == DT::F64 && t.c == DT::F64) launch_t<double, double, double, double>(st, dev, np, tiles, wide);
+        else if (t.a == DT::F32 && t.b == DT::F64 && t.c == DT::F64) launch_t<float, double, double, double>(st, dev, np, tiles, wide);
+        else if (t.a == DT::F64 && t.b == DT::F64 && t.c == DT::F32) launch_t<double, double, float, double>(st, dev, np, tiles, wide);
         else TSVD_REQUIRE(false, "unsupported GEMM type combination");
     }
     TSVD_HIP(hipGetLastError());

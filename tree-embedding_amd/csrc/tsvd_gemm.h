@@ -34,6 +34,10 @@ struct GemmTypes {
 class GemmRunner {
 public:
     void run(hipStream_t st, std::vector<GemmProb> &probs, GemmTypes t, bool wide, const char *label = "gemm");
+    // The same, with the problem descriptors staged in a caller-provided
+    // device array of `cap` slots (allocates nothing).
+    void run_into(hipStream_t st, GemmProb *probs, int nprobs, GemmTypes t, bool wide, GemmProb *dev, int cap,
+                  const char *label = "gemm");
     int64_t launches = 0;
 
 private:

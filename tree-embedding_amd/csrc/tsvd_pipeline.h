@@ -29,6 +29,12 @@ private:
     DBuf<rocblas_int> info_;
 };
 
+// Hand-written batched FP64 symmetric eigensolver (jacobi.hip): one workgroup
+// per l x l matrix (l <= JACOBI_LMAX), same contract as Eig::solve
+// (eigenvectors over the input, eigenvalues ascending).  Allocates nothing.
+constexpr int JACOBI_LMAX = 136;
+void jacobi_eig(hipStream_t st, double *G, int l, int count, double *lam);
+
 // Column blocks of the |S| x n matrix (Tree_u_NC.cpp:396-432): common = n /
 // nparts, block of v = min(v / common, nparts-1), the last block takes the
 // remainder.  Block j's factored matrix is kept dense over its active columns

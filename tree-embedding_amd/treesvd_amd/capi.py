@@ -27,7 +27,7 @@ EXPORTS = [
     "tsvd_ppr_create", "tsvd_ppr_destroy", "tsvd_ppr_refresh", "tsvd_ppr_push", "tsvd_ppr_read",
     "tsvd_ppr_last_push_stats", "tsvd_ppr_set_two_sided", "tsvd_ppr_refresh_edges",
     "tsvd_ppr_get_push_setting", "tsvd_ppr_set_push_setting",
-    "frPCA", "tsvd_frpca_csr", "tsvd_frpca_csr_batch",
+    "frPCA", "tsvd_frpca_csr", "tsvd_frpca_csr_batch", "tsvd_frpca_csr_noalloc",
     "tsvd_pipeline_create", "tsvd_pipeline_destroy", "tsvd_pipeline_snapshot",
     "tsvd_pipeline_embedding", "tsvd_pipeline_root_sigma", "tsvd_pipeline_graph", "tsvd_pipeline_ppr",
     "tsvd_pipeline_block_matrix", "tsvd_pipeline_block_norms", "tsvd_pipeline_node", "tsvd_pipeline_block_columns",
@@ -141,6 +141,8 @@ def lib():
         if hasattr(L, "frPCA"):
             L.frPCA.restype = None
             L.frPCA.argtypes = [ctypes.POINTER(MatCSR), vp, vp, vp, ctypes.c_int, ctypes.c_int]
+        if hasattr(L, "tsvd_frpca_csr_noalloc"):
+            L.tsvd_frpca_csr_noalloc.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, ctypes.c_uint64]
         if hasattr(L, "tsvd_frpca_csr"):
             L.tsvd_frpca_csr.argtypes = [ctypes.POINTER(MatCSR), ctypes.POINTER(Mat), ctypes.POINTER(Mat),
                                          ctypes.POINTER(Mat), ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
@@ -474,6 +476,30 @@ def frpca_csr(A_dense_or_csr, k: int, q: int, seed: int = 1):
     (U, S, V), (mu, ms, mv) = _outputs(m, n, k)
     check(lib().tsvd_frpca_csr(ctypes.byref(csr), ctypes.byref(mu), ctypes.byref(ms), ctypes.byref(mv), k, q, seed))
     return U.T.copy(), S, V.T.copy()
+
+
+def frpca_csr_noalloc(A_dense_or_csr, k: int, q: int, seed: int = 1):
+    """tsvd_frpca_csr_noalloc: the matrix is staged in device arrays (1-based
+    CSR as mat_csr), factored with no allocation inside the call, and U, S, V
+    read back.  Returns U (m x k), S (k, ascending), V (n x k)."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A_dense_or_csr, dtype=np.float64)
+    A.sort_indices()
+    m, n = A.shape
+    vals = DeviceArray(A.data.astype(np.float64) if A.nnz else np.zeros(1))
+    cols = DeviceArray((A.indices + 1).astype(np.int32) if A.nnz else np.zeros(1, np.int32))
+    pb = DeviceArray((A.indptr[:-1] + 1).astype(np.int32))
+    pe = DeviceArray((A.indptr[1:] + 1).astype(np.int32))
+    U = DeviceArray(np.zeros(m * k))
+    S = DeviceArray(np.zeros(k))
+    V = DeviceArray(np.zeros(n * k))
+    try:
+        check(lib().tsvd_frpca_csr_noalloc(m, n, int(A.nnz), vals.ptr, cols.ptr, pb.ptr, pe.ptr, U.ptr, S.ptr, V.ptr,
+                                           k, q, ctypes.c_uint64(seed)))
+        return U.numpy().reshape(k, m).T.copy(), S.numpy().copy(), V.numpy().reshape(k, n).T.copy()
+    finally:
+        for x in (vals, cols, pb, pe, U, S, V):
+            x.free()
 
 
 def frpca_csr_batch(mats, k: int, q: int, seed: int = 1):
