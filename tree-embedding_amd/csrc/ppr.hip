@@ -123,6 +123,27 @@ __device__ inline void stage_flush(const Stage &sg)
     }
 }
 
+// Block-uniform drain of the staged items once more than `thresh` are
+// waiting (every thread of the workgroup calls it, at the same point of a
+// loop whose trip count is the same for the whole workgroup).
+__device__ inline void stage_drain(const Stage &sg, uint32_t thresh)
+{
+    __syncthreads();
+    const uint32_t waiting = *sg.n;
+    if (waiting <= thresh) return;
+    const uint32_t n = min(waiting, STAGE);
+    if (threadIdx.x == 0) *sg.base = atomicAdd(sg.count, n);
+    __syncthreads();
+    const uint32_t b = *sg.base;
+    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
+        if (b + t < sg.cap) sg.list[b + t] = sg.items[t];
+        else atomicOr(sg.overflow, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *sg.n = 0u;
+    __syncthreads();
+}
+
 #define TSVD_STAGE(name, list_, count_, cap_, overflow_)                                     \
     __shared__ uint32_t name##_items[STAGE];                                                 \
     __shared__ uint32_t name##_n, name##_base;                                               \
@@ -431,9 +452,19 @@ constexpr int HUB_CHUNK = 4096;   // default; TSVD_HUB_CHUNK overrides it per PP
 // wave's ballot of crossing lanes is OR-ed into the next round's mask of
 // (u, g) and the first setter appends the item.  Push counters are summed per
 // workgroup in LDS before one global update.
+//
+// The round runs on a fixed grid and reads its item count from the device
+// (*count_in, written by the previous round), so the host enqueues several
+// rounds without waiting: workgroups stride over the items, and the round
+// zeroes the counters that the round after it appends to (*zero_next, and
+// the other set of split-row counters, zero_hub[0..1]); *rounds counts the
+// rounds that had work.
 template <class T>
 __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__restrict__ R,
-                                                    const uint32_t *__restrict__ items, uint32_t nitems,
+                                                    const uint32_t *__restrict__ items,
+                                                    const uint32_t *__restrict__ count_in, uint32_t in_cap,
+                                                    uint32_t *__restrict__ zero_next,
+                                                    uint32_t *__restrict__ zero_hub, uint32_t *__restrict__ rounds,
                                                     unsigned long long *__restrict__ cm,
                                                     unsigned long long *__restrict__ nm, uint32_t *__restrict__ next,
                                                     uint32_t *__restrict__ next_count, uint32_t cap,
@@ -452,9 +483,17 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
     // [4] relaxations done here (split rows hand the rest to k_push_hub)
     __shared__ unsigned long long sstat[5];
     if (threadIdx.x < 5) sstat[threadIdx.x] = 0ull;
+    const uint32_t nitems = min(*count_in, in_cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *zero_next = 0u;
+        zero_hub[0] = 0u;
+        zero_hub[1] = 0u;
+        if (nitems > 0) atomicAdd(rounds, 1u);
+    }
     TSVD_STAGE(sg, next, next_count, cap, overflow);
-    const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
+    for (uint32_t wbase = blockIdx.x * WPB; wbase < nitems; wbase += gridDim.x * WPB) {
+    const uint32_t w = wbase + (threadIdx.x >> 6);
     if (w < nitems) {
         const uint32_t item = items[w];
         const int32_t v = (int32_t)(item / (uint32_t)ngroups);
@@ -544,6 +583,8 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
             relax_range(nb, 0, jend, pushing, delta, R + s, s_pad, deg, (uint32_t)ngroups, g, rm, two, lane, nm, sg);
         }
     }
+    stage_drain(sg, STAGE / 2);
+    }
     stage_flush(sg);
     if (threadIdx.x < 5 && sstat[threadIdx.x]) atomicAdd(stat_slot(stats) + threadIdx.x, sstat[threadIdx.x]);
 }
@@ -553,7 +594,8 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
 template <class T>
 __global__ void __launch_bounds__(TPB) k_push_hub(T *__restrict__ R, const uint32_t *__restrict__ hub_item,
                                                   const T *__restrict__ hub_delta,
-                                                  const unsigned long long *__restrict__ hub_chunks, uint32_t nchunks,
+                                                  const unsigned long long *__restrict__ hub_chunks,
+                                                  const uint32_t *__restrict__ chunk_count, uint32_t chunk_cap,
                                                   unsigned long long *__restrict__ nm, uint32_t *__restrict__ next,
                                                   uint32_t *__restrict__ next_count, uint32_t cap,
                                                   uint32_t *__restrict__ overflow,
@@ -562,8 +604,10 @@ __global__ void __launch_bounds__(TPB) k_push_hub(T *__restrict__ R, const uint3
                                                   int32_t s_pad, int32_t ngroups, T rm, int two, int32_t hub_chunk)
 {
     TSVD_STAGE(sg, next, next_count, cap, overflow);
-    const uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6);
+    const uint32_t nchunks = min(*chunk_count, chunk_cap);
     const int lane = threadIdx.x & 63;
+    for (uint32_t wbase = blockIdx.x * WPB; wbase < nchunks; wbase += gridDim.x * WPB) {
+    const uint32_t w = wbase + (threadIdx.x >> 6);
     if (w < nchunks) {
         const unsigned long long e = hub_chunks[w];
         const uint32_t slot = (uint32_t)(e >> 32), c = (uint32_t)(e & 0xFFFFFFFFull);
@@ -575,6 +619,8 @@ __global__ void __launch_bounds__(TPB) k_push_hub(T *__restrict__ R, const uint3
         const int32_t j0 = (int32_t)c * hub_chunk, j1 = min(dv, j0 + hub_chunk);
         relax_range(adj + rowoff[v], j0, j1, delta != T(0), delta, R + (int64_t)g * 64 + lane, s_pad, deg,
                     (uint32_t)ngroups, g, rm, two, lane, nm, sg);
+    }
+    stage_drain(sg, STAGE / 2);
     }
     stage_flush(sg);
 }
@@ -803,6 +849,39 @@ static void build_worklist(tsvd_ppr *s, int d)
     s->wl_version[d] = g->version;
 }
 
+// Workgroups of a device-driven round.  The host knows the item count of the
+// first round of each batch it enqueues, not of the later ones: the grid
+// gives that count 1.5x headroom at one item per wave (the shape of a round
+// launched for its exact count) and never less than 2 workgroups per compute
+// unit; a round that outgrows it strides over its items.
+static uint32_t cu_count()
+{
+    static int cus[64] = {};
+    int dev = 0;
+    TSVD_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (cus[dev] == 0) {
+        int c = 0;
+        TSVD_HIP(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
+        cus[dev] = std::max(c, 1);
+    }
+    return (uint32_t)cus[dev];
+}
+
+static uint32_t round_grid(uint32_t count_hint)
+{
+    const uint64_t want = (uint64_t)ceil_div((int64_t)count_hint + count_hint / 2, WPB);
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, (uint64_t)cu_count() * 2), 1u << 24);
+}
+
+// Rounds until no item is pending.  Push rounds are device-driven: each reads
+// its item count from a ring of three device counters (the count it was
+// given, the one it appends to, and the one it clears for the round after),
+// so the host enqueues ROUNDS_PER_SYNC rounds, then reads one count.  Pull
+// rounds (off unless TSVD_PULL_FRAC is set) are chosen by the frontier size,
+// so with them enabled the host reads the count after every round.
+constexpr int ROUNDS_PER_SYNC = 8;
+
 template <class T>
 void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
 {
@@ -814,111 +893,111 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
     uint32_t *h = s->h_counters;
     const uint32_t cap = (uint32_t)s->fnext.n;
     TSVD_HIP(hipMemsetAsync(s->stats.p, 0, s->stats.bytes(), st));
+    // ring slots cnt[8], cnt[9], split-row sets cnt[4..5] / cnt[6..7], rounds cnt[10]
+    TSVD_HIP(hipMemsetAsync(cnt + 4, 0, 7 * sizeof(uint32_t), st));
     TSVD_HIP(hipMemcpyAsync(h, cnt, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     TSVD_HIP(hipStreamSynchronize(st));
     TSVD_REQUIRE(h[3] == 0, "frontier overflow");
     uint32_t count = h[d];
     s->last_pull_rounds[d] = 0;
+    uint32_t *ring[3] = {cnt + d, cnt + 8, cnt + 9};
+    uint32_t *hubset[2] = {cnt + 4, cnt + 6};
+    const bool pull_enabled = !s->two_sided && s->ngroups <= 64 && s->pull_frac > 0.0;
+    const int per_sync = pull_enabled ? 1 : ROUNDS_PER_SYNC;
+    // split rows: at most one chunk list entry per (hub row, group) per hub_chunk neighbours
+    const uint32_t hub_cap = (uint32_t)std::min<int64_t>((int64_t)cap, (int64_t)s->ngroups *
+                                                         (ceil_div(D.cap_total, s->hub_chunk) + 1));
+    const uint32_t chunk_cap = hub_cap;
+    s->hub_item.ensure(std::max<uint32_t>(hub_cap, 1));
+    s->hub_delta.ensure((size_t)std::max<uint32_t>(hub_cap, 1) * 64 * sizeof(T));
+    s->hub_chunks.ensure(std::max<uint32_t>(chunk_cap, 1));
     Timer tm;
     tm.start(st);
-    int64_t rounds = 0;
-    unsigned long long prof_prev[3] = {0, 0, 0};
+    uint64_t i = 0;           // round index (selects the ring slots)
     while (count > 0) {
-        TSVD_HIP(hipMemsetAsync(cnt + 2, 0, sizeof(uint32_t), st));
-        // pull rounds implement the reference's positive-side threshold only
-        const bool pull = !s->two_sided && s->ngroups <= 64 && s->pull_frac > 0.0 &&
-                          (double)count > s->pull_frac * (double)s->n;
-        if (pull) {
-            // per-vertex pushed amounts and group masks of pull rounds, made on first use
-            if (s->pm.n == 0) {
-                s->delta.alloc((size_t)s->n * s->s_pad * s->esize);
-                s->pm.alloc((size_t)s->n);
-                TSVD_HIP(hipMemsetAsync(s->pm.p, 0, s->pm.bytes(), st));
+        const uint32_t grid = round_grid(count), hub_grid = cu_count() * 8;
+        for (int kk = 0; kk < per_sync; kk++, i++) {
+            uint32_t *in = ring[i % 3], *next = ring[(i + 1) % 3], *ahead = ring[(i + 2) % 3];
+            uint32_t *hub = hubset[i % 2], *hub_ahead = hubset[(i + 1) % 2];
+            // pull rounds implement the reference's positive-side threshold only
+            const bool pull = pull_enabled && (double)count > s->pull_frac * (double)s->n;
+            if (pull) {
+                // per-vertex pushed amounts and group masks of pull rounds, made on first use
+                if (s->pm.n == 0) {
+                    s->delta.alloc((size_t)s->n * s->s_pad * s->esize);
+                    s->pm.alloc((size_t)s->n);
+                    TSVD_HIP(hipMemsetAsync(s->pm.p, 0, s->pm.bytes(), st));
+                }
+                build_worklist(s, d);
+                TSVD_HIP(hipMemsetAsync(next, 0, sizeof(uint32_t), st));
+                TSVD_HIP(hipMemsetAsync(hub_ahead, 0, 2 * sizeof(uint32_t), st));
+                const GraphDir &Q = pull_dir(s, d);
+                {
+                    PROF("ppr_pull_pop", st);
+                    hipLaunchKernelGGL(k_pull_pop<T>, dim3((unsigned)ceil_div(count, WPB)), dim3(TPB), 0, st, P, R,
+                                       (T *)s->delta.p, s->pm.p, s->F[d].p, count, s->cm[d].p, D.deg.p, s->s_pad,
+                                       s->ngroups, s->alpha, (T)s->rmax, s->directed, s->stats.p);
+                }
+                {
+                    PROF("ppr_pull", st);
+                    const size_t shm = (size_t)WPB * s->ngroups * 64 * sizeof(T);
+                    hipLaunchKernelGGL(k_pull<T>, dim3((unsigned)ceil_div(s->wl_n[d], WPB)), dim3(TPB), shm, st, R,
+                                       (const T *)s->delta.p, s->pm.p, s->wl_u[d].p, s->wl_c[d].p,
+                                       (uint32_t)s->wl_n[d], Q.rowoff.p, Q.adj.p, Q.deg.p, D.deg.p, s->s_pad,
+                                       s->ngroups, (T)s->rmax, s->nm.p, s->fnext.p, next, cap, cnt + 3);
+                }
+                hipLaunchKernelGGL(k_pull_clear, dim3((unsigned)ceil_div(count, TPB)), dim3(TPB), 0, st, s->pm.p,
+                                   s->F[d].p, count, s->ngroups);
+                s->last_pull_rounds[d]++;
+            } else {
+                {
+                    PROF("ppr_push_round", st);
+                    hipLaunchKernelGGL(k_push_round<T>, dim3(grid), dim3(TPB), 0, st, P, R, s->F[d].p, in, cap,
+                                       ahead, hub_ahead, cnt + 10, s->cm[d].p, s->nm.p, s->fnext.p, next, cap,
+                                       cnt + 3, D.rowoff.p, D.adj.p, D.deg.p, s->s_pad, s->ngroups, s->alpha,
+                                       (T)s->rmax, s->two_sided, s->directed,
+                                       (T)(std::min(s->ride_frac, 1.0) * s->rmax), s0, s1, s->stats.p,
+                                       s->hub_item.p, (T *)s->hub_delta.p, s->hub_chunks.p, hub, hub_cap, chunk_cap,
+                                       s->hub_chunk);
+                }
+                {
+                    PROF("ppr_push_hub", st);
+                    hipLaunchKernelGGL(k_push_hub<T>, dim3(hub_grid), dim3(TPB), 0, st, R, s->hub_item.p,
+                                       (const T *)s->hub_delta.p, s->hub_chunks.p, hub + 1, chunk_cap, s->nm.p,
+                                       s->fnext.p, next, cap, cnt + 3, D.rowoff.p, D.adj.p, D.deg.p, s->s_pad,
+                                       s->ngroups, (T)s->rmax, s->two_sided, s->hub_chunk);
+                }
             }
-            build_worklist(s, d);
-            const GraphDir &Q = pull_dir(s, d);
-            {
-                PROF("ppr_pull_pop", st);
-                hipLaunchKernelGGL(k_pull_pop<T>, dim3((unsigned)ceil_div(count, WPB)), dim3(TPB), 0, st, P, R,
-                                   (T *)s->delta.p, s->pm.p, s->F[d].p, count, s->cm[d].p, D.deg.p, s->s_pad,
-                                   s->ngroups, s->alpha, (T)s->rmax, s->directed, s->stats.p);
-            }
-            {
-                PROF("ppr_pull", st);
-                const size_t shm = (size_t)WPB * s->ngroups * 64 * sizeof(T);
-                hipLaunchKernelGGL(k_pull<T>, dim3((unsigned)ceil_div(s->wl_n[d], WPB)), dim3(TPB), shm, st, R,
-                                   (const T *)s->delta.p, s->pm.p, s->wl_u[d].p, s->wl_c[d].p,
-                                   (uint32_t)s->wl_n[d], Q.rowoff.p, Q.adj.p, Q.deg.p, D.deg.p, s->s_pad,
-                                   s->ngroups, (T)s->rmax, s->nm.p, s->fnext.p, cnt + 2, cap, cnt + 3);
-            }
-            hipLaunchKernelGGL(k_pull_clear, dim3((unsigned)ceil_div(count, TPB)), dim3(TPB), 0, st, s->pm.p,
-                               s->F[d].p, count, s->ngroups);
-            s->last_pull_rounds[d]++;
-        } else {
-            // split rows: at most one chunk list entry per (hub row, group) per hub_chunk neighbours
-            const uint32_t hub_cap = (uint32_t)std::min<int64_t>((int64_t)cap, (int64_t)s->ngroups *
-                                                                 (ceil_div(D.cap_total, s->hub_chunk) + 1));
-            const uint32_t chunk_cap = hub_cap;
-            s->hub_item.ensure(std::max<uint32_t>(hub_cap, 1));
-            s->hub_delta.ensure((size_t)std::max<uint32_t>(hub_cap, 1) * 64 * sizeof(T));
-            s->hub_chunks.ensure(std::max<uint32_t>(chunk_cap, 1));
-            TSVD_HIP(hipMemsetAsync(cnt + 4, 0, 2 * sizeof(uint32_t), st));
-            {
-                PROF("ppr_push_round", st);
-                hipLaunchKernelGGL(k_push_round<T>, dim3((unsigned)ceil_div(count, WPB)), dim3(TPB), 0, st, P, R,
-                                   s->F[d].p, count, s->cm[d].p, s->nm.p, s->fnext.p, cnt + 2, cap, cnt + 3,
-                                   D.rowoff.p, D.adj.p, D.deg.p, s->s_pad, s->ngroups, s->alpha, (T)s->rmax,
-                                   s->two_sided, s->directed, (T)(std::min(s->ride_frac, 1.0) * s->rmax), s0, s1,
-                                   s->stats.p,
-                                   s->hub_item.p, (T *)s->hub_delta.p, s->hub_chunks.p,
-                                   cnt + 4, hub_cap, chunk_cap, s->hub_chunk);
-            }
-            TSVD_HIP(hipMemcpyAsync(h + 8, cnt + 4, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-            TSVD_HIP(hipStreamSynchronize(st));
-            const uint32_t nchunks = std::min(h[9], chunk_cap);
-            if (nchunks > 0) {
-                PROF("ppr_push_hub", st);
-                hipLaunchKernelGGL(k_push_hub<T>, dim3((unsigned)ceil_div(nchunks, WPB)), dim3(TPB), 0, st, R,
-                                   s->hub_item.p, (const T *)s->hub_delta.p, s->hub_chunks.p, nchunks, s->nm.p,
-                                   s->fnext.p, cnt + 2, cap, cnt + 3, D.rowoff.p, D.adj.p, D.deg.p, s->s_pad,
-                                   s->ngroups, (T)s->rmax, s->two_sided, s->hub_chunk);
-            }
+            TSVD_HIP(hipGetLastError());
+            std::swap(s->cm[d], s->nm);
+            std::swap(s->F[d], s->fnext);
         }
-        TSVD_HIP(hipGetLastError());
-        TSVD_HIP(hipMemcpyAsync(h + 2, cnt + 2, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        if (g_profile)
-            TSVD_HIP(hipMemcpyAsync(s->h_stats, s->stats.p, s->stats.bytes(), hipMemcpyDeviceToHost, st));
+        // the count of round i (the next one to run) and the overflow flag
+        TSVD_HIP(hipMemcpyAsync(h + 2, ring[i % 3], sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        TSVD_HIP(hipMemcpyAsync(h + 3, cnt + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         TSVD_HIP(hipStreamSynchronize(st));
-        if (g_profile) {
-            // algorithmic bytes per relaxation (column id + residue RMW) and per pop
-            // (row offsets, residue read/clear, reserve RMW); relaxations of split
-            // rows beyond their first chunk belong to k_push_hub
-            unsigned long long hs[STAT_FIELDS] = {};
-            for (int sh = 0; sh < STAT_SHARDS; sh++)
-                for (int f = 0; f < STAT_FIELDS; f++) hs[f] += s->h_stats[sh * STAT_FIELDS + f];
-            const double relax = (double)(hs[1] - prof_prev[1]), pops = (double)(hs[0] - prof_prev[0]);
-            const double own = pull ? relax : (double)(hs[4] - prof_prev[2]);
-            const double per_relax = sizeof(T) == 8 ? 20.0 : 12.0, per_pop = sizeof(T) == 8 ? 48.0 : 32.0;
-            prof_add_work(pull ? "ppr_pull" : "ppr_push_round", per_relax * own + per_pop * pops, 0.0);
-            if (relax > own) prof_add_work("ppr_push_hub", per_relax * (relax - own), 0.0);
-            prof_prev[0] = hs[0];
-            prof_prev[1] = hs[1];
-            prof_prev[2] = hs[4];
-        }
-        TSVD_REQUIRE(h[3] == 0, "frontier overflow in push round %lld", (long long)rounds);
-        std::swap(s->cm[d], s->nm);
-        std::swap(s->F[d], s->fnext);
+        TSVD_REQUIRE(h[3] == 0, "frontier overflow in push round %llu", (unsigned long long)i);
         count = h[2];
-        rounds++;
     }
     tm.stop(st);
     TSVD_HIP(hipMemsetAsync(cnt + d, 0, sizeof(uint32_t), st));
+    TSVD_HIP(hipMemcpyAsync(h + 10, cnt + 10, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     TSVD_HIP(hipMemcpyAsync(s->h_stats, s->stats.p, s->stats.bytes(), hipMemcpyDeviceToHost, st));
     TSVD_HIP(hipStreamSynchronize(st));
     unsigned long long hs[STAT_FIELDS] = {};
     for (int sh = 0; sh < STAT_SHARDS; sh++)
         for (int f = 0; f < STAT_FIELDS; f++) hs[f] += s->h_stats[sh * STAT_FIELDS + f];
-    s->last_rounds[d] = rounds;
+    if (g_profile) {
+        // algorithmic bytes per relaxation (column id + residue RMW) and per pop
+        // (row offsets, residue read/clear, reserve RMW); relaxations of split
+        // rows beyond their first chunk belong to k_push_hub (to the pull
+        // kernel in pull rounds)
+        const double per_relax = sizeof(T) == 8 ? 20.0 : 12.0, per_pop = sizeof(T) == 8 ? 48.0 : 32.0;
+        const double own = (double)hs[4], rest = (double)(hs[1] - hs[4]);
+        prof_add_work("ppr_push_round", per_relax * own + per_pop * (double)hs[0], 0.0);
+        if (rest > 0) prof_add_work(s->last_pull_rounds[d] ? "ppr_pull" : "ppr_push_hub", per_relax * rest, 0.0);
+    }
+    s->last_rounds[d] = (int64_t)h[10] + s->last_pull_rounds[d];
     s->last_pops[d] = (int64_t)hs[0];
     s->last_relax[d] = (int64_t)hs[1];
     s->last_items[d] = (int64_t)hs[2];
@@ -1151,7 +1230,7 @@ extern "C" int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t ns
             s->nm.alloc(items);
             s->fnext.alloc(items);
             TSVD_HIP(hipMemsetAsync(s->nm.p, 0, s->nm.bytes(), st));
-            s->counters.alloc(8);
+            s->counters.alloc(16);
             s->stats.alloc((size_t)STAT_SHARDS * STAT_FIELDS);
             TSVD_HIP(hipHostMalloc((void **)&s->h_stats, s->stats.bytes(), hipHostMallocDefault));
             TSVD_HIP(hipMemsetAsync(s->counters.p, 0, s->counters.bytes(), st));
