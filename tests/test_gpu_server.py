@@ -160,16 +160,35 @@ def test_served_overlap_matches_steps_one_at_a_time(capi):
     """Back parts of step t (frPCA, merges, root) overlap the front part of
     step t+1 on another stream; with the same steps (cap 1) the results equal
     those of a server that runs steps one at a time, within the push-order
-    tolerance, and requests complete in order."""
+    tolerance, and requests complete in order.  Two servers that both run one
+    step at a time already differ by their push order (float atomics), which
+    the NC transform log10(x / rmax) amplifies for reserves near rmax; it
+    shows in the last singular values at the rank-12 cut, which move by up to
+    4e-3 between identical runs (measured).  So sigma 1..10 are held to 2e-3,
+    the last two to 1e-2, the top half to 5e-4 and its subspace to 1e-5; the
+    overlapped run against the serial one and two serial runs against each
+    other must both meet those bars."""
     n, snaps, sub = workload(seed=11)
     outs = {}
-    for overlap in (True, False):
+    for key, overlap in (("overlap", True), ("serial", False), ("serial again", False)):
         srv = capi.Server(n, False, sub, max_step_requests=1, overlap=overlap, **ARGS)
         tickets = [srv.submit(s, d) for s, d in snaps]
         res = [srv.wait(t) for t in tickets]
         assert [r["snapshot"] for _, r in res] == list(range(1, len(snaps) + 1))
         assert all(r["done_ms"] >= r["start_ms"] for _, r in res)
-        outs[overlap] = [E for E, _ in res]
+        outs[key] = [E for E, _ in res]
         srv.close()
-    for t, (Ea, Eb) in enumerate(zip(outs[True], outs[False])):
-        same_embedding(Eb, Ea, ARGS["rank"], True, f"ticket {t}, overlapped vs one at a time")
+    k = ARGS["rank"]
+    h = k // 2
+    for other in ("overlap", "serial again"):
+        for t, (Ea, Eb) in enumerate(zip(outs[other], outs["serial"])):
+            sa, sb = sigma_of(Ea), sigma_of(Eb)
+            rel = np.abs(sa - sb) / sb
+            Ua, _ = np.linalg.qr(Ea)
+            Ub, _ = np.linalg.qr(Eb)
+            subsp = 1.0 - np.linalg.norm(Ub[:, :h].T @ Ua) ** 2 / h
+            msg = (f"ticket {t}, {other} vs one at a time: sigma rel {rel[:k - 2].max():.2e} (top half "
+                   f"{rel[:h].max():.2e}, last two {rel[k - 2:].max():.2e}), subspace {subsp:.2e}")
+            print(msg)
+            assert rel[:k - 2].max() < 2e-3 and rel[k - 2:].max() < 1e-2, msg
+            assert rel[:h].max() < TOL[True][1] and subsp < TOL[True][2], msg

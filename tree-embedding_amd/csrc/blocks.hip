@@ -8,7 +8,10 @@
 // pass over the reserves (p, or p + p^T for directed graphs, the matrix of
 // Tree_d_NC.cpp:689/816) computes ||A_new||_F^2 and ||A_new - A_old||_F^2 of
 // every block, reading the stored A_old through a per-vertex slot map; only
-// blocks found dirty are then re-gathered.  The stored matrix keeps only
+// blocks found dirty are then re-gathered.  Stored columns keep the reserves'
+// push-lane order (row s of a stored block is push lane s), so both passes
+// read and write contiguous 64-lane segments; the lane -> subset-row
+// permutation is applied where results leave the device.  The stored matrix keeps only
 // active columns (a vertex with an edge, or a source); every other column is
 // identically zero because its reserve is zero.
 #include "tsvd_pipeline.h"
@@ -38,7 +41,7 @@ __device__ inline double wave_sum(double v)
 template <class T>
 __global__ void __launch_bounds__(TPB) k_block_norms(const T *__restrict__ p0, const T *__restrict__ p1,
                                                      const T *__restrict__ A, const int32_t *__restrict__ slot,
-                                                     const int32_t *__restrict__ row_of, int32_t n, int32_t nsrc,
+                                                     int32_t n, int32_t nsrc,
                                                      int32_t s_pad, int32_t nparts, int32_t common, int transform,
                                                      T rm, double *__restrict__ norms)
 {
@@ -60,7 +63,7 @@ __global__ void __launch_bounds__(TPB) k_block_norms(const T *__restrict__ p0, c
             T x = x0[s];
             if (x1) x += x1[s];
             const T a = xform<T>(x, rm, transform);
-            const T ao = Ao ? Ao[row_of[s]] : T(0);
+            const T ao = Ao ? Ao[s] : T(0);
             a2 += (double)a * (double)a;
             const double d = (double)a - (double)ao;
             e2 += d * d;
@@ -130,7 +133,7 @@ template <class T>
 __global__ void __launch_bounds__(TPB) k_gather(const int32_t *__restrict__ blocks, const T *__restrict__ p0,
                                                 const T *__restrict__ p1, T *__restrict__ A,
                                                 const int32_t *__restrict__ colmap, const int32_t *__restrict__ ncols,
-                                                const int32_t *__restrict__ row_of, int32_t nsrc, int32_t s_pad,
+                                                int32_t nsrc, int32_t s_pad,
                                                 int32_t common, int transform, T rm)
 {
     const int32_t j = blocks[blockIdx.y];
@@ -149,7 +152,7 @@ __global__ void __launch_bounds__(TPB) k_gather(const int32_t *__restrict__ bloc
             if (x1) x += x1[s];
             a = xform<T>(x, rm, transform);
         }
-        col[row_of[s]] = a;
+        col[s] = a;
     }
 }
 
@@ -293,11 +296,11 @@ void blocks_norms(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g
     PROF("blocks_norms", st);
     if (b.precision == TSVD_FP64)
         hipLaunchKernelGGL(k_block_norms<double>, grid, dim3(TPB), 0, st, (const double *)p0, (const double *)p1,
-                           (const double *)b.A.p, b.slot.p, b.row_of.p, b.n, b.nsrc, b.s_pad, b.nparts, b.common,
+                           (const double *)b.A.p, b.slot.p, b.n, b.nsrc, b.s_pad, b.nparts, b.common,
                            b.transform, b.rmax, b.norms.p);
     else
         hipLaunchKernelGGL(k_block_norms<float>, grid, dim3(TPB), 0, st, (const float *)p0, (const float *)p1,
-                           (const float *)b.A.p, b.slot.p, b.row_of.p, b.n, b.nsrc, b.s_pad, b.nparts, b.common,
+                           (const float *)b.A.p, b.slot.p, b.n, b.nsrc, b.s_pad, b.nparts, b.common,
                            b.transform, (float)b.rmax, b.norms.p);
     TSVD_HIP(hipGetLastError());
     std::vector<double> h((size_t)2 * b.nparts);
@@ -327,11 +330,11 @@ void blocks_rebuild(Blocks &b, const std::vector<int32_t> &dirty, const void *p0
     }
     if (b.precision == TSVD_FP64)
         hipLaunchKernelGGL(k_gather<double>, grid, dim3(TPB), 0, st, b.d_dirty.p, (const double *)p0,
-                           (const double *)p1, (double *)b.A.p, b.colmap.p, b.d_ncols.p, b.row_of.p, b.nsrc,
+                           (const double *)p1, (double *)b.A.p, b.colmap.p, b.d_ncols.p, b.nsrc,
                            b.s_pad, b.common, b.transform, b.rmax);
     else
         hipLaunchKernelGGL(k_gather<float>, grid, dim3(TPB), 0, st, b.d_dirty.p, (const float *)p0,
-                           (const float *)p1, (float *)b.A.p, b.colmap.p, b.d_ncols.p, b.row_of.p, b.nsrc,
+                           (const float *)p1, (float *)b.A.p, b.colmap.p, b.d_ncols.p, b.nsrc,
                            b.s_pad, b.common, b.transform, (float)b.rmax);
     TSVD_HIP(hipGetLastError());
     TSVD_HIP(hipMemcpyAsync(b.ncols.data(), b.d_ncols.p, sizeof(int32_t) * b.nparts, hipMemcpyDeviceToHost, st));

@@ -539,8 +539,11 @@ extern "C" int tsvd_pipeline_embedding(tsvd_pipeline *p, double *E)
         const int32_t k = p->cfg.rank, ns = p->cfg.nsrc, sp = p->tree.s_pad;
         std::vector<double> h((size_t)sp * k);
         TSVD_HIP(hipMemcpy(h.data(), p->tree.E.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
-        for (int32_t i = 0; i < ns; i++)
-            for (int32_t c = 0; c < k; c++) E[(size_t)i * k + c] = h[(size_t)c * sp + i];
+        // device rows are push lanes; subset row i is lane lane_row(i)
+        for (int32_t i = 0; i < ns; i++) {
+            const size_t lane = (size_t)p->lane_row(i);
+            for (int32_t c = 0; c < k; c++) E[(size_t)i * k + c] = h[(size_t)c * sp + lane];
+        }
     });
 }
 
@@ -601,7 +604,7 @@ extern "C" int tsvd_pipeline_block_matrix(tsvd_pipeline *p, int32_t j, double *A
         for (int64_t c = 0; c < nc; c++) {
             const int64_t col = cm[c] - b.start[j];
             for (int32_t s = 0; s < b.nsrc; s++) {
-                const size_t e = (size_t)c * b.s_pad + s;
+                const size_t e = (size_t)c * b.s_pad + (size_t)p->lane_row(s);
                 A[(size_t)col * b.nsrc + s] = b.esize == 8 ? ((double *)h.data())[e] : (double)((float *)h.data())[e];
             }
         }
@@ -651,7 +654,7 @@ extern "C" int tsvd_pipeline_node(tsvd_pipeline *p, int32_t level, int32_t index
                            hipMemcpyDeviceToHost));
         for (int32_t c = 0; c < t.k; c++)
             for (int32_t s = 0; s < t.nsrc; s++) {
-                const size_t e = (size_t)c * t.s_pad + s;
+                const size_t e = (size_t)c * t.s_pad + (size_t)p->lane_row(s);
                 out[(size_t)c * t.nsrc + s] = t.esize == 8 ? ((double *)h.data())[e] : (double)((float *)h.data())[e];
             }
     });
@@ -770,7 +773,7 @@ extern "C" int tsvd_pipeline_lp_scores(tsvd_pipeline *p, const int32_t *src, con
             TSVD_REQUIRE(it != p->re.row_of.end(), "pair %lld: source %d is not a subset vertex", (long long)e,
                          src[e]);
             TSVD_REQUIRE(dst[e] >= 0 && dst[e] < p->cfg.n, "pair %lld: target %d out of range", (long long)e, dst[e]);
-            rows[(size_t)e] = it->second;
+            rows[(size_t)e] = p->lane_row(it->second);   // the root U rows are push lanes
         }
         hipStream_t st = p->g->stream;
         DBuf<int32_t> d_rows((size_t)m), d_dst((size_t)m);

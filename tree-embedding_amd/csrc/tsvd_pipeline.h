@@ -52,7 +52,9 @@ struct Blocks {
     DBuf<uint8_t> active, is_source;        // n
     DBuf<double> norms;                     // 2 * nparts: ||A_new||^2, ||A_new - A_old||^2
     DBuf<int32_t> d_dirty;                  // dirty block ids
-    DBuf<int32_t> row_of;                   // s_pad: matrix row of push lane s (a permutation of the sources)
+    DBuf<int32_t> row_of;                   // s_pad: subset row of push lane s (a permutation of the
+                                            // sources); stored blocks keep lane order, the full-matrix
+                                            // slabs of transform_chunk are in subset-row order
     std::vector<double> a2, e2, resid;
 };
 
@@ -217,6 +219,9 @@ struct tsvd_pipeline {
     tsvd::SparseState sp;                    // sparse_state pipelines only
     bool order_set = false;                  // push lanes ordered (at the first static snapshot)
     std::vector<int32_t> lane_of;            // subset row -> push lane (empty: the same)
+    // Stored blocks, tree nodes and the root keep push-lane row order; this is
+    // the device row that holds subset row i.
+    int32_t lane_row(int32_t i) const { return lane_of.empty() ? i : lane_of[(size_t)i]; }
     int32_t snapshot = 0;
     std::vector<int32_t> last_dirty;
     ~tsvd_pipeline();
