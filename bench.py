@@ -6,12 +6,16 @@ an edge-insertion batch, already resident in HBM, goes through ingest ->
 dynamic PPR update -> push -> block transform + lazy dirty test -> blockwise
 frPCA of the dirty blocks -> dirty-path merges -> root embedding.  The initial
 snapshot (static push over the first 60% of the edges) is setup and is timed
-separately.  Then the served form is compared with the plain method on one
-request stream (the rest of the graph arriving as lognormal 10k-edge batches
-at Poisson times): each replays the stream on its own clock against a server,
-plain admitting one request per snapshot, served admitting every queued
-request into the next snapshot.  Output: one JSON line (snapshots/s, p50/p99
-ms per update, served vs plain, the dominant kernel's roofline, and the CPU
+separately.  When warm-up + steps exceed the workload's insertion batches the
+batches replay from a rebuilt initial state (outside the clock).  Then the
+served form is compared with the plain method on request streams (the rest
+of the graph arriving as lognormal 10k-edge batches at Poisson times): each
+replays a stream on its own clock against a server, plain admitting one
+request per snapshot, served admitting every queued request into the next
+snapshot; at a rate above both capacities (capacities compared), at half the
+plain method's capacity (steady-state p99 compared), and at saturation with
+the two-sided threshold.  Output: one JSON line (snapshots/s, p50/p99 ms per
+update, served vs plain, the dominant kernel's roofline, and the CPU
 restatement timed on this host).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batched]
@@ -240,18 +244,22 @@ def accuracy_report(pipe, w, seed=7):
             "seconds": {"oracle": round(orac.seconds, 1), "reference_algorithm": round(t_ref, 1)}}
 
 
-def served_run(capi, w, reqs, arrivals, cap, label, seed=7, overlap=True, push_setting=None):
+def served_run(capi, w, reqs, arrivals, cap, label, seed=7, overlap=True, push_setting=None, two_sided=False,
+               window=None):
     """Replays a request stream against a server on its own clock: a feeder
     thread submits request i at arrivals[i] seconds, this thread collects the
     results in order.  cap = 1 is the plain per-snapshot method (steps one at
     a time), cap = 0 the served form (every queued request admitted into the
-    next step, the back part of each step overlapping the next step's push)."""
+    next step, the back part of each step overlapping the next step's push).
+    Latency percentiles cover the requests that arrive inside `window`
+    (seconds from the first arrival; default: all of them).  Returns the
+    metrics and the root singular values of the final state."""
     import threading
     srv = capi.Server(w.n, w.directed, w.subset, w.alpha, w.rmax, w.nparts, w.fanout, w.rank, w.q,
                       precision=capi.TSVD_FP64 if w.fp64_state else capi.TSVD_FP32,
                       transform=capi.TSVD_TRANSFORM_NC if w.transform == "NC" else capi.TSVD_TRANSFORM_LP,
                       delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed, max_step_requests=cap,
-                      sparse_state=w.sparse_state, overlap=overlap and cap != 1)
+                      sparse_state=w.sparse_state, overlap=overlap and cap != 1, two_sided=two_sided)
     t = time.perf_counter()
     if push_setting is not None:          # the plain run's autotuned push setting, for both replays
         srv.pipeline.ppr.push_setting = push_setting
@@ -284,14 +292,17 @@ def served_run(capi, w, reqs, arrivals, cap, label, seed=7, overlap=True, push_s
             log(f"{label}: step {r['snapshot']} done: {r['step_requests']} requests, {r['step_edges']} edges, "
                 f"{r['done_ms'] - r['start_ms']:.0f} ms, latency of request {i}: {r['latency_ms']:.0f} ms")
     th.join()
+    sigma = srv.pipeline.root_sigma()
     srv.close()
-    lat = np.array([r["latency_ms"] for r in res])
+    inside = [i for i in range(len(res)) if window is None or window[0] <= arrivals[i] <= window[1]]
+    lat = np.array([res[i]["latency_ms"] for i in inside])
     span = (max(r["done_ms"] for r in res) - min(r["submit_ms"] for r in res)) / 1e3
     steps = sorted({r["snapshot"] for r in res})
-    return {"snapshots_per_s": round(len(res) / span, 4), "p50_ms": round(float(np.percentile(lat, 50)), 1),
-            "p99_ms": round(float(np.percentile(lat, 99)), 1), "mean_ms": round(float(lat.mean()), 1),
+    return {"requests_per_s": round(len(res) / span, 4), "device_snapshots_per_s": round(len(steps) / span, 4),
+            "p50_ms": round(float(np.percentile(lat, 50)), 1), "p99_ms": round(float(np.percentile(lat, 99)), 1),
+            "mean_ms": round(float(lat.mean()), 1), "latency_requests": len(inside),
             "steps": len(steps), "requests_per_step": round(len(res) / len(steps), 2),
-            "makespan_s": round(span, 2)}
+            "makespan_s": round(span, 2)}, sigma
 
 
 def main():
@@ -316,6 +327,8 @@ def main():
     ap.add_argument("--served-rate", type=float, default=12.0, help="Poisson arrival rate of that stream (1/s)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the push autotuner on the first warm-up snapshot")
+    ap.add_argument("--served-two-sided", action=argparse.BooleanOptionalAction, default=True,
+                    help="also compare at saturation with the two-sided push threshold")
     ap.add_argument("--served-no-overlap", action="store_true",
                     help="run the served form's steps one at a time (no back/front overlap)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -454,10 +467,15 @@ def main():
     # the timed steps is set against what a pure stream of such atomics
     # reaches on this device
     wave_atomics = sum(s["push_wave_relax"] for s in stats)
+    line_atomics = sum(s["push_line_relax"] for s in stats)
     ceiling = capi.probe_atomic_rate()
+    probe_noret = capi.probe_atomic_lanes(returning=False, lanes=64)
+    probe_line = capi.probe_atomic_lanes(returning=True, lanes=32)     # one 128-byte line per instruction
     achieved_rate = wave_atomics / (push_ms * 1e-3) if push_ms > 0 else 0.0
+    line_rate = line_atomics / (push_ms * 1e-3) if push_ms > 0 else 0.0
     log(f"push: {wave_atomics:.3e} wave-atomics in {push_ms:.0f} ms = {achieved_rate / 1e9:.2f} G/s; "
-        f"probe ceiling {ceiling / 1e9:.2f} G/s")
+        f"probe ceiling {ceiling / 1e9:.2f} G/s (non-returning {probe_noret / 1e9:.2f}); "
+        f"{line_rate / 1e9:.2f} G lines/s against {probe_line / 1e9:.2f} G lines/s for one-line atomics")
     ndirty = float(np.mean([s["ndirty"] for s in stats]))
     nmerged = float(np.mean([s["nmerged"] for s in stats]))
     phases = {k: round(float(np.mean([s[k] for s in stats])), 3)
@@ -494,8 +512,15 @@ def main():
         "push_tuning": tuning,
         "push_atomic_roofline": {"achieved": round(achieved_rate / 1e9, 3), "ceiling": round(ceiling / 1e9, 3),
                                  "unit": "G wave-atomics/s", "frac": round(achieved_rate / ceiling, 4),
+                                 "ceiling_non_returning": round(probe_noret / 1e9, 3),
+                                 "lines": {"achieved": round(line_rate / 1e9, 3),
+                                           "ceiling": round(probe_line / 1e9, 3), "unit": "G lines/s",
+                                           "frac": round(line_rate / probe_line, 4)},
                                  "note": "plain-method push over the timed steps (round + hub kernels); ceiling = "
-                                         "returning 64-lane FP32 atomics at hashed rows (tsvd_probe_atomic_rate)"},
+                                         "64-lane FP32 atomics at hashed 256-byte rows (tsvd_probe_atomic_rate; "
+                                         "returning and non-returning run at the same rate); lines = 128-byte "
+                                         "lines the relaxation atomics touch against 32-lane (one-line) atomics, "
+                                         "the unit the L2 atomic pipes are paced by"},
     }
     if args.accuracy:
         log("accuracy against the dense FP64 oracle ...")
@@ -511,22 +536,54 @@ def main():
                                                                      for s, d in w.snapshots[1:]])
         # request sizes of the workload's stream: median 100k at the 1.47e9-edge shape, 10k otherwise
         med = 100_000 if args.workload == "atscale" else 10_000
-        reqs, arr = gen.request_stream(w, args.served_requests, args.served_rate, seed=11,
-                                       median=med / (1 << args.scale_down))
-        log(f"served vs plain: {len(reqs)} requests ({sum(s.size for s, _ in reqs)} edges, lognormal median "
-            f"{med / (1 << args.scale_down):.0f}) at {args.served_rate}/s Poisson")
         ps = (tuning["chosen"]["hub_chunk"], tuning["chosen"]["ride_frac"]) if tuning else None
-        plain = served_run(capi, w, reqs, arr, 1, "plain (one snapshot per request)", push_setting=ps)
-        served = served_run(capi, w, reqs, arr, 0, "served (continuous batching)",
-                            overlap=not args.served_no_overlap, push_setting=ps)
+
+        def compare(name, rate, count, two_sided, window_frac=None):
+            """One stream replayed by the plain method and by the served form."""
+            reqs, arr = gen.request_stream(w, count, rate, seed=11, median=med / (1 << args.scale_down))
+            window = None
+            if window_frac is not None:   # latencies of the arrivals after the first warm-up part
+                window = (window_frac * arr[-1], arr[-1])
+            log(f"served vs plain [{name}]: {len(reqs)} requests ({sum(s.size for s, _ in reqs)} edges, lognormal "
+                f"median {med / (1 << args.scale_down):.0f}) at {rate:.2f}/s Poisson, "
+                f"{'two-sided' if two_sided else 'reference'} threshold")
+            plain, s_plain = served_run(capi, w, reqs, arr, 1, f"plain [{name}]", push_setting=ps,
+                                        two_sided=two_sided, window=window)
+            served, s_served = served_run(capi, w, reqs, arr, 0, f"served [{name}]",
+                                          overlap=not args.served_no_overlap, push_setting=ps,
+                                          two_sided=two_sided, window=window)
+            gap = np.abs(s_served - s_plain) / np.maximum(s_plain, 1e-300)
+            return {"rate_per_s": round(rate, 3), "requests": len(reqs), "edges": int(sum(s.size for s, _ in reqs)),
+                    "threshold": "two-sided |r| > rmax" if two_sided else "reference r > rmax (ppr:1445)",
+                    "latency_window_s": None if window is None else [round(window[0], 2), round(window[1], 2)],
+                    "plain": plain, "served": served,
+                    "throughput_ratio": round(served["requests_per_s"] / plain["requests_per_s"], 2),
+                    "p99_ratio": round(served["p99_ms"] / plain["p99_ms"], 4),
+                    "final_sigma_served_vs_plain": {"top16_max_rel": float(gap[:16].max()),
+                                                    "all_max_rel": float(gap.max())}}
+
+        rows = {}
+        # capacity: an arrival rate above what either method sustains, so each
+        # runs flat out and its requests/s is its capacity (the p99 of an
+        # overloaded queue grows with the stream's length, so it is not a
+        # steady-state figure)
+        rows["saturation"] = compare("saturation", args.served_rate, args.served_requests, False)
+        plain_cap = rows["saturation"]["plain"]["requests_per_s"]
+        # steady state: half the plain method's capacity, latencies of the
+        # arrivals after the first quarter of the stream (both queues stable)
+        steady_rate = 0.5 * plain_cap
+        rows["steady"] = compare("steady", steady_rate, max(12, args.served_requests // 2), False, window_frac=0.25)
+        if args.served_two_sided:
+            rows["saturation_two_sided"] = compare("saturation, two-sided", args.served_rate, args.served_requests,
+                                                   True)
         out["served_vs_plain"] = {
-            "stream": {"requests": len(reqs), "edges": int(sum(s.size for s, _ in reqs)),
-                       "rate_per_s": args.served_rate,
-                       "sizes": f"lognormal median {med // 1000}k, sigma 1, clip [1k, 1M]",
-                       "graph": f"{args.workload} workload after its initial snapshot"},
-            "plain": plain, "served": served, "served_overlap": not args.served_no_overlap,
-            "throughput_ratio": round(served["snapshots_per_s"] / plain["snapshots_per_s"], 2),
-            "p99_ratio": round(served["p99_ms"] / plain["p99_ms"], 4)}
+            "stream": {"sizes": f"lognormal median {med // 1000}k, sigma 1, clip [1k, 1M]",
+                       "graph": f"{args.workload} workload after its initial snapshot", "arrivals": "Poisson"},
+            "served_overlap": not args.served_no_overlap, **rows,
+            "note": ("requests_per_s = requests completed / makespan; device_snapshots_per_s = pipeline steps / "
+                     "makespan; saturation compares capacities, steady compares p99 at a rate both sustain; "
+                     "final_sigma_served_vs_plain compares the root singular values after the whole stream, "
+                     "grouped one request per step (plain) or as the served form batched them")}
     # kernel events cover every launch of this run, as a rocprofv3 trace of the
     # same command does
     capi.profile_enable(False)

@@ -56,6 +56,12 @@ int tsvd_profile_read(char *json, int len);
  * `rows` x 256 B array, `iters` per wavefront.  Returns wave-level atomic
  * instructions per second (each updates 64 consecutive floats). */
 int tsvd_probe_atomic_rate(int64_t rows, int32_t iters, double *wave_atomics_per_s);
+/* The same probe with the returning (returning = 1) or non-returning form of
+ * the atomic and only the first `lanes` lanes of each wavefront taking part
+ * (32 lanes = one 128-byte line): the L2 atomic units are paced by the lines
+ * an instruction touches, not by its active lanes. */
+int tsvd_probe_atomic_lanes(int64_t rows, int32_t iters, int32_t returning, int32_t lanes,
+                            double *wave_atomics_per_s);
 
 /* ------------------------------------------------------------------ graph
  * Replaces UGraph::initializeDynamicUGraph / inputDynamicGraph
@@ -229,6 +235,8 @@ typedef struct {
     int64_t state_entries;    /* sparse_state: (vertex, value) pairs stored, all lists */
     int32_t state_batches;    /* sparse_state: source batches pushed per snapshot (0: one
                                * resident batch, the state kept in HBM without lists)  */
+    int64_t push_line_relax;  /* 128-byte lines the relaxation atomics touched: neighbour
+                               * visits x lines of the group holding a pushing lane    */
 } tsvd_snapshot_stats;
 
 int tsvd_pipeline_create(const tsvd_config *cfg, tsvd_pipeline **out);

@@ -212,6 +212,7 @@ static void sparse_front(tsvd_pipeline *p, int32_t it, const int32_t *dsrc, cons
             S.push_relax += gp->last_relax[d];
             S.push_items += gp->last_items[d];
             S.push_wave_relax += gp->last_wave_relax[d];
+            S.push_line_relax += gp->last_line_relax[d];
         }
         const double t2 = now_ms();
         for (int d = 0; d < 2 && !ss.resident; d++) {
@@ -369,6 +370,7 @@ bool pipeline_front(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, in
             S.push_relax += p->ppr->last_relax[d];
             S.push_items += p->ppr->last_items[d];
             S.push_wave_relax += p->ppr->last_wave_relax[d];
+            S.push_line_relax += p->ppr->last_line_relax[d];
         }
         const double t3 = now_ms();
         S.ms_push = t3 - t2;

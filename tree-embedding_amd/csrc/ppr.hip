@@ -480,9 +480,11 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                                                     uint32_t chunk_cap, int32_t hub_chunk)
 {
     // [0] pops, [1] relaxations (lanes x degree), [2] items, [3] row visits,
-    // [4] relaxations done here (split rows hand the rest to k_push_hub)
-    __shared__ unsigned long long sstat[5];
-    if (threadIdx.x < 5) sstat[threadIdx.x] = 0ull;
+    // [4] relaxations done here (split rows hand the rest to k_push_hub),
+    // [5] 128-byte lines the row's atomics touch (row visits x lines holding
+    //     a pushing lane: the L2 atomic units work per line, not per lane)
+    __shared__ unsigned long long sstat[6];
+    if (threadIdx.x < 6) sstat[threadIdx.x] = 0ull;
     const uint32_t nitems = min(*count_in, in_cap);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *zero_next = 0u;
@@ -550,6 +552,11 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                 atomicAdd(&sstat[1], (unsigned long long)__popcll(pm) * (unsigned long long)dv);
                 atomicAdd(&sstat[2], 1ull);
                 atomicAdd(&sstat[3], (unsigned long long)dv);
+                constexpr int per_line = 128 / (int)sizeof(T);   // lanes per 128-byte line
+                unsigned lines = 0;
+                for (int q = 0; q < 64; q += per_line)
+                    lines += ((pm >> q) & ((1ull << per_line) - 1ull)) != 0ull;
+                atomicAdd(&sstat[5], (unsigned long long)lines * (unsigned long long)dv);
             }
             const T delta = pushing ? (T)((1.0 - alpha) * (double)x / (double)dv) : T(0);
             const int32_t *nb = adj + rowoff[v];
@@ -586,7 +593,7 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
     stage_drain(sg, STAGE / 2);
     }
     stage_flush(sg);
-    if (threadIdx.x < 5 && sstat[threadIdx.x]) atomicAdd(stat_slot(stats) + threadIdx.x, sstat[threadIdx.x]);
+    if (threadIdx.x < 6 && sstat[threadIdx.x]) atomicAdd(stat_slot(stats) + threadIdx.x, sstat[threadIdx.x]);
 }
 
 
@@ -1002,6 +1009,7 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
     s->last_relax[d] = (int64_t)hs[1];
     s->last_items[d] = (int64_t)hs[2];
     s->last_wave_relax[d] = (int64_t)hs[3];
+    s->last_line_relax[d] = (int64_t)hs[5];
     s->last_ms[d] = tm.ms();
 }
 
@@ -1362,43 +1370,64 @@ __device__ inline uint32_t probe_hash(uint32_t x)
 
 // Each wavefront adds 1 to `iters` rows at hashed indices with one returning
 // 64-lane atomic per row (the relaxation's access, without the graph).
+// One 64-lane FP32 atomic add per iteration at a hashed 256-byte row; RET:
+// the returning form (as the push uses, to see threshold crossings) or the
+// non-returning one; only lanes below `lanes` take part (32 lanes = one
+// 128-byte line).
+template <bool RET>
 __global__ void __launch_bounds__(TPB) k_probe_atomic(float *__restrict__ R, uint32_t rows, int32_t iters,
-                                                      float *__restrict__ sink)
+                                                      int32_t lanes, float *__restrict__ sink)
 {
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     float acc = 0.f;
-    for (int32_t i = 0; i < iters; i++) {
-        const uint32_t row = probe_hash(wave * 7919u + (uint32_t)i) % rows;
-        acc += atomicAdd(R + (size_t)row * 64 + lane, 1.0f);
+    if (lane < lanes) {
+        for (int32_t i = 0; i < iters; i++) {
+            const uint32_t row = probe_hash(wave * 7919u + (uint32_t)i) % rows;
+            if (RET) acc += atomicAdd(R + (size_t)row * 64 + lane, 1.0f);
+            else atomicAdd(R + (size_t)row * 64 + lane, 1.0f);
+        }
     }
-    if (acc < 0.f) sink[0] = acc;   // keeps the returned values live
+    if (RET && acc < 0.f) sink[0] = acc;   // keeps the returned values live
 }
 
 }  // namespace
 }  // namespace tsvd
 
+static void probe_atomics(int64_t rows, int32_t iters, int returning, int32_t lanes, double *wave_atomics_per_s)
+{
+    TSVD_REQUIRE(wave_atomics_per_s && rows >= 1 && rows <= (int64_t)1 << 26 && iters >= 1 && iters <= 4096,
+                 "rows in [1, 2^26], iters in [1, 4096]");
+    TSVD_REQUIRE(lanes >= 1 && lanes <= 64, "lanes in [1, 64]");
+    DBuf<float> R((size_t)rows * 64), sink(1);
+    TSVD_HIP(hipMemset(R.p, 0, R.bytes()));
+    const int blocks = 8192;
+    Timer t;
+    double best = 0.0;
+    for (int rep = 0; rep < 3; rep++) {
+        t.start(nullptr);
+        if (returning)
+            hipLaunchKernelGGL(k_probe_atomic<true>, dim3(blocks), dim3(TPB), 0, nullptr, R.p, (uint32_t)rows,
+                               iters, lanes, sink.p);
+        else
+            hipLaunchKernelGGL(k_probe_atomic<false>, dim3(blocks), dim3(TPB), 0, nullptr, R.p, (uint32_t)rows,
+                               iters, lanes, sink.p);
+        TSVD_HIP(hipGetLastError());
+        t.stop(nullptr);
+        const double ms = t.ms();
+        const double waves = (double)blocks * TPB / 64;
+        best = std::max(best, waves * iters / (ms * 1e-3));
+    }
+    *wave_atomics_per_s = best;
+}
+
 extern "C" int tsvd_probe_atomic_rate(int64_t rows, int32_t iters, double *wave_atomics_per_s)
 {
-    using namespace tsvd;
-    return guard([&] {
-        TSVD_REQUIRE(wave_atomics_per_s && rows >= 1 && rows <= (int64_t)1 << 26 && iters >= 1 && iters <= 4096,
-                     "rows in [1, 2^26], iters in [1, 4096]");
-        DBuf<float> R((size_t)rows * 64), sink(1);
-        TSVD_HIP(hipMemset(R.p, 0, R.bytes()));
-        const int blocks = 8192;
-        Timer t;
-        double best = 0.0;
-        for (int rep = 0; rep < 3; rep++) {
-            t.start(nullptr);
-            hipLaunchKernelGGL(k_probe_atomic, dim3(blocks), dim3(TPB), 0, nullptr, R.p, (uint32_t)rows, iters,
-                               sink.p);
-            TSVD_HIP(hipGetLastError());
-            t.stop(nullptr);
-            const double ms = t.ms();
-            const double waves = (double)blocks * TPB / 64;
-            best = std::max(best, waves * iters / (ms * 1e-3));
-        }
-        *wave_atomics_per_s = best;
-    });
+    return tsvd::guard([&] { probe_atomics(rows, iters, 1, 64, wave_atomics_per_s); });
+}
+
+extern "C" int tsvd_probe_atomic_lanes(int64_t rows, int32_t iters, int32_t returning, int32_t lanes,
+                                       double *wave_atomics_per_s)
+{
+    return tsvd::guard([&] { probe_atomics(rows, iters, returning, lanes, wave_atomics_per_s); });
 }

@@ -20,8 +20,10 @@ struct tsvd_ppr {
     tsvd::DBuf<uint8_t> P[2], R[2];
     tsvd::DBuf<unsigned long long> cm[2], nm;
     tsvd::DBuf<uint32_t> F[2], fnext;
-    tsvd::DBuf<uint32_t> counters;           // [0],[1] pending per dir, [2] next, [3] overflow,
-                                             // [4] split rows, [5] their further chunks
+    tsvd::DBuf<uint32_t> counters;           // [0],[1] pending per dir, [3] overflow, [4..5] and
+                                             // [6..7] split rows and their further chunks (two
+                                             // sets, alternating rounds), [8],[9] the round-count
+                                             // ring with [dir], [10] rounds that had work
     tsvd::DBuf<uint32_t> hub_item;           // split rows of a push round: item,
     tsvd::DBuf<uint8_t> hub_delta;           //   its 64 pushed amounts,
     tsvd::DBuf<unsigned long long> hub_chunks;  //   (slot, chunk) work entries
@@ -29,7 +31,7 @@ struct tsvd_ppr {
     uint32_t *h_counters = nullptr;          // pinned
     unsigned long long *h_stats = nullptr;   // pinned copy of the sharded push counters
     int64_t last_rounds[2] = {0, 0}, last_pops[2] = {0, 0}, last_relax[2] = {0, 0};
-    int64_t last_items[2] = {0, 0}, last_wave_relax[2] = {0, 0};
+    int64_t last_items[2] = {0, 0}, last_wave_relax[2] = {0, 0}, last_line_relax[2] = {0, 0};
     double last_ms[2] = {0, 0};
     int64_t last_pull_rounds[2] = {0, 0};
     // pull rounds: per-vertex pushed amounts, popped-group masks, and the

@@ -34,7 +34,7 @@ EXPORTS = [
     "tsvd_pipeline_dense_gram", "tsvd_pipeline_gram_apply", "tsvd_pipeline_right_embedding",
     "tsvd_pipeline_lp_scores", "tsvd_pipeline_source_state", "tsvd_pipeline_snapshot_tuned",
     "tsvd_server_create", "tsvd_server_destroy", "tsvd_server_submit", "tsvd_server_wait",
-    "tsvd_server_pipeline", "tsvd_server_edge_budget", "tsvd_probe_atomic_rate", "tsvd_gen_rmat",
+    "tsvd_server_pipeline", "tsvd_server_edge_budget", "tsvd_probe_atomic_rate", "tsvd_probe_atomic_lanes", "tsvd_gen_rmat",
     "tsvd_gen_chung_lu",
 ]
 
@@ -71,7 +71,8 @@ class SnapshotStats(ctypes.Structure):
                 ("push_wave_relax", ctypes.c_int64), ("ms_total", ctypes.c_double), ("ms_ingest", ctypes.c_double),
                 ("ms_refresh", ctypes.c_double), ("ms_push", ctypes.c_double), ("ms_blocks", ctypes.c_double),
                 ("ms_frpca", ctypes.c_double), ("ms_merge", ctypes.c_double),
-                ("state_entries", ctypes.c_int64), ("state_batches", ctypes.c_int32)]
+                ("state_entries", ctypes.c_int64), ("state_batches", ctypes.c_int32),
+                ("push_line_relax", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -177,6 +178,7 @@ def lib():
             L.tsvd_server_edge_budget.restype = i64
             L.tsvd_server_edge_budget.argtypes = [vp]
             L.tsvd_probe_atomic_rate.argtypes = [i64, i32, vp]
+            L.tsvd_probe_atomic_lanes.argtypes = [i64, i32, i32, i32, vp]
             L.tsvd_gen_rmat.argtypes = [i32, i32, i64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_uint64, vp, vp]
             L.tsvd_gen_chung_lu.argtypes = [i32, i64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, vp, vp]
@@ -667,6 +669,14 @@ def probe_atomic_rate(rows=1 << 22, iters=64):
     this device (tsvd_probe_atomic_rate)."""
     r = ctypes.c_double()
     check(lib().tsvd_probe_atomic_rate(int(rows), int(iters), ctypes.byref(r)))
+    return r.value
+
+
+def probe_atomic_lanes(returning=True, lanes=64, rows=1 << 22, iters=64):
+    """tsvd_probe_atomic_lanes: wave-level FP32 atomics per second with the
+    returning or non-returning form and only `lanes` active lanes."""
+    r = ctypes.c_double()
+    check(lib().tsvd_probe_atomic_lanes(int(rows), int(iters), int(returning), int(lanes), ctypes.byref(r)))
     return r.value
 
 
