@@ -275,6 +275,21 @@ int tsvd_pipeline_snapshot_tuned(tsvd_pipeline *p, const int32_t *src, const int
  * and of the per-block norms ||A||, ||A_new - A_old||, residual. */
 int tsvd_pipeline_block_matrix(tsvd_pipeline *p, int32_t j, double *A);
 int tsvd_pipeline_block_norms(tsvd_pipeline *p, double *a_norm, double *e_norm, double *resid, int32_t *dirty);
+/* Replaces Log_sparse_matrix_entries_with_norm_computation[_LP]
+ * (ppr_computation_store_dynamic.h:2649-2732, 2742-2826), which the drivers
+ * run per block on threads (Tree_u_NC.cpp:646-666), and with test = 0 the
+ * plain Log_sparse_matrix_entries[_LP] (:2527-2561, 2598-2638) used before
+ * lazy_start, over blocks [j0, j1): block j's new matrix is the pipeline's
+ * transform (NC or LP) of the current reserves (p, or p + p^T when
+ * directed); a_norm[j - j0] = ||A_new||_F and e_norm[j - j0] =
+ * ||A_new - A_old||_F against the stored matrix; record[j - j0] = -1 when
+ * test = 1 and resid_j + e < delta sqrt(2) a (clean, resid_j the block's
+ * residual from its last factorisation), else iter (the reference's
+ * update_mat_tree_record).  Blocks recorded iter get A_new as their stored
+ * matrix, as the reference replaces mat_mapping[i].  Output arrays may be
+ * NULL.  Dense-state pipelines only. */
+int tsvd_pipeline_block_transform(tsvd_pipeline *p, int32_t j0, int32_t j1, int32_t iter, double delta, int32_t test,
+                                  double *a_norm, double *e_norm, int32_t *record);
 /* Vertex ids of block j's stored columns (the active columns at its last
  * rebuild, ascending; *count of them; cols may be NULL).  frPCA's sketch of
  * block j has one row per stored column, in this order. */

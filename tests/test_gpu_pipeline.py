@@ -45,9 +45,22 @@ def active_columns(pipe, n, directed, sources):
     return act
 
 
+@pytest.fixture(params=["density-rule", "all-sparse", "all-dense"])
+def leaf_products(request, monkeypatch):
+    """Leaf products by the library's density rule, from sparse CSR/CSC
+    copies for every dirty block, or by dense GEMM for every dirty block
+    (TSVD_SPARSE_DENSITY, read when the pipeline is created)."""
+    value = {"density-rule": None, "all-sparse": "1.01", "all-dense": "0"}[request.param]
+    if value is None:
+        monkeypatch.delenv("TSVD_SPARSE_DENSITY", raising=False)
+    else:
+        monkeypatch.setenv("TSVD_SPARSE_DENSITY", value)
+    return request.param
+
+
 @pytest.mark.parametrize("directed,prec,mode", [(False, 1, "NC"), (False, 0, "NC"), (True, 0, "NC"),
                                                  (False, 0, "LP")])
-def test_pipeline_stagewise(capi, directed, prec, mode):
+def test_pipeline_stagewise(capi, directed, prec, mode, leaf_products):
     n, snaps, sub = small_workload(directed)
     rmax, nparts, fanout, k, q, delta = 1e-5, 8, 2, 12, 6, 0.12
     seed = 42
@@ -166,3 +179,56 @@ def test_dense_gram_gram_apply_and_iterative_oracle(capi, directed):
     assert 1.0 - np.linalg.norm(V.T @ it.U) ** 2 / k < 1e-8
     dense = bench.DenseOracle(pipe, k, None)
     assert dense.G is not None and np.abs(dense.lam - lam).max() <= 1e-10 * lam[0]
+
+
+@pytest.mark.parametrize("directed,mode", [(False, "NC"), (True, "LP")])
+def test_block_transform_entry_point(capi, directed, mode):
+    """tsvd_pipeline_block_transform, the per-block-range replacement of
+    Log_sparse_matrix_entries_with_norm_computation[_LP] (ppr:2649-2826) and,
+    with test = 0, of Log_sparse_matrix_entries[_LP] (ppr:2527-2638).  After
+    two snapshots, a third insertion goes through the graph and push handles
+    only (append, dynamic refresh, push), as a driver calling the pieces
+    would; the block transform over a sub-range then gives the norms of the
+    oracle's transform of the device reserves, the Eqn-2 record (-1 clean,
+    iter dirty) on those norms and the stored residual, replaces exactly the
+    dirty blocks' stored matrices and leaves blocks outside the range alone."""
+    n, snaps, sub = small_workload(directed)
+    rmax, nparts, k, delta = 1e-5, 8, 12, 0.12
+    pipe = capi.Pipeline(n, directed, sub, 0.2, rmax, nparts, 2, k, 6, precision=1, delta=delta, seed=3,
+                         transform=capi.TSVD_TRANSFORM_NC if mode == "NC" else capi.TSVD_TRANSFORM_LP)
+    for s, d in snaps[:2]:
+        pipe.snapshot(s, d)
+    pipe.graph.append(*snaps[2])
+    for d in ((0, 1) if directed else (0,)):
+        pipe.ppr.refresh(capi.TSVD_REFRESH_DYNAMIC, d)
+        pipe.ppr.push(d)
+    A_full = transform_of(pipe, n, directed, rmax, mode)
+    starts, widths = orc.block_map(n, nparts)
+    resid = pipe.block_norms()[2]
+    old = [pipe.block_matrix(j, widths[j]) for j in range(nparts)]
+    j0, j1, it = 2, 7, 3
+    a, e, rec = pipe.block_transform(j0, j1, it, delta, test=True)
+    assert a.size == e.size == rec.size == j1 - j0
+    for i, j in enumerate(range(j0, j1)):
+        Aj = A_full[:, starts[j]:starts[j] + widths[j]]
+        _, an, en = orc.dirty_test(Aj, old[j], resid[j], delta)
+        assert a[i] == pytest.approx(an, rel=1e-10) and e[i] == pytest.approx(en, rel=1e-9, abs=1e-12)
+        dirty = not (resid[j] + e[i] < delta * np.sqrt(2.0) * a[i])
+        assert rec[i] == (it if dirty else -1)
+        now = pipe.block_matrix(j, widths[j])
+        if dirty:
+            assert np.allclose(now, Aj, rtol=1e-12, atol=1e-12)
+        else:
+            assert np.array_equal(now, old[j])
+    for j in list(range(j0)) + list(range(j1, nparts)):
+        assert np.array_equal(pipe.block_matrix(j, widths[j]), old[j])
+    # test = 0: every block recorded and replaced, norms against what is stored now
+    stored = [pipe.block_matrix(j, widths[j]) for j in range(nparts)]
+    a0, e0, rec0 = pipe.block_transform(0, nparts, 4, delta, test=False)
+    assert (rec0 == 4).all()
+    for j in range(nparts):
+        Aj = A_full[:, starts[j]:starts[j] + widths[j]]
+        assert e0[j] == pytest.approx(np.linalg.norm(Aj - stored[j]), rel=1e-9, abs=1e-12)
+        assert np.allclose(pipe.block_matrix(j, widths[j]), Aj, rtol=1e-12, atol=1e-12)
+    with pytest.raises(capi.TreeSVDError):
+        pipe.block_transform(3, nparts + 1, 5, delta)

@@ -43,9 +43,9 @@ __global__ void __launch_bounds__(TPB) k_block_norms(const T *__restrict__ p0, c
                                                      const T *__restrict__ A, const int32_t *__restrict__ slot,
                                                      int32_t n, int32_t nsrc,
                                                      int32_t s_pad, int32_t nparts, int32_t common, int transform,
-                                                     T rm, double *__restrict__ norms)
+                                                     T rm, double *__restrict__ norms, int32_t j0)
 {
-    const int32_t j = blockIdx.y;
+    const int32_t j = j0 + (int32_t)blockIdx.y;
     const int64_t start = (int64_t)j * common;
     const int64_t width = (j == nparts - 1) ? n - start : common;
     const int64_t v0 = start + (int64_t)blockIdx.x * VPB;
@@ -192,6 +192,18 @@ __global__ void k_gaussian(T *__restrict__ X, int64_t rows, int64_t cols, int64_
     X[c * ld + i] = (T)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
 }
 
+template <class T>
+__global__ void k_gaussian_rm(T *__restrict__ X, int64_t rows, int64_t cols, uint64_t seed)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * cols) return;
+    const int64_t c = t / rows, i = t - c * rows;
+    const uint64_t h = mix64(seed ^ mix64((uint64_t)t));
+    const double u1 = ((double)(h >> 40) + 1.0) / 16777217.0;
+    const double u2 = (double)(h & 0xFFFFFFull) / 16777216.0;
+    X[i * cols + c] = (T)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+}
+
 __global__ void k_eig_rotation(double *__restrict__ W, const double *__restrict__ V, const double *__restrict__ lam,
                                int l, double eps_abs, double eps_rel)
 {
@@ -280,33 +292,42 @@ void blocks_setup(Blocks &b, int32_t n, int32_t nparts, int32_t nsrc, int32_t s_
 
 void blocks_norms(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g, hipStream_t st)
 {
+    blocks_norms_range(b, p0, p1, g, 0, b.nparts, st);
+}
+
+void blocks_norms_range(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g, int32_t j0, int32_t j1,
+                        hipStream_t st)
+{
+    if (j1 <= j0) return;
     hipLaunchKernelGGL(k_active, dim3((unsigned)ceil_div(b.n, TPB)), dim3(TPB), 0, st, g->dir[0].deg.p,
                        g->ndir > 1 ? g->dir[1].deg.p : nullptr, b.is_source.p, b.n, b.active.p);
     TSVD_HIP(hipMemsetAsync(b.norms.p, 0, b.norms.bytes(), st));
-    int64_t maxw = 0;
-    for (auto w : b.width) maxw = std::max(maxw, w);
-    dim3 grid((unsigned)ceil_div(maxw, VPB), (unsigned)b.nparts);
+    int64_t maxw = 0, cols = 0, stored = 0;
+    for (int32_t j = j0; j < j1; j++) {
+        maxw = std::max(maxw, b.width[j]);
+        cols += b.width[j];
+        stored += b.ncols[j];
+    }
+    dim3 grid((unsigned)ceil_div(maxw, VPB), (unsigned)(j1 - j0));
     if (g_profile) {
-        // reserves of every vertex (one or two directions) + the stored matrices
-        double stored = 0.0;
-        for (int32_t j = 0; j < b.nparts; j++) stored += (double)b.ncols[j];
+        // reserves of every vertex of the range (one or two directions) + the stored matrices
         const double dirs = p1 ? 2.0 : 1.0;
-        prof_add_work("blocks_norms", (dirs * b.n + stored) * b.nsrc * (double)b.esize, 0.0);
+        prof_add_work("blocks_norms", (dirs * (double)cols + (double)stored) * b.nsrc * (double)b.esize, 0.0);
     }
     PROF("blocks_norms", st);
     if (b.precision == TSVD_FP64)
         hipLaunchKernelGGL(k_block_norms<double>, grid, dim3(TPB), 0, st, (const double *)p0, (const double *)p1,
                            (const double *)b.A.p, b.slot.p, b.n, b.nsrc, b.s_pad, b.nparts, b.common,
-                           b.transform, b.rmax, b.norms.p);
+                           b.transform, b.rmax, b.norms.p, j0);
     else
         hipLaunchKernelGGL(k_block_norms<float>, grid, dim3(TPB), 0, st, (const float *)p0, (const float *)p1,
                            (const float *)b.A.p, b.slot.p, b.n, b.nsrc, b.s_pad, b.nparts, b.common,
-                           b.transform, (float)b.rmax, b.norms.p);
+                           b.transform, (float)b.rmax, b.norms.p, j0);
     TSVD_HIP(hipGetLastError());
     std::vector<double> h((size_t)2 * b.nparts);
     TSVD_HIP(hipMemcpyAsync(h.data(), b.norms.p, b.norms.bytes(), hipMemcpyDeviceToHost, st));
     TSVD_HIP(hipStreamSynchronize(st));
-    for (int32_t j = 0; j < b.nparts; j++) {
+    for (int32_t j = j0; j < j1; j++) {
         b.a2[j] = h[2 * j];
         b.e2[j] = h[2 * j + 1];
     }
@@ -392,6 +413,19 @@ void gaussian_fill(void *X, int precision, int64_t rows, int64_t cols, int64_t l
     else
         hipLaunchKernelGGL(k_gaussian<float>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st, (float *)X, rows,
                            cols, ld, seed);
+    TSVD_HIP(hipGetLastError());
+}
+
+void gaussian_fill_rowmajor(void *X, int precision, int64_t rows, int64_t cols, uint64_t seed, hipStream_t st)
+{
+    const int64_t tot = rows * cols;
+    if (tot <= 0) return;
+    if (precision == TSVD_FP64)
+        hipLaunchKernelGGL(k_gaussian_rm<double>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st,
+                           (double *)X, rows, cols, seed);
+    else
+        hipLaunchKernelGGL(k_gaussian_rm<float>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st, (float *)X,
+                           rows, cols, seed);
     TSVD_HIP(hipGetLastError());
 }
 

@@ -53,6 +53,38 @@ static GemmTypes types_for(int precision, DT a, DT b, DT c, DT acc)
     return GemmTypes{a, b, c, acc};
 }
 
+// G = X^T X (l x l, FP64, split over K with atomic adds) of a rows x l
+// operand: column-major with leading dimension ldx, or row-major (ld l).
+static GemmProb gram_prob(const void *X, int64_t ldx, int32_t rows, int32_t l, bool rowmajor, double *G)
+{
+    GemmProb p{};
+    if (!rowmajor) {
+        p.A = X; p.lda = ldx; p.transA = 1;       // op(A)(i, k) = X(k, i)
+        p.B = X; p.ldb = ldx;
+    } else {
+        p.A = X; p.lda = l; p.transA = 0;         // A(i, k) = X[k * l + i] = X(k, i)
+        p.B = X; p.ldb = l; p.transB = 1;         // B(k, j) = X[k * l + j]
+    }
+    p.C = G; p.ldc = l;
+    p.M = l; p.N = l; p.K = rows; p.alpha = 1.0; p.beta = 0.0; p.kchunk = 512;
+    return p;
+}
+
+// Out (rows x n, column-major, leading dimension ldo) = X W with W l x n
+// column-major; X column-major (leading dimension ldx) or row-major (ld l).
+static GemmProb right_prob(const void *X, int64_t ldx, int32_t rows, int32_t l, bool rowmajor, const double *W,
+                           int32_t n, void *Out, int64_t ldo)
+{
+    GemmProb p{};
+    p.A = X;
+    if (!rowmajor) { p.lda = ldx; p.transA = 0; }
+    else { p.lda = l; p.transA = 1; }             // op(A)(r, c) = X[r * l + c]
+    p.B = W; p.ldb = l;
+    p.C = Out; p.ldc = ldo;
+    p.M = rows; p.N = n; p.K = l; p.alpha = 1.0; p.beta = 0.0;
+    return p;
+}
+
 // Q <- orth(Q) for every item: G = Q^T Q (FP64), eig, Q2 = Q V lam^-1/2, swap.
 static void orth(std::vector<FrpcaItem> &items, int precision, int32_t l, double *G, double *W, double *lam,
                  GemmRunner &gemm, Eig &eig, hipStream_t st)
@@ -63,37 +95,50 @@ static void orth(std::vector<FrpcaItem> &items, int precision, int32_t l, double
     std::vector<GemmProb> gp;
     for (int i = 0; i < cnt; i++) {
         const FrpcaItem &it = items[i];
-        GemmProb p{};
-        p.A = it.Q; p.lda = it.ldq; p.transA = 1;
-        p.B = it.Q; p.ldb = it.ldq;
-        p.C = G + (size_t)i * l * l; p.ldc = l;
-        p.M = l; p.N = l; p.K = it.m; p.alpha = 1.0; p.beta = 0.0; p.kchunk = 512;
-        gp.push_back(p);
+        gp.push_back(gram_prob(it.Q, it.ldq, it.m, l, it.sp >= 0, G + (size_t)i * l * l));
     }
     gemm.run(st, gp, types_for(precision, T, T, DT::F64, DT::F64), l > 80, "gemm_gram");
     eig.solve(st, G, l, cnt, lam);
     const double eps_rel = precision == TSVD_FP64 ? 1e-13 : 1e-6;
     eig_rotation(W, G, lam, l, cnt, 1e-4, eps_rel, st);
-    std::vector<GemmProb> rp;
+    std::vector<GemmProb> rp, rp_rm;
     for (int i = 0; i < cnt; i++) {
         FrpcaItem &it = items[i];
         GemmProb p{};
-        p.A = it.Q; p.lda = it.ldq; p.transA = 0;
-        p.B = W + (size_t)i * l * l; p.ldb = l;
-        p.C = it.Q2; p.ldc = it.ldq;
-        p.M = it.m; p.N = l; p.K = l; p.alpha = 1.0; p.beta = 0.0;
-        rp.push_back(p);
+        if (it.sp < 0) {
+            p.A = it.Q; p.lda = it.ldq; p.transA = 0;
+            p.B = W + (size_t)i * l * l; p.ldb = l;
+            p.C = it.Q2; p.ldc = it.ldq;
+            p.M = it.m; p.N = l; p.K = l;
+        } else {
+            // row-major Q2 is the column-major Q2^T = W^T Q^T (l x m, ld l)
+            p.A = W + (size_t)i * l * l; p.lda = l; p.transA = 1;
+            p.B = it.Q; p.ldb = l;
+            p.C = it.Q2; p.ldc = l;
+            p.M = l; p.N = it.m; p.K = l;
+        }
+        p.alpha = 1.0; p.beta = 0.0;
+        (it.sp < 0 ? rp : rp_rm).push_back(p);
     }
     gemm.run(st, rp, types_for(precision, T, DT::F64, T, DT::F64), l > 80, "gemm_rotate");
+    // row-major items: the FP64 rotation is the first operand
+    gemm.run(st, rp_rm, types_for(precision, DT::F64, T, T, DT::F64), l > 80, "gemm_rotate");
     for (auto &it : items) std::swap(it.Q, it.Q2);
 }
 
 static void product(std::vector<FrpcaItem> &items, int precision, int32_t l, bool transposed, GemmRunner &gemm,
-                     hipStream_t st)
+                     hipStream_t st, SparseLeaves *sp)
 {
     const DT T = precision == TSVD_FP64 ? DT::F64 : DT::F32;
     std::vector<GemmProb> ps;
+    bool any_sparse = false;
     for (auto &it : items) {
+        if (it.sp >= 0) {
+            sp->items[(size_t)it.sp].Q = it.Q;
+            sp->items[(size_t)it.sp].Qt = it.Qt;
+            any_sparse = true;
+            continue;
+        }
         GemmProb p{};
         p.A = it.A; p.lda = it.lda; p.alpha = 1.0; p.beta = 0.0; p.N = l;
         if (!transposed) {   // Q = A Qt
@@ -106,39 +151,41 @@ static void product(std::vector<FrpcaItem> &items, int precision, int32_t l, boo
         ps.push_back(p);
     }
     gemm.run(st, ps, GemmTypes{T, T, T, T}, l > 80, transposed ? "gemm_sketch_t" : "gemm_sketch_n");
+    if (any_sparse) leaf_spmm(*sp, precision, l, transposed, st);
 }
 
 void frpca_core(std::vector<FrpcaItem> &items, int precision, int32_t l, int32_t q, double *G, double *W,
-                double *lam, GemmRunner &gemm, Eig &eig, hipStream_t st)
+                double *lam, GemmRunner &gemm, Eig &eig, hipStream_t st, SparseLeaves *sp)
 {
     TSVD_REQUIRE(q >= 1, "frPCA needs q >= 1");
     const DT T = precision == TSVD_FP64 ? DT::F64 : DT::F32;
     if (q % 2 == 0) {
-        for (auto &it : items) gaussian_fill(it.Qt, precision, it.nc, l, it.ldqt, it.seed, st);
-        product(items, precision, l, false, gemm, st);
+        for (auto &it : items) {
+            if (it.sp < 0) gaussian_fill(it.Qt, precision, it.nc, l, it.ldqt, it.seed, st);
+            else gaussian_fill_rowmajor(it.Qt, precision, it.nc, l, it.seed, st);
+        }
+        product(items, precision, l, false, gemm, st, sp);
         orth(items, precision, l, G, W, lam, gemm, eig, st);
     } else {
-        for (auto &it : items) gaussian_fill(it.Q, precision, it.m, l, it.ldq, it.seed, st);
+        for (auto &it : items) {
+            if (it.sp < 0) gaussian_fill(it.Q, precision, it.m, l, it.ldq, it.seed, st);
+            else gaussian_fill_rowmajor(it.Q, precision, it.m, l, it.seed, st);
+        }
     }
     const int niter = (q - 1) / 2;
     for (int i = 1; i <= niter; i++) {
-        product(items, precision, l, true, gemm, st);
-        product(items, precision, l, false, gemm, st);
+        product(items, precision, l, true, gemm, st, sp);
+        product(items, precision, l, false, gemm, st, sp);
         orth(items, precision, l, G, W, lam, gemm, eig, st);
     }
-    product(items, precision, l, true, gemm, st);
+    product(items, precision, l, true, gemm, st, sp);
     // eigSVD(Qt): Gram of Qt and its eigenvectors
     const int cnt = (int)items.size();
     TSVD_HIP(hipMemsetAsync(G, 0, sizeof(double) * (size_t)l * l * cnt, st));
     std::vector<GemmProb> gp;
     for (int i = 0; i < cnt; i++) {
         const FrpcaItem &it = items[i];
-        GemmProb p{};
-        p.A = it.Qt; p.lda = it.ldqt; p.transA = 1;
-        p.B = it.Qt; p.ldb = it.ldqt;
-        p.C = G + (size_t)i * l * l; p.ldc = l;
-        p.M = l; p.N = l; p.K = it.nc; p.alpha = 1.0; p.beta = 0.0; p.kchunk = 512;
-        gp.push_back(p);
+        gp.push_back(gram_prob(it.Qt, it.ldqt, it.nc, l, it.sp >= 0, G + (size_t)i * l * l));
     }
     gemm.run(st, gp, GemmTypes{T, T, DT::F64, DT::F64}, l > 80, "gemm_gram");
     eig.solve(st, G, l, cnt, lam);
@@ -157,6 +204,9 @@ void frpca_setup(FrpcaWork &w, const Blocks &b, int32_t k, int32_t q)
     w.G.alloc((size_t)b.nparts * w.l * w.l);
     w.W.alloc((size_t)b.nparts * w.l * w.l);
     w.lam.alloc((size_t)b.nparts * w.l);
+    // TSVD_SPARSE_DENSITY overrides the density below which a dirty block is
+    // factored from its sparse copies (0: always dense)
+    if (const char *e = getenv("TSVD_SPARSE_DENSITY")) w.sparse_density = atof(e);
     TSVD_HIP(hipMemset(w.Q.p, 0, w.Q.bytes()));
     TSVD_HIP(hipMemset(w.Q2.p, 0, w.Q2.bytes()));
 }
@@ -182,20 +232,30 @@ void frpca_blocks(FrpcaWork &w, Blocks &b, const std::vector<int32_t> &dirty, vo
         it.seed = seed * 0x9E3779B97F4A7C15ull + (uint64_t)j * 0xD1B54A32D192ED03ull + 1;
         items.push_back(it);
     }
-    frpca_core(items, b.precision, l, w.q, w.G.p, w.W.p, w.lam.p, gemm, eig, st);
+    // blocks with few enough non-zeros are factored from CSR / CSC copies
+    std::vector<SpItem> sparse;
+    w.last_sparse = 0;
+    if (w.sparse_density > 0.0 && l <= 192) {
+        std::vector<SpItem> cand;
+        for (auto &it : items) cand.push_back(SpItem{it.A, it.lda, it.m, it.nc, 0, it.Q, it.Qt});
+        const std::vector<double> dens = leaf_densities(w.sp, cand, b.precision, st);
+        for (size_t i = 0; i < items.size(); i++) {
+            if (items[i].nc <= 0 || dens[i] >= w.sparse_density) continue;
+            items[i].sp = (int32_t)sparse.size();
+            sparse.push_back(cand[i]);
+        }
+    }
+    w.last_sparse = (int64_t)sparse.size();
+    leaf_sparse_build(w.sp, std::move(sparse), b.precision, st);
+    frpca_core(items, b.precision, l, w.q, w.G.p, w.W.p, w.lam.p, gemm, eig, st, &w.sp);
     const int cnt = (int)items.size();
     // leaf = U diag(S) = Q V[:, top k] diag(sqrt(lam))   (Tree_u_NC.cpp:147-154)
     eig_select(w.W.p, w.G.p, w.lam.p, l, k, cnt, 0, 0, 1e-4, st);
     const DT T = b.precision == TSVD_FP64 ? DT::F64 : DT::F32;
     std::vector<GemmProb> lp;
-    for (int i = 0; i < cnt; i++) {
-        GemmProb p{};
-        p.A = items[i].Q; p.lda = items[i].ldq; p.transA = 0;
-        p.B = w.W.p + (size_t)i * l * k; p.ldb = l;
-        p.C = (uint8_t *)leaves + (size_t)dirty[i] * b.s_pad * k * es; p.ldc = b.s_pad;
-        p.M = b.nsrc; p.N = k; p.K = l; p.alpha = 1.0; p.beta = 0.0;
-        lp.push_back(p);
-    }
+    for (int i = 0; i < cnt; i++)
+        lp.push_back(right_prob(items[i].Q, items[i].ldq, b.nsrc, l, items[i].sp >= 0, w.W.p + (size_t)i * l * k, k,
+                                (uint8_t *)leaves + (size_t)dirty[i] * b.s_pad * k * es, b.s_pad));
     gemm.run(st, lp, GemmTypes{T, DT::F64, T, DT::F64}, k > 80, "gemm_leaf");
     std::vector<double> lam((size_t)cnt * l);
     TSVD_HIP(hipMemcpyAsync(lam.data(), w.lam.p, sizeof(double) * lam.size(), hipMemcpyDeviceToHost, st));

@@ -241,6 +241,7 @@ void GemmRunner::run_into(hipStream_t st, GemmProb *probs, int nprobs, GemmTypes
         else if (t.a == DT::F64 && t.b == DT::F64 && t.c == DT::F64) launch_t<double, double, double, double>(st, dev, np, tiles, wide);
         else if (t.a == DT::F32 && t.b == DT::F64 && t.c == DT::F64) launch_t<float, double, double, double>(st, dev, np, tiles, wide);
         else if (t.a == DT::F64 && t.b == DT::F64 && t.c == DT::F32) launch_t<double, double, float, double>(st, dev, np, tiles, wide);
+        else if (t.a == DT::F64 && t.b == DT::F32 && t.c == DT::F32) launch_t<double, float, float, double>(st, dev, np, tiles, wide);
         else TSVD_REQUIRE(false, "unsupported GEMM type combination");
     }
     TSVD_HIP(hipGetLastError());

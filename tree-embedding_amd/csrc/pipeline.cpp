@@ -613,6 +613,36 @@ extern "C" int tsvd_pipeline_block_matrix(tsvd_pipeline *p, int32_t j, double *A
     });
 }
 
+extern "C" int tsvd_pipeline_block_transform(tsvd_pipeline *p, int32_t j0, int32_t j1, int32_t iter, double delta,
+                                             int32_t test, double *a_norm, double *e_norm, int32_t *record)
+{
+    return guard([&] {
+        TSVD_REQUIRE(p, "null pipeline");
+        TSVD_REQUIRE(!p->cfg.sparse_state && p->ppr, "the block transform reads the dense device state");
+        Blocks &b = p->blocks;
+        TSVD_REQUIRE(j0 >= 0 && j0 <= j1 && j1 <= b.nparts, "block range [%d, %d) outside [0, %d)", j0, j1,
+                     b.nparts);
+        TSVD_REQUIRE(test == 0 || test == 1, "test must be 0 or 1");
+        TSVD_REQUIRE(delta >= 0.0, "delta must be non-negative");
+        if (j1 == j0) return;
+        hipStream_t st = p->g->stream;
+        const void *p0 = p->ppr->p(0), *p1 = p->ppr->ndir > 1 ? p->ppr->p(1) : nullptr;
+        blocks_norms_range(b, p0, p1, p->g, j0, j1, st);
+        // ppr:2718-2730: clean (-1) when resid + ||A_new - A_old|| < delta sqrt(2) ||A_new||
+        const double thr = delta * std::sqrt(2.0);
+        std::vector<int32_t> rebuild;
+        for (int32_t j = j0; j < j1; j++) {
+            const double a = std::sqrt(b.a2[j]), e = std::sqrt(b.e2[j]);
+            const bool dirty = test == 0 || !(b.resid[j] + e < thr * a);
+            if (a_norm) a_norm[j - j0] = a;
+            if (e_norm) e_norm[j - j0] = e;
+            if (record) record[j - j0] = dirty ? iter : -1;
+            if (dirty) rebuild.push_back(j);
+        }
+        blocks_rebuild(b, rebuild, p0, p1, st);
+    });
+}
+
 extern "C" int tsvd_pipeline_block_columns(tsvd_pipeline *p, int32_t j, int32_t *cols, int32_t *count)
 {
     return guard([&] {

@@ -58,12 +58,40 @@ struct Blocks {
     std::vector<double> a2, e2, resid;
 };
 
+// Sparse leaf products (spmm.hip).  One dirty block factored from CSR / CSC
+// copies of its stored matrix; its Q (m x l) and Qt (nc x l) are row-major.
+struct SpItem {
+    const void *A;     // stored block, m x nc column-major, leading dimension lda
+    int64_t lda;
+    int32_t m, nc;
+    int64_t base;      // first of the item's m row counters, then its nc column counters
+    void *Q, *Qt;
+};
+struct SparseLeaves {
+    std::vector<SpItem> items;
+    DBuf<SpItem> ditems;
+    DBuf<unsigned long long> nnz;   // per item
+    DBuf<int64_t> cnt, off;         // counts and their exclusive scan (one run per row, then per column)
+    DBuf<int32_t> idx;              // CSR column ids / CSC row ids
+    DBuf<uint8_t> val, tmp;
+    int64_t counts = 0, entries = 0;
+};
+// Non-zero share of each candidate block (items need A, lda, m, nc).
+std::vector<double> leaf_densities(SparseLeaves &s, const std::vector<SpItem> &cand, int precision, hipStream_t st);
+// CSR and CSC of the listed blocks (their Q / Qt pointers are used by leaf_spmm).
+void leaf_sparse_build(SparseLeaves &s, std::vector<SpItem> items, int precision, hipStream_t st);
+// Q = A Qt (transposed = false) or Qt = A^T Q for every built item.
+void leaf_spmm(SparseLeaves &s, int precision, int32_t l, bool transposed, hipStream_t st);
+
 // frPCA workspace for all blocks (rank k, l = k + 5).
 struct FrpcaWork {
     int32_t k = 0, l = 0, q = 6;
     DBuf<uint8_t> Qt;       // n * l * esize   (block j at start_j * l, ld = width_j)
     DBuf<uint8_t> Q, Q2;    // nparts * s_pad * l * esize
     DBuf<double> G, W, lam; // nparts * l * l, nparts * l * l, nparts * l
+    SparseLeaves sp;        // sparse dirty blocks of the current factorisation
+    double sparse_density = 0.08;   // blocks with fewer non-zeros than this share use the sparse products
+    int64_t last_sparse = 0;        // dirty blocks factored sparse in the last call
 };
 
 // Merge tree: level 0 = leaves.  Level sizes from the leaves up; parent i of
@@ -87,6 +115,9 @@ void blocks_setup(Blocks &b, int32_t n, int32_t nparts, int32_t nsrc, int32_t s_
                   double rmax, const std::vector<int32_t> &sources, hipStream_t st);
 // ||A_new||^2 and ||A_new - A_old||^2 of every block (p0 [+ p1] transformed).
 void blocks_norms(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g, hipStream_t st);
+// The same for blocks [j0, j1) only (the other blocks' norms are left as they are).
+void blocks_norms_range(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g, int32_t j0, int32_t j1,
+                        hipStream_t st);
 // Rebuild the stored matrix of the listed blocks from p0 [+ p1].
 void blocks_rebuild(Blocks &b, const std::vector<int32_t> &dirty, const void *p0, const void *p1, hipStream_t st);
 
@@ -118,6 +149,8 @@ int32_t tree_update(Tree &t, const std::vector<int32_t> &dirty_leaves, int trans
 
 // Elementwise helpers (blocks.hip)
 void gaussian_fill(void *X, int precision, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, hipStream_t st);
+// The same numbers stored row-major (entry (i, c) at X[i * cols + c]).
+void gaussian_fill_rowmajor(void *X, int precision, int64_t rows, int64_t cols, uint64_t seed, hipStream_t st);
 // W[:, c] = V[:, c] * scale(c) with scale = 1/sqrt(lam) (zeroed below the
 // guard) for the orthonormalising rotation; batch of `count` l x l matrices.
 void eig_rotation(double *W, const double *V, const double *lam, int l, int count, double eps_abs, double eps_rel,
@@ -153,12 +186,13 @@ struct FrpcaItem {
     void *Q, *Q2;     // m x l each, leading dimension ldq
     int64_t ldq;
     uint64_t seed;
+    int32_t sp = -1;  // >= 0: item of the SparseLeaves set; Q, Q2 and Qt are then row-major (ld l)
 };
 // Runs the passes of frPCA (frpca.c:239-316) for all items at once.  On
 // return Q holds the final orthonormal basis, Qt = A^T Q, and G / lam (item i
 // at i*l*l / i*l) the eigenvectors and ascending eigenvalues of Qt^T Qt.
 void frpca_core(std::vector<FrpcaItem> &items, int precision, int32_t l, int32_t q, double *G, double *W,
-                double *lam, GemmRunner &gemm, Eig &eig, hipStream_t st);
+                double *lam, GemmRunner &gemm, Eig &eig, hipStream_t st, SparseLeaves *sp = nullptr);
 
 }  // namespace tsvd
 

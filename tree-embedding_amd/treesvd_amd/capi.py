@@ -30,7 +30,7 @@ EXPORTS = [
     "frPCA", "tsvd_frpca_csr", "tsvd_frpca_csr_batch", "tsvd_frpca_csr_noalloc",
     "tsvd_pipeline_create", "tsvd_pipeline_destroy", "tsvd_pipeline_snapshot",
     "tsvd_pipeline_embedding", "tsvd_pipeline_root_sigma", "tsvd_pipeline_graph", "tsvd_pipeline_ppr",
-    "tsvd_pipeline_block_matrix", "tsvd_pipeline_block_norms", "tsvd_pipeline_node", "tsvd_pipeline_block_columns",
+    "tsvd_pipeline_block_matrix", "tsvd_pipeline_block_norms", "tsvd_pipeline_block_transform", "tsvd_pipeline_node", "tsvd_pipeline_block_columns",
     "tsvd_pipeline_dense_gram", "tsvd_pipeline_gram_apply", "tsvd_pipeline_right_embedding",
     "tsvd_pipeline_lp_scores", "tsvd_pipeline_source_state", "tsvd_pipeline_snapshot_tuned",
     "tsvd_server_create", "tsvd_server_destroy", "tsvd_server_submit", "tsvd_server_wait",
@@ -161,6 +161,7 @@ def lib():
             L.tsvd_pipeline_ppr.argtypes = [vp]
             L.tsvd_pipeline_block_matrix.argtypes = [vp, i32, vp]
             L.tsvd_pipeline_block_norms.argtypes = [vp, vp, vp, vp, vp]
+            L.tsvd_pipeline_block_transform.argtypes = [vp, i32, i32, i32, ctypes.c_double, i32, vp, vp, vp]
             L.tsvd_pipeline_block_columns.argtypes = [vp, i32, vp, vp]
             L.tsvd_pipeline_node.argtypes = [vp, i32, i32, vp]
             L.tsvd_pipeline_dense_gram.argtypes = [vp, vp]
@@ -598,6 +599,16 @@ class Pipeline:
         A = np.zeros((width, self.nsrc))         # column-major nsrc x width
         check(lib().tsvd_pipeline_block_matrix(self.h, j, _p(A)))
         return A.T.copy()
+
+    def block_transform(self, j0, j1, iter_, delta, test=True):
+        """tsvd_pipeline_block_transform over blocks [j0, j1): (a_norm,
+        e_norm, record) with record -1 (clean) or iter_ (dirty, matrix
+        replaced)."""
+        cnt = max(j1 - j0, 1)
+        a, e, rec = np.zeros(cnt), np.zeros(cnt), np.zeros(cnt, np.int32)
+        check(lib().tsvd_pipeline_block_transform(self.h, int(j0), int(j1), int(iter_), float(delta), int(test),
+                                                  _p(a), _p(e), _p(rec)))
+        return a[:j1 - j0], e[:j1 - j0], rec[:j1 - j0]
 
     def block_columns(self, j):
         """Vertex ids of block j's stored columns, in sketch-row order."""
