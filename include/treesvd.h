@@ -167,6 +167,16 @@ int tsvd_frpca_csr(const mat_csr *A, mat *U, mat *S, mat *V, int k, int q, uint6
 int tsvd_frpca_csr_batch(const mat_csr *const *A, mat *const *U, mat *const *S, mat *const *V, int count, int k,
                          int q, uint64_t seed);
 
+/* Replaces LAPACKE_dsyevd (frpca.c:102, eigSVD frpca.c:99-136) and the merge
+ * SVDs' small dense problem (dgesvd, matrix_vector_functions_intel_mkl.c:1278,
+ * taken through the FP64 Gram): `count` symmetric n x n host matrices
+ * (n <= 1024) at stride n*n are solved on the device, eigenvalues ascending
+ * into lam (stride n), eigenvectors as the columns of V (column-major, same
+ * order).  top in [1, n] writes only the last `top` columns (the largest
+ * eigenvalues' vectors) and leaves the others unspecified; top <= 0 writes
+ * all.  *info (may be null) is 1 if an iteration limit was reached. */
+int tsvd_eig_batched(const double *A, int32_t n, int32_t count, int32_t top, double *V, double *lam, int32_t *info);
+
 /* No reference counterpart: for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  The computation of tsvd_frpca_csr on a device-resident matrix:

@@ -22,31 +22,6 @@
 
 namespace tsvd {
 
-Eig::Eig()
-{
-    if (rocblas_create_handle(&h_) != rocblas_status_success)
-        throw Error{set_error(TSVD_EHIP, "rocblas_create_handle failed")};
-}
-
-Eig::~Eig()
-{
-    if (h_) rocblas_destroy_handle(h_);
-}
-
-void Eig::solve(hipStream_t st, double *A, int n, int count, double *w)
-{
-    if (count <= 0) return;
-    e_.ensure((size_t)n * count);
-    info_.ensure((size_t)count);
-    PROF("eig_syevd", st);
-    if (rocblas_set_stream(h_, st) != rocblas_status_success)
-        throw Error{set_error(TSVD_EHIP, "rocblas_set_stream failed")};
-    rocblas_status s = rocsolver_dsyevd_strided_batched(h_, rocblas_evect_original, rocblas_fill_upper, n, A, n,
-                                                        (rocblas_stride)n * n, w, n, e_.p, n, info_.p, count);
-    if (s != rocblas_status_success)
-        throw Error{set_error(TSVD_EHIP, "rocsolver_dsyevd_strided_batched failed (%d)", (int)s)};
-}
-
 static GemmTypes types_for(int precision, DT a, DT b, DT c, DT acc)
 {
     (void)precision;

@@ -27,7 +27,7 @@ namespace {
 
 constexpr int64_t NA_MMAX = 4096;
 constexpr int64_t NA_NMAX = 32768;
-constexpr int64_t NA_LMAX = JACOBI_LMAX;
+constexpr int64_t NA_LMAX = 136;
 constexpr int NA_DESC = 8;
 constexpr size_t na_round(size_t x) { return (x + 255) / 256 * 256; }
 constexpr size_t OFF_FLAG = 0;
@@ -38,7 +38,9 @@ constexpr size_t OFF_Q2 = na_round(OFF_Q + NA_MMAX * NA_LMAX * sizeof(double));
 constexpr size_t OFF_G = na_round(OFF_Q2 + NA_MMAX * NA_LMAX * sizeof(double));
 constexpr size_t OFF_W = na_round(OFF_G + NA_LMAX * NA_LMAX * sizeof(double));
 constexpr size_t OFF_LAM = na_round(OFF_W + NA_LMAX * NA_LMAX * sizeof(double));
-constexpr size_t SCRATCH_BYTES = na_round(OFF_LAM + NA_LMAX * sizeof(double));
+constexpr size_t OFF_EIG = na_round(OFF_LAM + NA_LMAX * sizeof(double));
+constexpr size_t EIG_BYTES = eig_workspace_bytes((int)NA_LMAX, 1);
+constexpr size_t SCRATCH_BYTES = na_round(OFF_EIG + EIG_BYTES);
 
 }  // namespace
 }  // namespace tsvd
@@ -259,7 +261,7 @@ double *na_orth(double *Q, double *Q2, int32_t m, int32_t l, uint8_t *sc, hipStr
     double *G = (double *)(sc + OFF_G), *W = (double *)(sc + OFF_W), *lam = (double *)(sc + OFF_LAM);
     GemmProb *desc = (GemmProb *)(sc + OFF_DESC);
     na_gram(Q, m, l, G, desc, st);
-    jacobi_eig(st, G, l, 1, lam);
+    eig_batched(st, G, l, 1, lam, 0, sc + OFF_EIG, EIG_BYTES, na().gemm, desc, NA_DESC);
     eig_rotation(W, G, lam, l, 1, 1e-4, 1e-13, st);
     GemmProb p{};
     p.A = W; p.lda = l; p.transA = 1;        // op(A)(i, k) = W[i * l + k] = W(k, i)
@@ -347,7 +349,7 @@ extern "C" int tsvd_frpca_csr_noalloc(int32_t m, int32_t n, int32_t nnz, const d
         na_product(a, l, true, Q, Qt, st);
         // eigSVD(Qt): U = Q V[:, sel], V = Qt V[:, sel] / S, S ascending (frpca.c:296-316)
         na_gram(Qt, n, l, G, desc, st);
-        jacobi_eig(st, G, l, 1, lam);
+        eig_batched(st, G, l, 1, lam, 0, sc + OFF_EIG, EIG_BYTES, na().gemm, desc, NA_DESC);
         eig_select(W, G, lam, l, k, 1, 3, 1, 1e-4, st);
         na_project(Q, m, l, W, k, U, desc, st);
         eig_select(W, G, lam, l, k, 1, 2, 1, 1e-4, st);

@@ -81,7 +81,7 @@ int32_t tree_update(Tree &t, const std::vector<int32_t> &dirty_leaves, int trans
             gp.push_back(p);
         }
         gemm.run(st, gp, GemmTypes{T, T, DT::F64, DT::F64}, true, "gemm_merge_gram");
-        eig.solve(st, t.G.p, nc, cnt, t.lam.p);
+        eig.solve(st, t.G.p, nc, cnt, t.lam.p, k);
         const bool root = lvl == top;
         const int mode = !root ? 3 : (transform == TSVD_TRANSFORM_NC ? 1 : 2);
         eig_select(t.Wm.p, t.G.p, t.lam.p, nc, k, cnt, mode, 0, root ? 1e-12 : 0.0, st);

@@ -6,34 +6,38 @@
 #include "tsvd_gemm.h"
 #include "tsvd_sparse.h"
 
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include <map>
 #include <unordered_map>
 
 namespace tsvd {
 
-// Batched FP64 symmetric eigensolver on the device (rocSOLVER syevd).
+// Batched FP64 symmetric eigensolver (eig.hip): `count` n x n matrices at
+// stride n*n (column-major, n <= EIG_NMAX); eigenvalues ascending into lam
+// (stride n, all n), eigenvectors in the same order over the input, of which
+// only the last `top` columns are written (top <= 0: all).  The workspace of
+// eig_workspace_bytes(n, count) bytes is the caller's; GEMM descriptors go
+// through desc_dev (cap slots) when given, else the runner's own buffer.
+// info_host, when given, receives 1 if a QR segment did not converge (it
+// synchronizes the stream).
+constexpr int EIG_NMAX = 1024;
+constexpr size_t eig_workspace_bytes(int n, int count)
+{
+    return 1024 + (size_t)(count > 0 ? count : 0) *
+                      ((4 * (size_t)n * (size_t)n + 5 * (size_t)n) * sizeof(double) + sizeof(int));
+}
+void eig_batched(hipStream_t st, double *A, int n, int count, double *lam, int top, void *ws, size_t ws_bytes,
+                 GemmRunner &gemm, GemmProb *desc_dev = nullptr, int desc_cap = 0, int *info_host = nullptr);
+
 class Eig {
 public:
-    Eig();
-    ~Eig();
-    // `count` n x n matrices at stride n*n (column-major, overwritten by the
-    // eigenvectors); eigenvalues ascending into w (stride n).
-    void solve(hipStream_t st, double *A, int n, int count, double *w);
+    // eig_batched with a workspace that grows as needed
+    void solve(hipStream_t st, double *A, int n, int count, double *w, int top = 0);
 
 private:
-    rocblas_handle h_ = nullptr;
-    DBuf<double> e_;
-    DBuf<rocblas_int> info_;
+    DBuf<uint8_t> ws_;
+    GemmRunner gemm_;
 };
-
-// Hand-written batched FP64 symmetric eigensolver (jacobi.hip): one workgroup
-// per l x l matrix (l <= JACOBI_LMAX), same contract as Eig::solve
-// (eigenvectors over the input, eigenvalues ascending).  Allocates nothing.
-constexpr int JACOBI_LMAX = 136;
-void jacobi_eig(hipStream_t st, double *G, int l, int count, double *lam);
 
 // Column blocks of the |S| x n matrix (Tree_u_NC.cpp:396-432): common = n /
 // nparts, block of v = min(v / common, nparts-1), the last block takes the

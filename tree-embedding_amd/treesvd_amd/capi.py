@@ -35,7 +35,7 @@ EXPORTS = [
     "tsvd_pipeline_lp_scores", "tsvd_pipeline_source_state", "tsvd_pipeline_snapshot_tuned",
     "tsvd_server_create", "tsvd_server_destroy", "tsvd_server_submit", "tsvd_server_wait",
     "tsvd_server_pipeline", "tsvd_server_edge_budget", "tsvd_probe_atomic_rate", "tsvd_probe_atomic_lanes", "tsvd_gen_rmat",
-    "tsvd_gen_chung_lu",
+    "tsvd_gen_chung_lu", "tsvd_eig_batched",
 ]
 
 
@@ -142,6 +142,8 @@ def lib():
         if hasattr(L, "frPCA"):
             L.frPCA.restype = None
             L.frPCA.argtypes = [ctypes.POINTER(MatCSR), vp, vp, vp, ctypes.c_int, ctypes.c_int]
+        if hasattr(L, "tsvd_eig_batched"):
+            L.tsvd_eig_batched.argtypes = [vp, i32, i32, i32, vp, vp, vp]
         if hasattr(L, "tsvd_frpca_csr_noalloc"):
             L.tsvd_frpca_csr_noalloc.argtypes = [i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, ctypes.c_uint64]
         if hasattr(L, "tsvd_frpca_csr"):
@@ -479,6 +481,24 @@ def frpca_csr(A_dense_or_csr, k: int, q: int, seed: int = 1):
     (U, S, V), (mu, ms, mv) = _outputs(m, n, k)
     check(lib().tsvd_frpca_csr(ctypes.byref(csr), ctypes.byref(mu), ctypes.byref(ms), ctypes.byref(mv), k, q, seed))
     return U.T.copy(), S, V.T.copy()
+
+
+def eig_batched(mats, top: int = 0):
+    """tsvd_eig_batched on a (count, n, n) stack of symmetric FP64 matrices.
+    Returns (lam (count, n) ascending, V (count, n, n) with V[b][:, i] the
+    eigenvector of lam[b, i], info).  With 0 < top < n only the last `top`
+    columns of each V are defined."""
+    A = np.ascontiguousarray(mats, dtype=np.float64)
+    if A.ndim == 2:
+        A = A[None]
+    count, n, n2 = A.shape
+    if n != n2:
+        raise ValueError("matrices must be square")
+    Vt = np.zeros_like(A)
+    lam = np.zeros((count, n))
+    info = ctypes.c_int32(0)
+    check(lib().tsvd_eig_batched(_p(A), n, count, int(top), _p(Vt), _p(lam), ctypes.byref(info)))
+    return lam, np.swapaxes(Vt, 1, 2).copy(), int(info.value)
 
 
 def frpca_csr_noalloc(A_dense_or_csr, k: int, q: int, seed: int = 1):
