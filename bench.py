@@ -49,7 +49,7 @@ def pipeline_for(capi, w, seed=7):
                          precision=capi.TSVD_FP64 if w.fp64_state else capi.TSVD_FP32,
                          transform=capi.TSVD_TRANSFORM_NC if w.transform == "NC" else capi.TSVD_TRANSFORM_LP,
                          delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed,
-                         sparse_state=w.sparse_state)
+                         sparse_state=w.sparse_state, merge_bf16x3=w.merge_bf16x3)
 
 
 def roofline_of(prof, precision_fp64):
@@ -259,7 +259,8 @@ def served_run(capi, w, reqs, arrivals, cap, label, seed=7, overlap=True, push_s
                       precision=capi.TSVD_FP64 if w.fp64_state else capi.TSVD_FP32,
                       transform=capi.TSVD_TRANSFORM_NC if w.transform == "NC" else capi.TSVD_TRANSFORM_LP,
                       delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed, max_step_requests=cap,
-                      sparse_state=w.sparse_state, overlap=overlap and cap != 1, two_sided=two_sided)
+                      sparse_state=w.sparse_state, overlap=overlap and cap != 1, two_sided=two_sided,
+                      merge_bf16x3=w.merge_bf16x3)
     t = time.perf_counter()
     if push_setting is not None:          # the plain run's autotuned push setting, for both replays
         srv.pipeline.ppr.push_setting = push_setting

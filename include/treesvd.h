@@ -231,6 +231,11 @@ typedef struct {
     int32_t sparse_state;
     int32_t sparse_groups;   /* groups of 64 sources per batch (0: all of them resident
                               * when they fit in HBM, else as HBM allows)          */
+    /* 1: the merge products C W_k of an FP32 pipeline run as bf16x3 on MFMA
+     * (each FP32 operand split into bf16 hi + lo, hi*hi + hi*lo + lo*hi,
+     * FP32 accumulation, about 16 significant bits); the merge Grams and
+     * eigensolves stay FP64.  0: FP64 accumulation. */
+    int32_t merge_bf16x3;
 } tsvd_config;
 
 typedef struct {

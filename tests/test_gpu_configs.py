@@ -51,7 +51,8 @@ def pipeline(capi, w, seed, **kw):
     return capi.Pipeline(w.n, w.directed, w.subset, w.alpha, w.rmax, w.nparts, w.fanout, w.rank, w.q,
                          precision=capi.TSVD_FP64 if w.fp64_state else capi.TSVD_FP32,
                          transform=capi.TSVD_TRANSFORM_NC if w.transform == "NC" else capi.TSVD_TRANSFORM_LP,
-                         delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed, **kw)
+                         delta=w.delta, dyn_start=2, lazy_start=2, slack=0.1, seed=seed,
+                         merge_bf16x3=w.merge_bf16x3, **kw)
 
 
 def merge_error(P, children):
@@ -309,7 +310,9 @@ def config_check(capi, name, nsample=32, nleaves=4):
                 err = leaf_error(pipe, j, starts[j], widths[j], seed, w.rank, w.q, True)
                 assert err <= 2e-3, f"leaf {j} at snapshot {it}: {err:.3e}"
                 worst = max(worst, err)
-            tree = check_tree(pipe, w.nparts, w.fanout, w.rank, w.transform, 2e-4) if it in (1, 3) else None
+            # bf16x3 merge products (the precision configuration) carry ~16 bits
+            tree_tol = 1e-3 if w.merge_bf16x3 else 2e-4
+            tree = check_tree(pipe, w.nparts, w.fanout, w.rank, w.transform, tree_tol) if it in (1, 3) else None
             say(f"snapshot {it}: push bound ratio {ratio:.3f} over {nsample} sources; {dirty.size} dirty, leaves "
                 f"{list(pick)} worst {worst:.2e}; tree {tree if tree is None else f'{tree:.2e}'} "
                 f"({time.time() - t0:.0f}s)")

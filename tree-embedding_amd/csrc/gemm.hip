@@ -166,6 +166,137 @@ __global__ void __launch_bounds__(TPB) k_gemm(const GemmProb *__restrict__ probs
     }
 }
 
+// bf16x3 products (the merge products of the FP32 pipelines): every FP32
+// operand x is split into bf16 hi = rne(x) and lo = rne(x - hi) when staged to
+// LDS, and each 16 x 16 x 16 slab is hi*hi + hi*lo + lo*hi on
+// v_mfma_f32_16x16x16bf16_1k with FP32 accumulation (about 16 significant
+// bits; lo*lo is dropped).  Lane l holds A(l % 16, 4 (l / 16) + e), B(4 (l /
+// 16) + e, l % 16), e = 0..3, and the accumulator C(4 (l / 16) + r, l % 16)
+// (checked on gfx950).  A is M x K column-major, B K x N column-major (FP32
+// or FP64, rounded to FP32 first), no transposes, no K split.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline unsigned short bf16_rne(float x)
+{
+    const unsigned u = __float_as_uint(x);
+    return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ inline float bf16_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+template <class TB, class TC, int NT>
+__global__ void __launch_bounds__(TPB) k_gemm_bf3(const GemmProb *__restrict__ probs, int nprob)
+{
+    constexpr int BN = 16 * NT;
+    constexpr int A_PER = BM * BK / TPB;
+    constexpr int B_PER = (BN * BK + TPB - 1) / TPB;
+    __shared__ unsigned short Ah[BK][BM], Al[BK][BM];
+    __shared__ unsigned short Bh[BK][BN], Bl[BK][BN];
+    const int64_t tile = blockIdx.x;
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (probs[mid].tile_begin <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    const GemmProb &P = probs[lo];
+    int64_t t = tile - P.tile_begin;
+    const int tm = (int)(t % P.tiles_m);
+    const int tn = (int)(t / P.tiles_m);
+    const int M = P.M, N = P.N, K = P.K;
+    const int i0 = tm * BM, j0 = tn * BN;
+    const float *__restrict__ A = (const float *)P.A;
+    const TB *__restrict__ B = (const TB *)P.B;
+    const int64_t lda = P.lda, ldb = P.ldb;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int n = 0; n < NT; n++) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float ra[A_PER], rb[B_PER];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < A_PER; q++) {
+            const int e = tid + q * TPB, kk = e / BM, ii = e % BM;
+            const int gi = i0 + ii, gk = k0 + kk;
+            ra[q] = (gi < M && gk < K) ? A[(int64_t)gk * lda + gi] : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < B_PER; q++) {
+            const int e = tid + q * TPB, jj = e / BK, kk = e % BK;
+            const int gj = j0 + jj, gk = k0 + kk;
+            rb[q] = (e < BN * BK && gj < N && gk < K) ? (float)B[(int64_t)gj * ldb + gk] : 0.f;
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < A_PER; q++) {
+            const int e = tid + q * TPB, kk = e / BM, ii = e % BM;
+            const unsigned short h = bf16_rne(ra[q]);
+            Ah[kk][ii] = h;
+            Al[kk][ii] = bf16_rne(ra[q] - bf16_val(h));
+        }
+#pragma unroll
+        for (int q = 0; q < B_PER; q++) {
+            const int e = tid + q * TPB;
+            if (e < BN * BK) {
+                const unsigned short h = bf16_rne(rb[q]);
+                Bh[e % BK][e / BK] = h;
+                Bl[e % BK][e / BK] = bf16_rne(rb[q] - bf16_val(h));
+            }
+        }
+    };
+    if (K > 0) fetch(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        __syncthreads();
+        stash();
+        __syncthreads();
+        if (k0 + BK < K) fetch(k0 + BK);
+        bf16x4 ah, al;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int kk = 4 * (lane >> 4) + e, ii = wave * 16 + (lane & 15);
+            ah[e] = (short)Ah[kk][ii];
+            al[e] = (short)Al[kk][ii];
+        }
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            bf16x4 bh, bl;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int kk = 4 * (lane >> 4) + e, jj = n * 16 + (lane & 15);
+                bh[e] = (short)Bh[kk][jj];
+                bl[e] = (short)Bl[kk][jj];
+            }
+            acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[n], 0, 0, 0);
+            acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[n], 0, 0, 0);
+            acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[n], 0, 0, 0);
+        }
+    }
+    TC *__restrict__ C = (TC *)P.C;
+    const int64_t ldc = P.ldc;
+    const double alpha = P.alpha, beta = P.beta;
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+        const int gj = j0 + n * 16 + (lane & 15);
+        if (gj >= N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int gi = i0 + wave * 16 + 4 * (lane >> 4) + r;
+            if (gi >= M) continue;
+            TC *c = C + (int64_t)gj * ldc + gi;
+            const double v = alpha * (double)acc[n][r];
+            *c = (TC)(beta == 0.0 ? v : v + beta * (double)*c);
+        }
+    }
+}
+
+template <class TB, class TC>
+void launch_bf3(hipStream_t st, const GemmProb *d, int nprob, int64_t tiles, bool wide)
+{
+    if (wide) hipLaunchKernelGGL((k_gemm_bf3<TB, TC, 9>), dim3((unsigned)tiles), dim3(TPB), 0, st, d, nprob);
+    else hipLaunchKernelGGL((k_gemm_bf3<TB, TC, 5>), dim3((unsigned)tiles), dim3(TPB), 0, st, d, nprob);
+}
+
 template <class TA, class TB, class TC, class TACC>
 void launch_t(hipStream_t st, const GemmProb *d, int nprob, int64_t tiles, bool wide)
 {
@@ -221,7 +352,7 @@ void GemmRunner::run_into(hipStream_t st, GemmProb *probs, int nprobs, GemmTypes
     // the staging array must outlive the copy
     TSVD_HIP(hipStreamSynchronize(st));
     if (g_profile) {
-        auto sz = [](DT d) { return d == DT::F64 ? 8.0 : 4.0; };
+        auto sz = [](DT d) { return d == DT::F64 ? 8.0 : 4.0; };   // BF3 is never an operand type
         double flops = 0.0, bytes = 0.0;
         for (int i = 0; i < nprobs; i++) {
             const GemmProb &p = probs[i];
@@ -232,7 +363,16 @@ void GemmRunner::run_into(hipStream_t st, GemmProb *probs, int nprobs, GemmTypes
         prof_add_work(label, bytes, flops);
     }
     PROF(label, st);
-    if (t.acc == DT::F32) {
+    if (t.acc == DT::BF3) {
+        for (int i = 0; i < nprobs; i++)
+            TSVD_REQUIRE(probs[i].transA == 0 && probs[i].transB == 0 && probs[i].tiles_k <= 1,
+                         "bf16x3 products take untransposed operands and no K split");
+        TSVD_REQUIRE(t.a == DT::F32, "bf16x3 products take an FP32 A");
+        if (t.b == DT::F64 && t.c == DT::F32) launch_bf3<double, float>(st, dev, np, tiles, wide);
+        else if (t.b == DT::F64 && t.c == DT::F64) launch_bf3<double, double>(st, dev, np, tiles, wide);
+        else if (t.b == DT::F32 && t.c == DT::F32) launch_bf3<float, float>(st, dev, np, tiles, wide);
+        else TSVD_REQUIRE(false, "unsupported bf16x3 type combination");
+    } else if (t.acc == DT::F32) {
         TSVD_REQUIRE(t.a == DT::F32 && t.b == DT::F32 && t.c == DT::F32, "FP32 accumulate takes FP32 operands");
         launch_t<float, float, float, float>(st, dev, np, tiles, wide);
     } else {

@@ -468,6 +468,7 @@ extern "C" int tsvd_pipeline_create(const tsvd_config *cfg, tsvd_pipeline **out)
                          cfg->rmax, p->sources, st);
             frpca_setup(p->fw, p->blocks, cfg->rank, cfg->q);
             tree_setup(p->tree, cfg->nparts, cfg->fanout, cfg->rank, cfg->nsrc, s_pad, p->blocks.esize, st);
+            p->tree.merge_bf3 = cfg->merge_bf16x3 != 0;
             if (cfg->sparse_state) sparse_setup(p, st);
             p->eig = new Eig();
             for (int32_t i = 0; i < cfg->nsrc; i++) p->re.row_of[cfg->sources[i]] = i;

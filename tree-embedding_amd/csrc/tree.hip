@@ -98,7 +98,9 @@ int32_t tree_update(Tree &t, const std::vector<int32_t> &dirty_leaves, int trans
             p.M = t.nsrc; p.N = k; p.K = nc; p.alpha = 1.0; p.beta = 0.0;
             pp.push_back(p);
         }
-        gemm.run(st, pp, GemmTypes{T, DT::F64, root ? DT::F64 : T, DT::F64}, k > 80, "gemm_merge");
+        const bool bf3 = t.merge_bf3 && T == DT::F32;
+        gemm.run(st, pp, GemmTypes{T, DT::F64, root ? DT::F64 : T, bf3 ? DT::BF3 : DT::F64}, k > 80,
+                 bf3 ? "gemm_merge_bf16x3" : "gemm_merge");
         if (root) {
             std::vector<double> lam((size_t)nc);
             TSVD_HIP(hipMemcpyAsync(lam.data(), t.lam.p, sizeof(double) * nc, hipMemcpyDeviceToHost, st));

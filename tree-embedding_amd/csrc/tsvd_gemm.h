@@ -4,7 +4,8 @@
 
 namespace tsvd {
 
-enum class DT : int { F32 = 0, F64 = 1 };
+// BF3 is an accumulate type only: bf16x3 products with FP32 accumulation
+enum class DT : int { F32 = 0, F64 = 1, BF3 = 2 };
 
 // One product  C (M x N) = alpha * op(A) * B + beta * C, all column-major:
 //   transA = 0: A is M x K,  A(i,k) = A[k*lda + i]

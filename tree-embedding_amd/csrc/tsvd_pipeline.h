@@ -113,6 +113,7 @@ struct Tree {
     std::vector<double> root_sigma;
     size_t esize = 4;
     bool have_root = false;
+    bool merge_bf3 = false;             // FP32 levels: merge products as bf16x3
 };
 
 void blocks_setup(Blocks &b, int32_t n, int32_t nparts, int32_t nsrc, int32_t s_pad, int precision, int transform,

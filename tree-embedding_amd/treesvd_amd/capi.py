@@ -61,7 +61,8 @@ class Config(ctypes.Structure):
                 ("transform", ctypes.c_int32), ("delta", ctypes.c_double), ("dyn_start", ctypes.c_int32),
                 ("lazy_start", ctypes.c_int32), ("slack", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("two_sided", ctypes.c_int32), ("right_embedding", ctypes.c_int32),
-                ("sparse_state", ctypes.c_int32), ("sparse_groups", ctypes.c_int32)]
+                ("sparse_state", ctypes.c_int32), ("sparse_groups", ctypes.c_int32),
+                ("merge_bf16x3", ctypes.c_int32)]
 
 
 class SnapshotStats(ctypes.Structure):
@@ -550,12 +551,14 @@ class Pipeline:
 
     def __init__(self, n, directed, sources, alpha, rmax, nparts, fanout, rank, q=6, precision=TSVD_FP32,
                  transform=TSVD_TRANSFORM_NC, delta=0.65, dyn_start=2, lazy_start=2, slack=0.1, seed=0,
-                 handle=None, two_sided=False, right_embedding=False, sparse_state=False, sparse_groups=0):
+                 handle=None, two_sided=False, right_embedding=False, sparse_state=False, sparse_groups=0,
+                 merge_bf16x3=False):
         self.sources = np.ascontiguousarray(sources, dtype=np.int32)
         self.nsrc, self.rank, self.n, self.nparts = self.sources.size, rank, n, nparts
         self.cfg = Config(n, int(directed), self.nsrc, self.sources.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                           alpha, rmax, precision, nparts, fanout, rank, q, transform, delta, dyn_start, lazy_start,
-                          slack, seed, int(two_sided), int(right_embedding), int(sparse_state), int(sparse_groups))
+                          slack, seed, int(two_sided), int(right_embedding), int(sparse_state), int(sparse_groups),
+                          int(merge_bf16x3))
         self.owned = handle is None
         if handle is None:
             h = ctypes.c_void_p()
@@ -727,12 +730,13 @@ class Server:
     def __init__(self, n, directed, sources, alpha, rmax, nparts, fanout, rank, q=6, precision=TSVD_FP32,
                  transform=TSVD_TRANSFORM_NC, delta=0.65, dyn_start=2, lazy_start=2, slack=0.1, seed=0,
                  max_step_edges=0, max_step_requests=0, two_sided=False, right_embedding=False,
-                 sparse_state=False, sparse_groups=0, overlap=True):
+                 sparse_state=False, sparse_groups=0, overlap=True, merge_bf16x3=False):
         self.sources = np.ascontiguousarray(sources, dtype=np.int32)
         self.nsrc, self.rank = self.sources.size, rank
         self.cfg = Config(n, int(directed), self.nsrc, self.sources.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                           alpha, rmax, precision, nparts, fanout, rank, q, transform, delta, dyn_start, lazy_start,
-                          slack, seed, int(two_sided), int(right_embedding), int(sparse_state), int(sparse_groups))
+                          slack, seed, int(two_sided), int(right_embedding), int(sparse_state), int(sparse_groups),
+                          int(merge_bf16x3))
         self.scfg = ServerConfig(int(max_step_edges), int(max_step_requests), int(not overlap))
         h = ctypes.c_void_p()
         check(lib().tsvd_server_create(ctypes.byref(self.cfg), ctypes.byref(self.scfg), ctypes.byref(h)))
@@ -741,7 +745,7 @@ class Server:
                                  transform, delta, dyn_start, lazy_start, slack, seed,
                                  handle=ctypes.c_void_p(lib().tsvd_server_pipeline(h)), two_sided=two_sided,
                                  right_embedding=right_embedding, sparse_state=sparse_state,
-                                 sparse_groups=sparse_groups)
+                                 sparse_groups=sparse_groups, merge_bf16x3=merge_bf16x3)
 
     @property
     def edge_budget(self):

@@ -159,6 +159,7 @@ class Workload:
     transform: str = "NC"      # "NC": log10(x/rmax) for x>rmax ; "LP": log10(1+x/rmax)
     delta: float = 0.65
     sparse_state: bool = False  # the n >= 1e7-node path's pair-list state (directed LP)
+    merge_bf16x3: bool = False  # merge products as bf16x3 on MFMA (FP32 accumulation)
 
 
 def workload(name: str, scale_down: int = 0, nsub: int | None = None,
@@ -203,7 +204,7 @@ def workload(name: str, scale_down: int = 0, nsub: int | None = None,
         sizes = [rest // 8 + (1 if i < rest % 8 else 0) for i in range(8)]
         snaps = shuffle_split(s, d, 2, 0.8, sizes)
         sub = sample_subset(snaps[0][0], nsub or 3000, 2)
-        return Workload(name, n, True, snaps, sub, 0.2, 1e-7, 512, 4, 128, 6, False)
+        return Workload(name, n, True, snaps, sub, 0.2, 1e-7, 512, 4, 128, 6, False, merge_bf16x3=True)
     if name == "served":
         n = 6_216_199 // sd
         m = 177_862_656 // sd
