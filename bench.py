@@ -327,7 +327,8 @@ def main():
                     help="requests in the served-vs-plain stream (0: skip that comparison)")
     ap.add_argument("--served-rate", type=float, default=12.0, help="Poisson arrival rate of that stream (1/s)")
     ap.add_argument("--no-tune", action="store_true",
-                    help="skip the push autotuner on the first warm-up snapshot")
+                    help="skip the push autotuner on the first warm-up snapshot and the MFMA tile-width "
+                         "autotuner over the initial and warm-up snapshots")
     ap.add_argument("--served-two-sided", action=argparse.BooleanOptionalAction, default=True,
                     help="also compare at saturation with the two-sided push threshold")
     ap.add_argument("--served-no-overlap", action="store_true",
@@ -387,6 +388,10 @@ def main():
     # command does); the snapshot rate comes from the timed steps only
     capi.profile_reset()
     capi.profile_enable(True)
+    if not args.no_tune:
+        # product tile widths chosen on the initial snapshot (every block refactored,
+        # every merge recomputed) and the warm-up steps, kept for the timed steps
+        capi.gemm_tuning(True)
     pipe = pipeline_for(capi, w)
     t = time.perf_counter()
     st0 = pipe.snapshot(*batches[0])
@@ -439,6 +444,12 @@ def main():
             st = pipe.snapshot(*batches[j])
         warm.append(st)
         log(f"warmup {i} (batch {j}): {st['ms_total']:.1f} ms dirty={st['ndirty']}")
+    capi.gemm_tuning(False)
+    gemm_tuning = capi.gemm_tuning_read()
+    if gemm_tuning:
+        log("tile autotuner: " + ", ".join(f"{k.split('|')[0]} N{k.split('|')[2][1:]} -> {v['tile_n']} "
+                                           f"({v['ms_tile80']:.2f}/{v['ms_tile144']:.2f} ms)"
+                                           for k, v in gemm_tuning.items()))
 
     # K timed steps: each is one synchronous snapshot call (the library
     # synchronises its streams before returning), timed on the host clock;
@@ -511,6 +522,7 @@ def main():
                             "from the initial snapshot outside the clock and the batches replay from the first"},
         "dirty_blocks_per_step": ndirty, "merged_nodes_per_step": nmerged, "pair_list_state": state,
         "push_tuning": tuning,
+        "gemm_tile_tuning": gemm_tuning,
         "push_atomic_roofline": {"achieved": round(achieved_rate / 1e9, 3), "ceiling": round(ceiling / 1e9, 3),
                                  "unit": "G wave-atomics/s", "frac": round(achieved_rate / ceiling, 4),
                                  "ceiling_non_returning": round(probe_noret / 1e9, 3),

@@ -51,6 +51,15 @@ int tsvd_memcpy(void *dst, const void *src, int64_t bytes, int kind);
 int tsvd_profile_enable(int on);
 int tsvd_profile_reset(void);
 int tsvd_profile_read(char *json, int len);
+/* The project's autotuner for the MFMA products' tile width (no reference
+ * counterpart).  While on, the first call of each product kind (label, types,
+ * N, K class) whose problems can be run twice (no K split, beta = 0) is timed
+ * with the 80- and the 144-column tile and the faster is kept, process-wide,
+ * for every later call; off stops tuning new kinds and keeps the choices.
+ * read returns {"kind": {"tile_n": .., "ms_tile80": .., "ms_tile144": ..,
+ * "calls": ..}, ...}. */
+int tsvd_gemm_tuning(int on);
+int tsvd_gemm_tuning_read(char *json, int len);
 /* Throughput ceiling of the push's memory operation, for its roofline: 64-lane
  * returning FP32 atomic adds on 256-byte rows at hashed row indices over a
  * `rows` x 256 B array, `iters` per wavefront.  Returns wave-level atomic

@@ -16,6 +16,9 @@
 // accumulator C(l/16 + 4r, l%16), r = 0..3 (both checked on gfx950).
 #include "tsvd_gemm.h"
 
+#include <map>
+#include <mutex>
+
 namespace tsvd {
 
 namespace {
@@ -314,8 +317,109 @@ void GemmRunner::run(hipStream_t st, std::vector<GemmProb> &probs, GemmTypes t, 
     run_into(st, probs.data(), (int)probs.size(), t, wide, dprobs_.p, nlive, label);
 }
 
+namespace {
+
+struct TuneRec {
+    int wide = -1;
+    double ms[2] = {0.0, 0.0};
+    int64_t calls = 0;
+};
+
+std::mutex g_tune_mu;
+bool g_tune_on = false;
+std::map<std::string, TuneRec> g_tuned;
+
+std::string tune_key(const char *label, GemmTypes t, const GemmProb *probs, int nprobs)
+{
+    int64_t nmax = 0, kmax = 0;
+    for (int i = 0; i < nprobs; i++) {
+        nmax = std::max<int64_t>(nmax, probs[i].N);
+        kmax = std::max<int64_t>(kmax, probs[i].K);
+    }
+    int kcls = 0;
+    while ((1ll << kcls) < kmax) kcls++;
+    char buf[160];
+    snprintf(buf, sizeof buf, "%s|%d%d%d%d|N%lld|K2^%d", label ? label : "gemm", (int)t.a, (int)t.b, (int)t.c,
+             (int)t.acc, (long long)nmax, kcls);
+    return buf;
+}
+
+bool idempotent(const GemmProb *probs, int nprobs)
+{
+    for (int i = 0; i < nprobs; i++) {
+        const GemmProb &p = probs[i];
+        if (p.M <= 0 || p.N <= 0) continue;
+        if ((p.kchunk > 0 && p.K > p.kchunk) || p.beta != 0.0 || p.C == p.A || p.C == p.B) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+void gemm_tuning_enable(bool on)
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune_on = on;
+}
+
+std::string gemm_tuning_json()
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    std::string out = "{";
+    bool first = true;
+    for (auto &kv : g_tuned) {
+        char buf[320];
+        snprintf(buf, sizeof buf, "%s\"%s\": {\"tile_n\": %d, \"ms_tile80\": %.4f, \"ms_tile144\": %.4f, \"calls\": %lld}",
+                 first ? "" : ", ", kv.first.c_str(), kv.second.wide ? 144 : 80, kv.second.ms[0], kv.second.ms[1],
+                 (long long)kv.second.calls);
+        out += buf;
+        first = false;
+    }
+    return out + "}";
+}
+
 void GemmRunner::run_into(hipStream_t st, GemmProb *probs, int nprobs, GemmTypes t, bool wide, GemmProb *dev,
                           int cap, const char *label)
+{
+    const std::string key = tune_key(label, t, probs, nprobs);
+    bool tune_now = false;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end() && it->second.wide >= 0) {
+            wide = it->second.wide != 0;
+            it->second.calls++;
+        } else if (g_tune_on && idempotent(probs, nprobs)) {
+            tune_now = true;
+        }
+    }
+    if (!tune_now) {
+        launch_once(st, probs, nprobs, t, wide, dev, cap, label);
+        return;
+    }
+    // both widths on the same inputs (each overwrites C), the faster one kept
+    hipEvent_t ev[3];
+    for (auto &e : ev) TSVD_HIP(hipEventCreate(&e));
+    TSVD_HIP(hipEventRecord(ev[0], st));
+    launch_once(st, probs, nprobs, t, false, dev, cap, label);
+    TSVD_HIP(hipEventRecord(ev[1], st));
+    launch_once(st, probs, nprobs, t, true, dev, cap, label);
+    TSVD_HIP(hipEventRecord(ev[2], st));
+    TSVD_HIP(hipEventSynchronize(ev[2]));
+    float a = 0.f, b = 0.f;
+    TSVD_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+    TSVD_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+    for (auto &e : ev) TSVD_HIP(hipEventDestroy(e));
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    TuneRec &r = g_tuned[key];
+    r.ms[0] = a;
+    r.ms[1] = b;
+    r.wide = b < a ? 1 : 0;
+    r.calls++;
+}
+
+void GemmRunner::launch_once(hipStream_t st, GemmProb *probs, int nprobs, GemmTypes t, bool wide, GemmProb *dev,
+                             int cap, const char *label)
 {
     const int BN = wide ? 144 : 80;
     int64_t tiles = 0;

@@ -1,4 +1,7 @@
 // profile.cpp -- per-kernel HIP-event timing registry (see tsvd_profile.h).
+#include "tsvd_gemm.h"
+
+#include <cstring>
 #include "tsvd_profile.h"
 #include "tsvd_internal.h"
 
@@ -90,6 +93,21 @@ void prof_add_work(const char *name, double bytes, double flops)
 extern "C" int tsvd_profile_enable(int on)
 {
     return tsvd::guard([&] { tsvd::g_profile = on != 0; });
+}
+
+extern "C" int tsvd_gemm_tuning(int on)
+{
+    return tsvd::guard([&] { tsvd::gemm_tuning_enable(on != 0); });
+}
+
+extern "C" int tsvd_gemm_tuning_read(char *json, int len)
+{
+    return tsvd::guard([&] {
+        TSVD_REQUIRE(json && len > 2, "null or short buffer");
+        const std::string s = tsvd::gemm_tuning_json();
+        TSVD_REQUIRE((int)s.size() < len, "buffer too short for the tuning record (%d bytes)", (int)s.size() + 1);
+        memcpy(json, s.c_str(), s.size() + 1);
+    });
 }
 
 extern "C" int tsvd_profile_reset(void)

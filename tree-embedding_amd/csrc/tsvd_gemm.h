@@ -2,6 +2,8 @@
 #pragma once
 #include "tsvd_internal.h"
 
+#include <string>
+
 namespace tsvd {
 
 // BF3 is an accumulate type only: bf16x3 products with FP32 accumulation
@@ -32,6 +34,13 @@ struct GemmTypes {
     DT a, b, c, acc;
 };
 
+// Tile-width autotuning (process-wide): while enabled, the first call of a
+// (label, types, N, K class) whose problems are idempotent (no K split,
+// beta = 0, C distinct from A and B) runs with both tile widths, timed with
+// events, and the faster width is kept for every later call of that key.
+void gemm_tuning_enable(bool on);
+std::string gemm_tuning_json();
+
 class GemmRunner {
 public:
     void run(hipStream_t st, std::vector<GemmProb> &probs, GemmTypes t, bool wide, const char *label = "gemm");
@@ -42,6 +51,8 @@ public:
     int64_t launches = 0;
 
 private:
+    void launch_once(hipStream_t st, GemmProb *probs, int nprobs, GemmTypes t, bool wide, GemmProb *dev, int cap,
+                     const char *label);
     DBuf<GemmProb> dprobs_;
 };
 

@@ -20,7 +20,7 @@ TSVD_TRANSFORM_NC, TSVD_TRANSFORM_LP = 0, 1
 # Every symbol include/treesvd.h declares.
 EXPORTS = [
     "tsvd_last_error", "tsvd_device_info", "tsvd_device_alloc", "tsvd_device_free", "tsvd_memcpy",
-    "tsvd_profile_enable", "tsvd_profile_reset", "tsvd_profile_read",
+    "tsvd_profile_enable", "tsvd_profile_reset", "tsvd_profile_read", "tsvd_gemm_tuning", "tsvd_gemm_tuning_read",
     "tsvd_graph_create", "tsvd_graph_destroy", "tsvd_graph_append", "tsvd_graph_num_edges",
     "tsvd_graph_degrees", "tsvd_graph_export_csr", "tsvd_graph_set_staging_limit", "tsvd_read_edge_file", "tsvd_read_subset_file",
     "tsvd_read_lines", "tsvd_write_embedding_csv",
@@ -117,6 +117,9 @@ def lib():
         L.tsvd_memcpy.argtypes = [vp, vp, i64, ctypes.c_int]
         L.tsvd_profile_enable.argtypes = [ctypes.c_int]
         L.tsvd_profile_read.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        if hasattr(L, "tsvd_gemm_tuning"):
+            L.tsvd_gemm_tuning.argtypes = [ctypes.c_int]
+            L.tsvd_gemm_tuning_read.argtypes = [ctypes.c_char_p, ctypes.c_int]
         L.tsvd_graph_create.argtypes = [i32, ctypes.c_int, f64, vp]
         L.tsvd_graph_destroy.argtypes = [vp]
         L.tsvd_graph_append.argtypes = [vp, vp, vp, i64, ctypes.c_int]
@@ -696,6 +699,19 @@ def profile_enable(on: bool = True):
 
 def profile_reset():
     check(lib().tsvd_profile_reset())
+
+
+def gemm_tuning(on: bool = True):
+    """Tile-width autotuning of the MFMA products (tsvd_gemm_tuning)."""
+    check(lib().tsvd_gemm_tuning(int(on)))
+
+
+def gemm_tuning_read():
+    """{kind: {"tile_n", "ms_tile80", "ms_tile144", "calls"}} chosen so far."""
+    import json
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(lib().tsvd_gemm_tuning_read(buf, 1 << 16))
+    return json.loads(buf.value.decode())
 
 
 def probe_atomic_rate(rows=1 << 22, iters=64):
