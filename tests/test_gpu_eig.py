@@ -7,8 +7,8 @@ Bars, for every matrix of every batch: eigenvalues within 1e-13 ||A||_2 of
 LAPACK's, residual ||A V - V diag(lam)||_F <= 1e-12 ||A||_2, and
 ||V^T V - I||_max <= 1e-12.  The shapes are those the pipeline solves (Grams of
 l = k + 5 <= 136 columns in LDS, merges of f * k = 512 in global memory) and
-the edges around them: n = 1 and 2, the single-leaf limit 96/97, the LDS limit
-136/137, 1024.  The spectra are those of the pipeline's Grams (graded, rank-
+the edges around them: n = 1 and 2, the LDS limit 136/137, sizes whose halves
+split unevenly (97, 129, 385), 1024.  The spectra are those of the pipeline's Grams (graded, rank-
 deficient, zero-padded blocks as ragged trees give) and the divide and
 conquer's deflation cases (exact clusters, identity, zero, diagonal) and an
 indefinite matrix, all in one batch per size, so that batch slots are not
@@ -66,7 +66,7 @@ def check_batch(capi, mats, top=0):
     return lam, V, worst
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 7, 64, 96, 97, 133, 136, 137, 200, 512, 1024])
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 64, 96, 97, 129, 133, 136, 137, 200, 385, 512, 1024])
 def test_eig_against_lapack(capi, n):
     rng = np.random.default_rng(n)
     _, _, worst = check_batch(capi, spectra(n, rng))

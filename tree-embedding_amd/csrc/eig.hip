@@ -45,7 +45,7 @@ namespace tsvd {
 
 namespace {
 
-constexpr int SEG = 64;          // leaf segment rows: one wave holds a segment's d, e
+constexpr int SEG = 48;          // leaf segment rows: one wave holds a segment's d, e
 constexpr int LDS_N = 136;       // tridiagonalization in LDS up to this n
 constexpr int TRI_T = 1024;
 constexpr int TRG_T = 1024;
@@ -947,15 +947,46 @@ __global__ void __launch_bounds__(BACK_T) k_eig_back(const double *__restrict__ 
         }
         if (tid < nk) taus[tid] = tau[ja + tid];
         for (int x = tid; x < nk * 64; x += BACK_T) P[x / 64][x % 64] = 0.0;
+        for (int x = tid; x < NBK_MAX * NBK_MAX; x += BACK_T) Gm[x / NBK_MAX][x % NBK_MAX] = 0.0;
         __syncthreads();
-        // Gm = Vb^T Vb (upper triangle): one dot per wave at a time
-        for (int pq = wave; pq < nk * nk; pq += NW) {
-            const int p = pq / nk, q = pq - p * nk;
-            if (p > q) continue;
-            double s = 0.0;
-            for (int r = ja + q + 1 + lane; r < n; r += 64) s += Vb[p * n + r] * Vb[q * n + r];
-            s = wave_sum(s);
-            if (lane == 0) Gm[p][q] = s;
+        // one FP64 MFMA pass over the rows: P = Vb^T Z and Gm = Vb^T Vb (16 x 16 per
+        // tile, K = rows, 4 per step; each wave takes every NW-th step and adds its
+        // partial tiles to LDS).  Lane l: A(l & 15, l >> 4), B(l >> 4, l & 15),
+        // accumulator rows (l >> 4) + 4r of column l & 15.
+        {
+            const int li = lane & 15, lg = lane >> 4;
+            const int ntc = (nc + 15) >> 4;
+            f64x4 accP[4], accG;
+#pragma unroll
+            for (int u = 0; u < 4; u++) accP[u] = f64x4{0.0, 0.0, 0.0, 0.0};
+            accG = f64x4{0.0, 0.0, 0.0, 0.0};
+            for (int rb = rlo + 4 * wave; rb < n; rb += 4 * NW) {
+                const int r = rb + lg;
+                const bool rok = r < n;
+                const double a = (rok && li < nk) ? Vb[li * n + r] : 0.0;
+                const double bg = (rok && li < nk) ? Vb[li * n + r] : 0.0;
+                accG = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bg, accG, 0, 0, 0);
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (u < ntc) {
+                        const int c = 16 * u + li;
+                        const double bz = (rok && c < nc) ? Zs[r * cb + c] : 0.0;
+                        accP[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bz, accP[u], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int q = lg + 4 * rr;
+                if (q < nk) {
+                    if (li < nk) atomicAdd(&Gm[q][li], accG[rr]);
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int c = 16 * u + li;
+                        if (u < ntc && c < nc) atomicAdd(&P[q][c], accP[u][rr]);
+                    }
+                }
+            }
         }
         __syncthreads();
         // T, column by column (wave 0; lanes over rows of T)
@@ -963,7 +994,7 @@ __global__ void __launch_bounds__(BACK_T) k_eig_back(const double *__restrict__ 
             for (int q = 0; q < nk; q++) {
                 double tv = 0.0;
                 if (lane < q) {
-                    for (int s = lane; s < q; s++) tv += Tm[lane][s] * Gm[s][q];
+                    for (int s2 = lane; s2 < q; s2++) tv += Tm[lane][s2] * Gm[s2][q];
                     tv *= -taus[q];
                 }
                 if (lane < q) Tm[lane][q] = tv;
@@ -972,41 +1003,50 @@ __global__ void __launch_bounds__(BACK_T) k_eig_back(const double *__restrict__ 
                 __builtin_amdgcn_wave_barrier();
             }
         }
-        // P = Vb^T Z: waves over row slices, lanes over columns (nk sums per lane)
-        if (lane < nc) {
-            double acc[NBK_MAX];
-#pragma unroll
-            for (int q = 0; q < NBK_MAX; q++) acc[q] = 0.0;
-            for (int r = rlo + wave; r < n; r += NW) {
-                const double z = Zs[r * cb + lane];
-#pragma unroll
-                for (int q = 0; q < NBK_MAX; q++)
-                    if (q < nk) acc[q] += Vb[q * n + r] * z;
-            }
-#pragma unroll
-            for (int q = 0; q < NBK_MAX; q++)
-                if (q < nk) atomicAdd(&P[q][lane], acc[q]);
-        }
         __syncthreads();
         // WZ = T P
         for (int x = tid; x < nk * nc; x += BACK_T) {
             const int q = x / nc, c = x - q * nc;
-            double s = 0.0;
-            for (int t2 = q; t2 < nk; t2++) s += Tm[q][t2] * P[t2][c];
-            WZ[q][c] = s;
+            double sv = 0.0;
+            for (int t2 = q; t2 < nk; t2++) sv += Tm[q][t2] * P[t2][c];
+            WZ[q][c] = sv;
         }
         __syncthreads();
-        // Z -= Vb WZ
-        if (lane < nc) {
-            double wz[NBK_MAX];
+        // Z -= Vb WZ on FP64 MFMA: 16-row tiles per wave, K = nk in steps of 4
+        {
+            const int li = lane & 15, lg = lane >> 4;
+            const int ntc = (nc + 15) >> 4;
+            for (int R0 = (rlo & ~15) + 16 * wave; R0 < n; R0 += 16 * NW) {
+                const int ar = R0 + li;
+                double af[NBK_MAX / 4];
 #pragma unroll
-            for (int q = 0; q < NBK_MAX; q++) wz[q] = q < nk ? WZ[q][lane] : 0.0;
-            for (int r = rlo + wave; r < n; r += NW) {
-                double s = 0.0;
+                for (int kb = 0; kb < NBK_MAX / 4; kb++) {
+                    const int q = 4 * kb + lg;
+                    af[kb] = (q < nk && ar >= rlo && ar < n) ? -Vb[q * n + ar] : 0.0;
+                }
 #pragma unroll
-                for (int q = 0; q < NBK_MAX; q++)
-                    if (q < nk) s += Vb[q * n + r] * wz[q];
-                Zs[r * cb + lane] -= s;
+                for (int u = 0; u < 4; u++) {
+                    if (u >= ntc) continue;
+                    const int c = 16 * u + li;
+                    const bool cok = c < nc;
+                    f64x4 acc;
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) {
+                        const int row = R0 + lg + 4 * rr;
+                        acc[rr] = (cok && row < n) ? Zs[row * cb + c] : 0.0;
+                    }
+#pragma unroll
+                    for (int kb = 0; kb < NBK_MAX / 4; kb++) {
+                        const int q = 4 * kb + lg;
+                        const double bv = (cok && q < nk) ? WZ[q][c] : 0.0;
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[kb], bv, acc, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) {
+                        const int row = R0 + lg + 4 * rr;
+                        if (cok && row >= rlo && row < n) Zs[row * cb + c] = acc[rr];
+                    }
+                }
             }
         }
         __syncthreads();
@@ -1020,25 +1060,30 @@ struct Plan {
     std::vector<MergeTable> levels;
 };
 
+// Every segment is split to the same depth (the smallest at which all leaves
+// have <= SEG rows), so each merge level's two buffers hold every block: the
+// levels alternate between Z0 and Z1 and a block that skipped a level would
+// be left behind in the other one.
 Plan make_plan(int n)
 {
     Plan p;
-    std::vector<std::vector<std::array<int, 3>>> lv;
-    std::function<int(int, int)> split = [&](int a, int len) -> int {
-        if (len <= SEG) {
+    int depth = 0;
+    while ((n + (1 << depth) - 1) >> depth > SEG) depth++;
+    std::vector<std::vector<std::array<int, 3>>> lv(depth);
+    std::function<void(int, int, int)> split = [&](int a, int len, int h) {
+        if (h == 0) {
             TSVD_REQUIRE(p.leaves.count < MAX_PIECES, "eigensolver: too many segments");
             p.leaves.a[p.leaves.count] = a;
             p.leaves.len[p.leaves.count] = len;
             p.leaves.count++;
-            return 0;
+            return;
         }
         const int n1 = len / 2;
-        const int h = 1 + std::max(split(a, n1), split(a + n1, len - n1));
-        if ((int)lv.size() < h) lv.resize(h);
+        split(a, n1, h - 1);
+        split(a + n1, len - n1, h - 1);
         lv[h - 1].push_back({a, n1, len - n1});
-        return h;
     };
-    split(0, n);
+    split(0, n, depth);
     for (auto &l : lv) {
         MergeTable t{};
         for (auto &x : l) {
