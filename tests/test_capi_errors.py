@@ -1,8 +1,10 @@
 """Error paths of the C ABI that need no GPU: bad arguments are refused with
-an error code and message, and a server whose creation fails part way (here:
-no device, so its pipeline cannot be created) is torn down without aborting
-the process.  Each case runs in a child process so that an abort would be
-reported as a failure rather than end the test run."""
+an error code and message, and a server whose creation fails before any of
+its threads starts (here: no device, so its pipeline cannot be created) is
+torn down without aborting the process.  The failure after the finisher
+thread has started is covered on the device by
+test_gpu_server_teardown.py.  Each case runs in a child process so that an
+abort would be reported as a failure rather than end the test run."""
 import os
 import subprocess
 import sys

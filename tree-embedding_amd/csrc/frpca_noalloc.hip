@@ -9,12 +9,13 @@
 // ends), sized for m <= 4096 rows, n <= 32768 columns and k + 5 <= 136:
 //   Qt   n x l   (row-major: one row of the sketch side per column of A)
 //   Q,Q2 m x l   (row-major)
-//   G, W l x l   (FP64 Gram / rotation), lam l, GEMM descriptors, flags.
+//   G, W l x l   (FP64 Gram / rotation), lam l, GEMM descriptors, flags,
+//   and the eigensolver's workspace (eig_workspace_bytes(136, 1)).
 // The sparse products are hand-written CSR kernels on those row-major
 // operands (one wavefront per row of A, its column ids and values broadcast
 // from one coalesced load; A^T Y scatters with FP64 atomic adds), the Grams
-// and rotations run on the grouped MFMA GEMM, the eigensolves on the LDS
-// Jacobi solver (rocSOLVER would allocate).  The sketch draws the same
+// and rotations run on the grouped MFMA GEMM, the eigensolves on the
+// library's own batched solver (eig.hip) in that workspace.  The sketch draws the same
 // numbers as tsvd_frpca_csr with the same seed, so the two agree to rounding.
 // Calls are serialised by a mutex; concurrent calls from other processes use
 // their own copy of the array.

@@ -32,6 +32,7 @@
 #include "tsvd_pipeline.h"
 
 #include <chrono>
+#include <cstdlib>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -240,6 +241,10 @@ extern "C" int tsvd_server_create(const tsvd_config *cfg, const tsvd_server_conf
             TSVD_HIP(hipStreamCreateWithFlags(&s->back_stream, hipStreamNonBlocking));
             s->finisher = std::thread([p = s.get()] { p->finish_loop(); });
         }
+        // TSVD_FAIL_AFTER_FINISHER (tests only): fail here, with the finisher
+        // running and the worker not started, to exercise the destructor's
+        // teardown of a half-built server
+        TSVD_REQUIRE(!getenv("TSVD_FAIL_AFTER_FINISHER"), "server creation failed after its finisher started");
         if (sc.max_step_edges > 0) {
             s->edge_budget = sc.max_step_edges;
         } else {
