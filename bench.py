@@ -488,6 +488,7 @@ def main():
     log(f"push: {wave_atomics:.3e} wave-atomics in {push_ms:.0f} ms = {achieved_rate / 1e9:.2f} G/s; "
         f"probe ceiling {ceiling / 1e9:.2f} G/s (non-returning {probe_noret / 1e9:.2f}); "
         f"{line_rate / 1e9:.2f} G lines/s against {probe_line / 1e9:.2f} G lines/s for one-line atomics")
+    graph_replays, graph_captures, graph_failures = pipe.ppr.graph_stats()
     ndirty = float(np.mean([s["ndirty"] for s in stats]))
     nmerged = float(np.mean([s["nmerged"] for s in stats]))
     phases = {k: round(float(np.mean([s[k] for s in stats])), 3)
@@ -521,6 +522,10 @@ def main():
                     "note": "when warm-up + steps exceed the workload's insertion batches, the pipeline is rebuilt "
                             "from the initial snapshot outside the clock and the batches replay from the first"},
         "dirty_blocks_per_step": ndirty, "merged_nodes_per_step": nmerged, "pair_list_state": state,
+        "norm_items_per_step": float(np.mean([s["norm_items"] for s in stats])),
+        "push_graphs": {"replays": graph_replays, "captures": graph_captures, "failed_captures": graph_failures,
+                        "note": "push rounds replayed from captured hipGraphs, six rounds per replay, counted over "
+                                "the last pipeline (rebuilt at each replay of the batches)"},
         "push_tuning": tuning,
         "gemm_tile_tuning": gemm_tuning,
         "push_atomic_roofline": {"achieved": round(achieved_rate / 1e9, 3), "ceiling": round(ceiling / 1e9, 3),

@@ -148,6 +148,14 @@ int tsvd_ppr_read(tsvd_ppr *s, int dir, int what, double *out);
  * within the push error bound of the exact PPR after dynamic updates.  No
  * reference counterpart. */
 int tsvd_ppr_set_two_sided(tsvd_ppr *s, int on);
+/* Push rounds replayed from captured hipGraphs, six rounds per replay (on = 1,
+ * the default; TSVD_PUSH_GRAPHS=0 at creation turns it off), or launched one
+ * by one (0).  No counterpart in the reference, whose push is a host loop
+ * (ppr:1426-1487); the results are the same either way within the push
+ * bound.  tsvd_ppr_graph_stats: replays, captures and failed captures of the
+ * handle so far (a failed capture makes the handle launch rounds one by one). */
+int tsvd_ppr_set_graphs(tsvd_ppr *s, int on);
+int tsvd_ppr_graph_stats(const tsvd_ppr *s, int64_t *launches, int64_t *captures, int64_t *failures);
 /* Counters of the last push on `dir`: rounds, popped (vertex, source) pairs,
  * edge relaxations (pairs x degree), and the push kernel time in ms. */
 int tsvd_ppr_last_push_stats(tsvd_ppr *s, int dir, int64_t *rounds, int64_t *pops, int64_t *relax,
@@ -261,6 +269,8 @@ typedef struct {
                                * resident batch, the state kept in HBM without lists)  */
     int64_t push_line_relax;  /* 128-byte lines the relaxation atomics touched: neighbour
                                * visits x lines of the group holding a pushing lane    */
+    int64_t norm_items;       /* (vertex, 64-source group) items whose block-norm sums
+                               * were recomputed (the others had unchanged reserves)  */
 } tsvd_snapshot_stats;
 
 int tsvd_pipeline_create(const tsvd_config *cfg, tsvd_pipeline **out);

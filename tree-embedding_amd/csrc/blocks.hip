@@ -88,6 +88,113 @@ __global__ void __launch_bounds__(TPB) k_block_norms(const T *__restrict__ p0, c
     }
 }
 
+// The same sums per (vertex, source group) item, for the items flagged in
+// `changed` (or every item when `all`).  One wave takes 64 consecutive items:
+// each lane reads one flag (and clears it), then the wave walks the flagged
+// ones, lanes over the item's 64 sources, and writes the item's two sums.
+// The transform and the stored-column lookup are those of k_block_norms, so
+// a block's sums over its items are that kernel's sums in another order.
+constexpr int DONE_SHARDS = 32;
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_item_norms(const T *__restrict__ p0, const T *__restrict__ p1,
+                                                    const T *__restrict__ A, const int32_t *__restrict__ slot,
+                                                    uint8_t *__restrict__ changed, int all, int64_t nitems,
+                                                    int32_t ngroups, int32_t nsrc, int32_t s_pad, int32_t nparts,
+                                                    int32_t common, int transform, T rm, double *__restrict__ sums,
+                                                    unsigned long long *__restrict__ done)
+{
+    __shared__ unsigned long long sdone;
+    if (threadIdx.x == 0) sdone = 0ull;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (int64_t)gridDim.x * (TPB / 64);
+    unsigned long long mine_done = 0;
+    for (int64_t w = (int64_t)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6); w * 64 < nitems; w += nw) {
+        const int64_t it0 = w * 64, mine = it0 + lane;
+        const bool want = mine < nitems && (all || changed[mine] != 0);
+        if (mine < nitems && (all || want)) changed[mine] = 0;
+        unsigned long long bal = __ballot(want);
+        while (bal != 0ull) {
+            const int q = __builtin_ctzll(bal);
+            bal &= bal - 1ull;
+            const int64_t it = it0 + q;
+            const int64_t v = it / ngroups;
+            const int32_t s = (int32_t)(it - v * ngroups) * 64 + lane;
+            const int64_t j = min(v / common, (int64_t)nparts - 1);
+            const int32_t sl = slot[v];
+            double a2 = 0.0, e2 = 0.0;
+            if (s < nsrc) {
+                T x = p0[v * s_pad + s];
+                if (p1) x += p1[v * s_pad + s];
+                const T a = xform<T>(x, rm, transform);
+                const T ao = sl >= 0 ? A[(j * common + sl) * s_pad + s] : T(0);
+                a2 = (double)a * (double)a;
+                const double d = (double)a - (double)ao;
+                e2 = d * d;
+            }
+            a2 = wave_sum(a2);
+            e2 = wave_sum(e2);
+            if (lane == 0) {
+                sums[2 * it] = a2;
+                sums[2 * it + 1] = e2;
+            }
+            mine_done++;
+        }
+    }
+    if (lane == 0 && mine_done) atomicAdd(&sdone, mine_done);
+    __syncthreads();
+    if (threadIdx.x == 0 && sdone) atomicAdd(done + blockIdx.x % DONE_SHARDS, sdone);
+}
+
+// Block sums over the item sums: `split` workgroups per block, each adding
+// its part into norms[2j], norms[2j + 1] (zeroed before).
+__global__ void __launch_bounds__(TPB) k_block_sums(const double *__restrict__ sums, int32_t n, int32_t ngroups,
+                                                    int32_t nparts, int32_t common, int32_t j0,
+                                                    double *__restrict__ norms)
+{
+    const int32_t j = j0 + (int32_t)blockIdx.y;
+    const int64_t start = (int64_t)j * common;
+    const int64_t width = (j == nparts - 1) ? n - start : common;
+    const int64_t i0 = start * ngroups, i1 = (start + width) * ngroups;
+    double a2 = 0.0, e2 = 0.0;
+    for (int64_t i = i0 + (int64_t)blockIdx.x * TPB + threadIdx.x; i < i1; i += (int64_t)gridDim.x * TPB) {
+        a2 += sums[2 * i];
+        e2 += sums[2 * i + 1];
+    }
+    a2 = wave_sum(a2);
+    e2 = wave_sum(e2);
+    __shared__ double red[2][TPB / 64];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][wave] = a2;
+        red[1][wave] = e2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0.0, se = 0.0;
+        for (int w = 0; w < TPB / 64; w++) {
+            sa += red[0][w];
+            se += red[1][w];
+        }
+        atomicAdd(norms + 2 * j, sa);
+        atomicAdd(norms + 2 * j + 1, se);
+    }
+}
+
+// A rebuilt block stores the transform of the reserves it was built from, so
+// the difference sums of its items are zero until they change again.
+__global__ void __launch_bounds__(TPB) k_zero_diff(const int32_t *__restrict__ blocks, int32_t n, int32_t ngroups,
+                                                   int32_t nparts, int32_t common, double *__restrict__ sums)
+{
+    const int32_t j = blocks[blockIdx.y];
+    const int64_t start = (int64_t)j * common;
+    const int64_t width = (j == nparts - 1) ? n - start : common;
+    const int64_t i0 = start * ngroups, i1 = (start + width) * ngroups;
+    for (int64_t i = i0 + (int64_t)blockIdx.x * TPB + threadIdx.x; i < i1; i += (int64_t)gridDim.x * TPB)
+        sums[2 * i + 1] = 0.0;
+}
+
 // Compact slots of the active columns of each listed block (one workgroup per
 // block; vertices in id order keep their order).
 __global__ void __launch_bounds__(TPB) k_slots(const int32_t *__restrict__ blocks, const uint8_t *__restrict__ active,
@@ -333,6 +440,67 @@ void blocks_norms_range(Blocks &b, const void *p0, const void *p1, const tsvd_gr
     }
 }
 
+void blocks_norms_items(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g, uint8_t *changed, bool all,
+                        int32_t j0, int32_t j1, hipStream_t st)
+{
+    if (j1 <= j0) return;
+    TSVD_REQUIRE(changed, "no changed-item flags");
+    const int32_t ngroups = b.s_pad / 64;
+    const int64_t nitems = (int64_t)b.n * ngroups;
+    if (b.item_sums.n == 0) {
+        b.item_sums.alloc((size_t)(2 * nitems));
+        b.item_done.alloc(DONE_SHARDS);
+        all = true;
+    }
+    hipLaunchKernelGGL(k_active, dim3((unsigned)ceil_div(b.n, TPB)), dim3(TPB), 0, st, g->dir[0].deg.p,
+                       g->ndir > 1 ? g->dir[1].deg.p : nullptr, b.is_source.p, b.n, b.active.p);
+    TSVD_HIP(hipMemsetAsync(b.norms.p, 0, b.norms.bytes(), st));
+    TSVD_HIP(hipMemsetAsync(b.item_done.p, 0, b.item_done.bytes(), st));
+    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(ceil_div(nitems, 64), TPB / 64), 16384);
+    int64_t span = 0, maxw = 0;
+    for (int32_t j = j0; j < j1; j++) {
+        span += b.width[j];
+        maxw = std::max(maxw, b.width[j]);
+    }
+    const unsigned split = (unsigned)std::min<int64_t>(std::max<int64_t>(ceil_div(maxw * ngroups, TPB * 16), 1), 64);
+    {
+        PROF("blocks_norms", st);
+        if (b.precision == TSVD_FP64)
+            hipLaunchKernelGGL(k_item_norms<double>, dim3(grid), dim3(TPB), 0, st, (const double *)p0,
+                               (const double *)p1, (const double *)b.A.p, b.slot.p, changed, all ? 1 : 0, nitems,
+                               ngroups, b.nsrc, b.s_pad, b.nparts, b.common, b.transform, b.rmax, b.item_sums.p,
+                               b.item_done.p);
+        else
+            hipLaunchKernelGGL(k_item_norms<float>, dim3(grid), dim3(TPB), 0, st, (const float *)p0,
+                               (const float *)p1, (const float *)b.A.p, b.slot.p, changed, all ? 1 : 0, nitems,
+                               ngroups, b.nsrc, b.s_pad, b.nparts, b.common, b.transform, (float)b.rmax,
+                               b.item_sums.p, b.item_done.p);
+        TSVD_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_block_sums, dim3(split, (unsigned)(j1 - j0)), dim3(TPB), 0, st, b.item_sums.p, b.n,
+                           ngroups, b.nparts, b.common, j0, b.norms.p);
+        TSVD_HIP(hipGetLastError());
+    }
+    std::vector<double> h((size_t)2 * b.nparts);
+    unsigned long long done[DONE_SHARDS];
+    TSVD_HIP(hipMemcpyAsync(h.data(), b.norms.p, b.norms.bytes(), hipMemcpyDeviceToHost, st));
+    TSVD_HIP(hipMemcpyAsync(done, b.item_done.p, sizeof(done), hipMemcpyDeviceToHost, st));
+    TSVD_HIP(hipStreamSynchronize(st));
+    b.last_items = 0;
+    for (int i = 0; i < DONE_SHARDS; i++) b.last_items += (int64_t)done[i];
+    if (g_profile) {
+        // recomputed items: their reserves (one or two directions) and stored
+        // values; every item's flag; the item sums of the range
+        const double dirs = p1 ? 2.0 : 1.0;
+        prof_add_work("blocks_norms",
+                      (double)b.last_items * 64.0 * (double)b.esize * (dirs + 1.0) + (double)nitems +
+                          16.0 * (double)span * ngroups, 0.0);
+    }
+    for (int32_t j = j0; j < j1; j++) {
+        b.a2[j] = h[2 * j];
+        b.e2[j] = h[2 * j + 1];
+    }
+}
+
 void blocks_rebuild(Blocks &b, const std::vector<int32_t> &dirty, const void *p0, const void *p1, hipStream_t st)
 {
     if (dirty.empty()) return;
@@ -358,6 +526,13 @@ void blocks_rebuild(Blocks &b, const std::vector<int32_t> &dirty, const void *p0
                            (const float *)p1, (float *)b.A.p, b.colmap.p, b.d_ncols.p, b.nsrc,
                            b.s_pad, b.common, b.transform, (float)b.rmax);
     TSVD_HIP(hipGetLastError());
+    if (b.item_sums.n > 0) {
+        const int32_t ngroups = b.s_pad / 64;
+        const unsigned split = (unsigned)std::min<int64_t>(std::max<int64_t>(ceil_div(maxw * ngroups, TPB * 16), 1), 64);
+        hipLaunchKernelGGL(k_zero_diff, dim3(split, (unsigned)nd), dim3(TPB), 0, st, b.d_dirty.p, b.n, ngroups,
+                           b.nparts, b.common, b.item_sums.p);
+        TSVD_HIP(hipGetLastError());
+    }
     TSVD_HIP(hipMemcpyAsync(b.ncols.data(), b.d_ncols.p, sizeof(int32_t) * b.nparts, hipMemcpyDeviceToHost, st));
     TSVD_HIP(hipStreamSynchronize(st));
 }

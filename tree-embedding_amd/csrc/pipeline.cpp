@@ -374,7 +374,11 @@ bool pipeline_front(tsvd_pipeline *p, const int32_t *src, const int32_t *dst, in
         }
         const double t3 = now_ms();
         S.ms_push = t3 - t2;
-        blocks_norms(b, p->ppr->p(0), p->ppr->ndir > 1 ? p->ppr->p(1) : nullptr, p->g, st);
+        // only the items whose reserves the update and the push changed are re-read
+        blocks_norms_items(b, p->ppr->p(0), p->ppr->ndir > 1 ? p->ppr->p(1) : nullptr, p->g, p->ppr->touched.p,
+                           p->ppr->touched_all, 0, b.nparts, st);
+        p->ppr->touched_all = false;
+        S.norm_items = b.last_items;
         S.ms_blocks = now_ms() - t3;
     }
     graph_release_edges(p->g);
@@ -628,7 +632,8 @@ extern "C" int tsvd_pipeline_block_transform(tsvd_pipeline *p, int32_t j0, int32
         if (j1 == j0) return;
         hipStream_t st = p->g->stream;
         const void *p0 = p->ppr->p(0), *p1 = p->ppr->ndir > 1 ? p->ppr->p(1) : nullptr;
-        blocks_norms_range(b, p0, p1, p->g, j0, j1, st);
+        blocks_norms_items(b, p0, p1, p->g, p->ppr->touched.p, p->ppr->touched_all, j0, j1, st);
+        p->ppr->touched_all = false;
         // ppr:2718-2730: clean (-1) when resid + ||A_new - A_old|| < delta sqrt(2) ||A_new||
         const double thr = delta * std::sqrt(2.0);
         std::vector<int32_t> rebuild;

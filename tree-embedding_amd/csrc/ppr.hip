@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict
                                                 int32_t s0, int32_t s1, double alpha, T rm, int two,
                                                 unsigned long long *__restrict__ cm, uint32_t *__restrict__ list,
                                                 uint32_t *__restrict__ count, uint32_t cap,
-                                                uint32_t *__restrict__ overflow)
+                                                uint32_t *__restrict__ overflow, uint8_t *__restrict__ changed)
 {
     TSVD_STAGE(sg, list, count, cap, overflow);
     const int64_t wave = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
@@ -250,6 +250,8 @@ __global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict
     }
     unsigned long long c2 = __ballot(self);
     if (c2 && lane == 0) stage_mark(sg, c2, (uint32_t)i * ngroups + g, cm);
+    // the row's reserves were rescaled: the block norms re-read this item
+    if (__ballot(active) != 0ull && lane == 0) changed[(int64_t)i * ngroups + g] = 1;
     }
     stage_flush(sg);
 }
@@ -469,20 +471,23 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                                                     unsigned long long *__restrict__ nm, uint32_t *__restrict__ next,
                                                     uint32_t *__restrict__ next_count, uint32_t cap,
                                                     uint32_t *__restrict__ overflow,
-                                                    const int64_t *__restrict__ rowoff,
-                                                    const int32_t *__restrict__ adj, const int32_t *__restrict__ deg,
+                                                    const PushCsr *__restrict__ csr,
                                                     int32_t s_pad, int32_t ngroups, double alpha, T rm, int two,
                                                     int directed, T ride_min, int32_t ride_s0, int32_t ride_s1,
                                                     unsigned long long *__restrict__ stats,
                                                     uint32_t *__restrict__ hub_item, T *__restrict__ hub_delta,
                                                     unsigned long long *__restrict__ hub_chunks,
                                                     uint32_t *__restrict__ hub_counts, uint32_t hub_cap,
-                                                    uint32_t chunk_cap, int32_t hub_chunk)
+                                                    uint32_t chunk_cap, int32_t hub_chunk,
+                                                    uint8_t *__restrict__ changed)
 {
     // [0] pops, [1] relaxations (lanes x degree), [2] items, [3] row visits,
     // [4] relaxations done here (split rows hand the rest to k_push_hub),
     // [5] 128-byte lines the row's atomics touch (row visits x lines holding
     //     a pushing lane: the L2 atomic units work per line, not per lane)
+    const int64_t *__restrict__ rowoff = csr->rowoff;
+    const int32_t *__restrict__ adj = csr->adj;
+    const int32_t *__restrict__ deg = csr->deg;
     __shared__ unsigned long long sstat[6];
     if (threadIdx.x < 6) sstat[threadIdx.x] = 0ull;
     const uint32_t nitems = min(*count_in, in_cap);
@@ -502,6 +507,9 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
         const int32_t g = (int32_t)(item - (uint32_t)v * (uint32_t)ngroups);
         const unsigned long long m = cm[item];
         if (lane == 0) cm[item] = 0ull;
+        // a popped lane may change its reserve (so may riders, which need a
+        // pushing lane): the block norms re-read this item
+        if (lane == 0 && m != 0ull) changed[item] = 1;
         const int32_t s = g * 64 + lane;
         const bool active = (m >> lane) & 1ull;
         const int32_t dv = deg[v];
@@ -606,10 +614,12 @@ __global__ void __launch_bounds__(TPB) k_push_hub(T *__restrict__ R, const uint3
                                                   unsigned long long *__restrict__ nm, uint32_t *__restrict__ next,
                                                   uint32_t *__restrict__ next_count, uint32_t cap,
                                                   uint32_t *__restrict__ overflow,
-                                                  const int64_t *__restrict__ rowoff,
-                                                  const int32_t *__restrict__ adj, const int32_t *__restrict__ deg,
+                                                  const PushCsr *__restrict__ csr,
                                                   int32_t s_pad, int32_t ngroups, T rm, int two, int32_t hub_chunk)
 {
+    const int64_t *__restrict__ rowoff = csr->rowoff;
+    const int32_t *__restrict__ adj = csr->adj;
+    const int32_t *__restrict__ deg = csr->deg;
     TSVD_STAGE(sg, next, next_count, cap, overflow);
     const uint32_t nchunks = min(*chunk_count, chunk_cap);
     const int lane = threadIdx.x & 63;
@@ -789,6 +799,7 @@ void refresh_t(tsvd_ppr *s, int mode, int d, int32_t s0, int32_t s1)
     const uint32_t cap = (uint32_t)s->F[d].n;
     uint32_t *cnt = s->counters.p;
     if (mode == TSVD_REFRESH_INIT) {
+        s->touched_all = true;
         if (s0 == 0 && s1 == s->nsrc) {
             TSVD_HIP(hipMemsetAsync(P, 0, s->P[d].bytes(), st));
             TSVD_HIP(hipMemsetAsync(R, 0, s->R[d].bytes(), st));
@@ -813,7 +824,7 @@ void refresh_t(tsvd_ppr *s, int mode, int d, int32_t s0, int32_t s1)
             hipLaunchKernelGGL(k_update<T>, dim3((unsigned)ceil_div(waves, WPB)), dim3(TPB), 0, st, P, R,
                                D.touched.p, D.ntouched, D.rowoff.p, D.adj.p, D.deg.p, D.former.p, s->s_pad,
                                s->ngroups, s0, s1, s->alpha, (T)s->rmax, s->two_sided, s->cm[d].p, s->F[d].p,
-                               cnt + d, cap, cnt + 3);
+                               cnt + d, cap, cnt + 3, s->touched.p);
         }
     }
     TSVD_HIP(hipGetLastError());
@@ -875,19 +886,41 @@ static uint32_t cu_count()
     return (uint32_t)cus[dev];
 }
 
+// A launch's work-item count (workgroups x 256) must stay below 2^32; larger
+// rounds stride over their items.
+constexpr uint64_t MAX_ROUND_GRID = 1u << 23;
+
 static uint32_t round_grid(uint32_t count_hint)
 {
     const uint64_t want = (uint64_t)ceil_div((int64_t)count_hint + count_hint / 2, WPB);
-    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, (uint64_t)cu_count() * 2), 1u << 24);
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, (uint64_t)cu_count() * 2), MAX_ROUND_GRID);
 }
 
 // Rounds until no item is pending.  Push rounds are device-driven: each reads
 // its item count from a ring of three device counters (the count it was
 // given, the one it appends to, and the one it clears for the round after),
-// so the host enqueues ROUNDS_PER_SYNC rounds, then reads one count.  Pull
-// rounds (off unless TSVD_PULL_FRAC is set) are chosen by the frontier size,
-// so with them enabled the host reads the count after every round.
+// so the host enqueues several rounds, then reads one count.  Pull rounds
+// (off unless TSVD_PULL_FRAC is set) are chosen by the frontier size, so with
+// them enabled the host reads the count after every round.
 constexpr int ROUNDS_PER_SYNC = 8;
+
+// Push rounds replayed from a captured hipGraph: GRAPH_ROUNDS rounds (round
+// and split-row kernels) per replay.  Six is the period of the three-slot
+// ring and of the double-buffered masks, lists and split-row counters, so a
+// replay starts from the same buffers as the capture did and one graph
+// serves every block of a push.  A graph is kept per launch shape (the round
+// grid is rounded up to a power of two, so a few graphs cover a push) and
+// captured again when any launch argument changes (a buffer grown or moved).
+constexpr int GRAPH_ROUNDS = 6;
+constexpr size_t GRAPH_CACHE = 16;
+
+static uint32_t graph_grid(uint32_t count_hint)
+{
+    const uint32_t g = round_grid(count_hint);
+    uint32_t p = 1;
+    while (p < g) p <<= 1;
+    return std::min<uint32_t>(p, MAX_ROUND_GRID);
+}
 
 template <class T>
 void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
@@ -902,6 +935,11 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
     TSVD_HIP(hipMemsetAsync(s->stats.p, 0, s->stats.bytes(), st));
     // ring slots cnt[8], cnt[9], split-row sets cnt[4..5] / cnt[6..7], rounds cnt[10]
     TSVD_HIP(hipMemsetAsync(cnt + 4, 0, 7 * sizeof(uint32_t), st));
+    // the direction's CSR arrays (an append may have moved them), read by the
+    // push kernels through this descriptor so that captured graphs stay valid
+    PushCsr *csr = s->csr_ref.p + d;
+    s->h_csr_ref[d] = PushCsr{D.rowoff.p, D.adj.p, D.deg.p};
+    TSVD_HIP(hipMemcpyAsync(csr, s->h_csr_ref + d, sizeof(PushCsr), hipMemcpyHostToDevice, st));
     TSVD_HIP(hipMemcpyAsync(h, cnt, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     TSVD_HIP(hipStreamSynchronize(st));
     TSVD_REQUIRE(h[3] == 0, "frontier overflow");
@@ -910,79 +948,202 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
     uint32_t *ring[3] = {cnt + d, cnt + 8, cnt + 9};
     uint32_t *hubset[2] = {cnt + 4, cnt + 6};
     const bool pull_enabled = !s->two_sided && s->ngroups <= 64 && s->pull_frac > 0.0;
-    const int per_sync = pull_enabled ? 1 : ROUNDS_PER_SYNC;
-    // split rows: at most one chunk list entry per (hub row, group) per hub_chunk neighbours
-    const uint32_t hub_cap = (uint32_t)std::min<int64_t>((int64_t)cap, (int64_t)s->ngroups *
-                                                         (ceil_div(D.cap_total, s->hub_chunk) + 1));
+    bool graphs = s->use_graphs != 0 && !pull_enabled;
+    // split rows: at most one chunk list entry per (hub row, group) per hub_chunk
+    // neighbours (rounded up to a power of two in graph mode, so that the
+    // capacities, which are launch arguments, change rarely)
+    uint32_t hub_cap = (uint32_t)std::min<int64_t>((int64_t)cap, (int64_t)s->ngroups *
+                                                   (ceil_div(D.cap_total, s->hub_chunk) + 1));
+    if (graphs) {
+        uint64_t p2 = 1;
+        while (p2 < hub_cap) p2 <<= 1;
+        hub_cap = (uint32_t)std::min<uint64_t>(p2, cap);
+    }
     const uint32_t chunk_cap = hub_cap;
     s->hub_item.ensure(std::max<uint32_t>(hub_cap, 1));
     s->hub_delta.ensure((size_t)std::max<uint32_t>(hub_cap, 1) * 64 * sizeof(T));
     s->hub_chunks.ensure(std::max<uint32_t>(chunk_cap, 1));
+    const T ride_min = (T)(std::min(s->ride_frac, 1.0) * s->rmax);
+    const uint32_t hub_grid = cu_count() * 8;
+    auto launch_round = [&](uint64_t ri, uint32_t grid) {
+        uint32_t *in = ring[ri % 3], *next = ring[(ri + 1) % 3], *ahead = ring[(ri + 2) % 3];
+        uint32_t *hub = hubset[ri % 2], *hub_ahead = hubset[(ri + 1) % 2];
+        hipLaunchKernelGGL(k_push_round<T>, dim3(grid), dim3(TPB), 0, st, P, R, s->F[d].p, in, cap, ahead,
+                           hub_ahead, cnt + 10, s->cm[d].p, s->nm.p, s->fnext.p, next, cap, cnt + 3, csr,
+                           s->s_pad, s->ngroups, s->alpha, (T)s->rmax, s->two_sided, s->directed,
+                           ride_min, s0, s1, s->stats.p, s->hub_item.p, (T *)s->hub_delta.p, s->hub_chunks.p, hub,
+                           hub_cap, chunk_cap, s->hub_chunk, s->touched.p);
+    };
+    auto launch_hub = [&](uint64_t ri) {
+        uint32_t *next = ring[(ri + 1) % 3], *hub = hubset[ri % 2];
+        hipLaunchKernelGGL(k_push_hub<T>, dim3(hub_grid), dim3(TPB), 0, st, R, s->hub_item.p,
+                           (const T *)s->hub_delta.p, s->hub_chunks.p, hub + 1, chunk_cap, s->nm.p, s->fnext.p,
+                           next, cap, cnt + 3, csr, s->s_pad, s->ngroups, (T)s->rmax, s->two_sided,
+                           s->hub_chunk);
+    };
+    // The graph for this launch shape: found, or captured from the same
+    // launches (their host-side buffer swaps come back to the start after
+    // GRAPH_ROUNDS rounds).  nullptr when capture is not possible.
+    auto block_graph = [&](uint64_t i0, uint32_t grid) -> PushGraph * {
+        const bool prof = g_profile;
+        auto bits = [](double x) {
+            uint64_t u;
+            memcpy(&u, &x, sizeof(u));
+            return u;
+        };
+        const std::vector<uint64_t> key = {
+            sizeof(T), (uint64_t)d, grid, hub_grid, (uint64_t)P, (uint64_t)R, (uint64_t)s->F[d].p,
+            (uint64_t)s->fnext.p, (uint64_t)s->cm[d].p, (uint64_t)s->nm.p, (uint64_t)cnt, cap, hub_cap,
+            (uint64_t)s->hub_item.p, (uint64_t)s->hub_delta.p, (uint64_t)s->hub_chunks.p, (uint64_t)s->stats.p,
+            (uint64_t)s->touched.p, (uint64_t)csr,
+            (uint64_t)s->s_pad, (uint64_t)s->ngroups, bits(s->alpha), bits(s->rmax), (uint64_t)s->two_sided,
+            (uint64_t)s->directed, bits((double)ride_min), (uint64_t)s0, (uint64_t)s1, (uint64_t)s->hub_chunk,
+            (uint64_t)prof, i0 % GRAPH_ROUNDS};
+        for (auto &pg : s->graphs)
+            if (pg->key == key) {
+                pg->used = ++s->graph_clock;
+                return pg.get();
+            }
+        auto pg = std::make_unique<PushGraph>();
+        pg->key = key;
+        if (prof) {
+            pg->ev.resize(2 * GRAPH_ROUNDS + 1, nullptr);
+            for (auto &e : pg->ev)
+                if (hipEventCreate(&e) != hipSuccess) {
+                    (void)hipGetLastError();
+                    return nullptr;
+                }
+        }
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        bool ok = true;
+        for (int k = 0; k < GRAPH_ROUNDS; k++) {
+            if (prof) ok &= hipEventRecordWithFlags(pg->ev[2 * k], st, hipEventRecordExternal) == hipSuccess;
+            launch_round(i0 + k, grid);
+            if (prof) ok &= hipEventRecordWithFlags(pg->ev[2 * k + 1], st, hipEventRecordExternal) == hipSuccess;
+            launch_hub(i0 + k);
+            std::swap(s->cm[d], s->nm);
+            std::swap(s->F[d], s->fnext);
+        }
+        if (prof) ok &= hipEventRecordWithFlags(pg->ev[2 * GRAPH_ROUNDS], st, hipEventRecordExternal) == hipSuccess;
+        ok &= hipGetLastError() == hipSuccess;
+        const hipError_t ec = hipStreamEndCapture(st, &graph);
+        ok &= ec == hipSuccess && graph != nullptr;
+        if (ok) ok = hipGraphInstantiate(&pg->exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!ok) {
+            const hipError_t e = hipGetLastError();
+            fprintf(stderr, "treesvd: push graph capture failed (grid %u, %s); rounds are launched one by one\n",
+                    grid, hipGetErrorString(e != hipSuccess ? e : ec));
+            return nullptr;
+        }
+        s->graph_captures++;
+        pg->used = ++s->graph_clock;
+        if (s->graphs.size() >= GRAPH_CACHE) {
+            auto lru = std::min_element(s->graphs.begin(), s->graphs.end(),
+                                        [](const auto &a, const auto &b) { return a->used < b->used; });
+            s->graphs.erase(lru);
+        }
+        s->graphs.push_back(std::move(pg));
+        return s->graphs.back().get();
+    };
     Timer tm;
     tm.start(st);
     uint64_t i = 0;           // round index (selects the ring slots)
     while (count > 0) {
-        const uint32_t grid = round_grid(count), hub_grid = cu_count() * 8;
-        for (int kk = 0; kk < per_sync; kk++, i++) {
-            uint32_t *in = ring[i % 3], *next = ring[(i + 1) % 3], *ahead = ring[(i + 2) % 3];
-            uint32_t *hub = hubset[i % 2], *hub_ahead = hubset[(i + 1) % 2];
-            // pull rounds implement the reference's positive-side threshold only
-            const bool pull = pull_enabled && (double)count > s->pull_frac * (double)s->n;
-            if (pull) {
-                // per-vertex pushed amounts and group masks of pull rounds, made on first use
-                if (s->pm.n == 0) {
-                    s->delta.alloc((size_t)s->n * s->s_pad * s->esize);
-                    s->pm.alloc((size_t)s->n);
-                    TSVD_HIP(hipMemsetAsync(s->pm.p, 0, s->pm.bytes(), st));
-                }
-                build_worklist(s, d);
-                TSVD_HIP(hipMemsetAsync(next, 0, sizeof(uint32_t), st));
-                TSVD_HIP(hipMemsetAsync(hub_ahead, 0, 2 * sizeof(uint32_t), st));
-                const GraphDir &Q = pull_dir(s, d);
-                {
-                    PROF("ppr_pull_pop", st);
-                    hipLaunchKernelGGL(k_pull_pop<T>, dim3((unsigned)ceil_div(count, WPB)), dim3(TPB), 0, st, P, R,
-                                       (T *)s->delta.p, s->pm.p, s->F[d].p, count, s->cm[d].p, D.deg.p, s->s_pad,
-                                       s->ngroups, s->alpha, (T)s->rmax, s->directed, s->stats.p);
-                }
-                {
-                    PROF("ppr_pull", st);
-                    const size_t shm = (size_t)WPB * s->ngroups * 64 * sizeof(T);
-                    hipLaunchKernelGGL(k_pull<T>, dim3((unsigned)ceil_div(s->wl_n[d], WPB)), dim3(TPB), shm, st, R,
-                                       (const T *)s->delta.p, s->pm.p, s->wl_u[d].p, s->wl_c[d].p,
-                                       (uint32_t)s->wl_n[d], Q.rowoff.p, Q.adj.p, Q.deg.p, D.deg.p, s->s_pad,
-                                       s->ngroups, (T)s->rmax, s->nm.p, s->fnext.p, next, cap, cnt + 3);
-                }
-                hipLaunchKernelGGL(k_pull_clear, dim3((unsigned)ceil_div(count, TPB)), dim3(TPB), 0, st, s->pm.p,
-                                   s->F[d].p, count, s->ngroups);
-                s->last_pull_rounds[d]++;
+        PushGraph *pg = nullptr;
+        if (graphs) {
+            pg = block_graph(i, graph_grid(count));
+            if (pg) {
+                TSVD_HIP(hipGraphLaunch(pg->exec, st));
+                s->graph_launches++;
+                i += GRAPH_ROUNDS;
             } else {
-                {
-                    PROF("ppr_push_round", st);
-                    hipLaunchKernelGGL(k_push_round<T>, dim3(grid), dim3(TPB), 0, st, P, R, s->F[d].p, in, cap,
-                                       ahead, hub_ahead, cnt + 10, s->cm[d].p, s->nm.p, s->fnext.p, next, cap,
-                                       cnt + 3, D.rowoff.p, D.adj.p, D.deg.p, s->s_pad, s->ngroups, s->alpha,
-                                       (T)s->rmax, s->two_sided, s->directed,
-                                       (T)(std::min(s->ride_frac, 1.0) * s->rmax), s0, s1, s->stats.p,
-                                       s->hub_item.p, (T *)s->hub_delta.p, s->hub_chunks.p, hub, hub_cap, chunk_cap,
-                                       s->hub_chunk);
-                }
-                {
-                    PROF("ppr_push_hub", st);
-                    hipLaunchKernelGGL(k_push_hub<T>, dim3(hub_grid), dim3(TPB), 0, st, R, s->hub_item.p,
-                                       (const T *)s->hub_delta.p, s->hub_chunks.p, hub + 1, chunk_cap, s->nm.p,
-                                       s->fnext.p, next, cap, cnt + 3, D.rowoff.p, D.adj.p, D.deg.p, s->s_pad,
-                                       s->ngroups, (T)s->rmax, s->two_sided, s->hub_chunk);
-                }
+                // capture failed: this handle launches its rounds one by one
+                // from here on (the ring position need not be a block start)
+                graphs = false;
+                s->use_graphs = 0;
+                s->graph_failures++;
             }
-            TSVD_HIP(hipGetLastError());
-            std::swap(s->cm[d], s->nm);
-            std::swap(s->F[d], s->fnext);
+        }
+        if (!pg) {
+            const uint32_t grid = round_grid(count);
+            const int per_sync = pull_enabled ? 1 : ROUNDS_PER_SYNC;
+            for (int kk = 0; kk < per_sync; kk++, i++) {
+                // pull rounds implement the reference's positive-side threshold only
+                const bool pull = pull_enabled && (double)count > s->pull_frac * (double)s->n;
+                if (pull) {
+                    uint32_t *next = ring[(i + 1) % 3];
+                    uint32_t *hub_ahead = hubset[(i + 1) % 2];
+                    // per-vertex pushed amounts and group masks of pull rounds, made on first use
+                    if (s->pm.n == 0) {
+                        s->delta.alloc((size_t)s->n * s->s_pad * s->esize);
+                        s->pm.alloc((size_t)s->n);
+                        TSVD_HIP(hipMemsetAsync(s->pm.p, 0, s->pm.bytes(), st));
+                    }
+                    build_worklist(s, d);
+                    s->touched_all = true;   // the pull pop does not mark items
+                    TSVD_HIP(hipMemsetAsync(next, 0, sizeof(uint32_t), st));
+                    TSVD_HIP(hipMemsetAsync(hub_ahead, 0, 2 * sizeof(uint32_t), st));
+                    const GraphDir &Q = pull_dir(s, d);
+                    {
+                        PROF("ppr_pull_pop", st);
+                        hipLaunchKernelGGL(k_pull_pop<T>, dim3((unsigned)ceil_div(count, WPB)), dim3(TPB), 0, st, P,
+                                           R, (T *)s->delta.p, s->pm.p, s->F[d].p, count, s->cm[d].p, D.deg.p,
+                                           s->s_pad, s->ngroups, s->alpha, (T)s->rmax, s->directed, s->stats.p);
+                    }
+                    {
+                        PROF("ppr_pull", st);
+                        const size_t shm = (size_t)WPB * s->ngroups * 64 * sizeof(T);
+                        hipLaunchKernelGGL(k_pull<T>, dim3((unsigned)ceil_div(s->wl_n[d], WPB)), dim3(TPB), shm, st,
+                                           R, (const T *)s->delta.p, s->pm.p, s->wl_u[d].p, s->wl_c[d].p,
+                                           (uint32_t)s->wl_n[d], Q.rowoff.p, Q.adj.p, Q.deg.p, D.deg.p, s->s_pad,
+                                           s->ngroups, (T)s->rmax, s->nm.p, s->fnext.p, next, cap, cnt + 3);
+                    }
+                    hipLaunchKernelGGL(k_pull_clear, dim3((unsigned)ceil_div(count, TPB)), dim3(TPB), 0, st,
+                                       s->pm.p, s->F[d].p, count, s->ngroups);
+                    s->last_pull_rounds[d]++;
+                } else {
+                    {
+                        PROF("ppr_push_round", st);
+                        launch_round(i, grid);
+                    }
+                    {
+                        PROF("ppr_push_hub", st);
+                        launch_hub(i);
+                    }
+                }
+                TSVD_HIP(hipGetLastError());
+                std::swap(s->cm[d], s->nm);
+                std::swap(s->F[d], s->fnext);
+            }
         }
         // the count of round i (the next one to run) and the overflow flag
         TSVD_HIP(hipMemcpyAsync(h + 2, ring[i % 3], sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         TSVD_HIP(hipMemcpyAsync(h + 3, cnt + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         TSVD_HIP(hipStreamSynchronize(st));
+        if (pg && !pg->ev.empty()) {
+            // kernel times of the replay, from the graph's event nodes; where
+            // they cannot be read the handle's later rounds run one by one,
+            // timed by events of their own
+            float ms[2 * GRAPH_ROUNDS];
+            bool timed = true;
+            for (int k = 0; k < 2 * GRAPH_ROUNDS && timed; k++)
+                timed = hipEventElapsedTime(&ms[k], pg->ev[k], pg->ev[k + 1]) == hipSuccess;
+            if (timed) {
+                for (int k = 0; k < 2 * GRAPH_ROUNDS; k++)
+                    prof_add_time(k % 2 == 0 ? "ppr_push_round" : "ppr_push_hub", ms[k]);
+            } else {
+                fprintf(stderr, "treesvd: push graph event times unreadable (%s); rounds are launched one by one\n",
+                        hipGetErrorString(hipGetLastError()));
+                graphs = false;
+                s->use_graphs = 0;
+                s->graph_failures++;
+            }
+        }
         TSVD_REQUIRE(h[3] == 0, "frontier overflow in push round %llu", (unsigned long long)i);
         count = h[2];
     }
@@ -1078,6 +1239,7 @@ void ppr_update_edges(tsvd_ppr *s, int dir, const int32_t *src, const int32_t *d
     TSVD_REQUIRE(!s->two_sided, "the edge-order update follows the positive-side threshold");
     if (s0 == s1) return;
     static_assert(NO_EDGE == 0x7F7F7F7F, "the marker is set with a byte fill");
+    s->touched_all = true;
     if (s->precision == TSVD_FP64) update_edges_t<double>(s, dir, src, dst, m, s0, s1);
     else update_edges_t<float>(s, dir, src, dst, m, s0, s1);
 }
@@ -1105,6 +1267,7 @@ void ppr_save(tsvd_ppr *s, PushSave &v)
 void ppr_restore(tsvd_ppr *s, const PushSave &v)
 {
     hipStream_t st = s->g->stream;
+    s->touched_all = true;
     uint32_t cnt[2] = {v.count[0], v.count[1]};
     for (int d = 0; d < s->ndir; d++) {
         TSVD_HIP(hipMemcpyAsync(s->P[d].p, v.P[d].p, s->P[d].bytes(), hipMemcpyDeviceToDevice, st));
@@ -1132,6 +1295,7 @@ void ppr_bind_sources(tsvd_ppr *s, const int32_t *sources, int32_t count)
     TSVD_HIP(hipStreamSynchronize(s->g->stream));
     s->h_sources.assign(sources, sources + count);
     s->nsrc = count;
+    s->touched_all = true;
 }
 
 void ppr_read_lane(tsvd_ppr *s, int dir, int what, int32_t lane, double *out)
@@ -1167,6 +1331,7 @@ void ppr_clamp_negative(tsvd_ppr *s, int dir)
 void ppr_clear(tsvd_ppr *s, int dir)
 {
     hipStream_t st = s->g->stream;
+    s->touched_all = true;
     TSVD_HIP(hipMemsetAsync(s->P[dir].p, 0, s->P[dir].bytes(), st));
     TSVD_HIP(hipMemsetAsync(s->R[dir].p, 0, s->R[dir].bytes(), st));
 }
@@ -1220,6 +1385,8 @@ extern "C" int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t ns
             // rows longer than this many neighbours are split across waves
             s->hub_chunk = HUB_CHUNK;
             if (const char *e = getenv("TSVD_HUB_CHUNK")) s->hub_chunk = std::max(64, atoi(e));
+            // 0: push rounds launched one by one instead of replayed from captured graphs
+            if (const char *e = getenv("TSVD_PUSH_GRAPHS")) s->use_graphs = atoi(e) != 0;
             s->h_sources.assign(sources, sources + nsrc);
             s->sources.alloc((size_t)nsrc);
             hipStream_t st = g->stream;
@@ -1238,6 +1405,10 @@ extern "C" int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t ns
             s->nm.alloc(items);
             s->fnext.alloc(items);
             TSVD_HIP(hipMemsetAsync(s->nm.p, 0, s->nm.bytes(), st));
+            s->touched.alloc(items);
+            TSVD_HIP(hipMemsetAsync(s->touched.p, 0, s->touched.bytes(), st));
+            s->csr_ref.alloc(2);
+            TSVD_HIP(hipHostMalloc((void **)&s->h_csr_ref, 2 * sizeof(PushCsr), hipHostMallocDefault));
             s->counters.alloc(16);
             s->stats.alloc((size_t)STAT_SHARDS * STAT_FIELDS);
             TSVD_HIP(hipHostMalloc((void **)&s->h_stats, s->stats.bytes(), hipHostMallocDefault));
@@ -1352,6 +1523,25 @@ extern "C" int tsvd_ppr_set_two_sided(tsvd_ppr *s, int on)
     return guard([&] {
         TSVD_REQUIRE(s, "null ppr");
         s->two_sided = on ? 1 : 0;
+    });
+}
+
+extern "C" int tsvd_ppr_set_graphs(tsvd_ppr *s, int on)
+{
+    return guard([&] {
+        TSVD_REQUIRE(s, "null ppr");
+        s->use_graphs = on ? 1 : 0;
+        if (!on) s->graphs.clear();
+    });
+}
+
+extern "C" int tsvd_ppr_graph_stats(const tsvd_ppr *s, int64_t *launches, int64_t *captures, int64_t *failures)
+{
+    return guard([&] {
+        TSVD_REQUIRE(s, "null ppr");
+        if (launches) *launches = s->graph_launches;
+        if (captures) *captures = s->graph_captures;
+        if (failures) *failures = s->graph_failures;
     });
 }
 

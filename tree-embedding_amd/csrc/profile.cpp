@@ -88,6 +88,15 @@ void prof_add_work(const char *name, double bytes, double flops)
     s.flops += flops;
 }
 
+void prof_add_time(const char *name, double ms)
+{
+    if (!g_profile) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Sum &s = g_sum[name];
+    s.launches += 1;
+    s.ms += ms;
+}
+
 }  // namespace tsvd
 
 extern "C" int tsvd_profile_enable(int on)

@@ -60,6 +60,12 @@ struct Blocks {
                                             // sources); stored blocks keep lane order, the full-matrix
                                             // slabs of transform_chunk are in subset-row order
     std::vector<double> a2, e2, resid;
+    // Incremental norm pass: per (vertex, source group) item, the two sums of
+    // its 64 lanes (||A_new||^2 part, ||A_new - A_old||^2 part), kept current
+    // between passes; only items whose reserves changed are recomputed.
+    DBuf<double> item_sums;                 // 2 * n * (s_pad / 64), made on first use
+    DBuf<unsigned long long> item_done;     // items recomputed by the last pass (sharded)
+    int64_t last_items = 0;
 };
 
 // Sparse leaf products (spmm.hip).  One dirty block factored from CSR / CSC
@@ -123,6 +129,12 @@ void blocks_norms(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g
 // The same for blocks [j0, j1) only (the other blocks' norms are left as they are).
 void blocks_norms_range(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g, int32_t j0, int32_t j1,
                         hipStream_t st);
+// The same for blocks [j0, j1), recomputing only the items flagged in
+// `changed` (n * (s_pad / 64) bytes, cleared here), or every item when `all`
+// or when no item sums exist yet.  Items of other blocks are brought up to
+// date too, but only [j0, j1) of a2/e2 are written.
+void blocks_norms_items(Blocks &b, const void *p0, const void *p1, const tsvd_graph *g, uint8_t *changed, bool all,
+                        int32_t j0, int32_t j1, hipStream_t st);
 // Rebuild the stored matrix of the listed blocks from p0 [+ p1].
 void blocks_rebuild(Blocks &b, const std::vector<int32_t> &dirty, const void *p0, const void *p1, hipStream_t st);
 

@@ -3,6 +3,8 @@
 #include "tsvd_internal.h"
 #include "tsvd_graph.h"
 
+#include <memory>
+
 // Layout: vertex-major.  p[v * s_pad + s] is source s's reserve at vertex v
 // (s_pad = |S| rounded up to 64), so one 64-lane wavefront covers one vertex
 // for 64 consecutive sources ("source group" g = s / 64) and every access of
@@ -10,6 +12,34 @@
 // segment.  The pending work of a direction is a list of (vertex, group)
 // items plus a 64-bit lane mask per item (cm); a push round pops the list
 // and builds the next one in (nm, fnext).
+namespace tsvd {
+// The CSR arrays of one push direction, read by the push kernels from device
+// memory (the arrays move when an append outgrows them; the descriptor does not).
+struct PushCsr {
+    const int64_t *rowoff;
+    const int32_t *adj;
+    const int32_t *deg;
+};
+
+// A block of push rounds captured as a hipGraph: the kernels of a fixed
+// number of rounds with every launch argument frozen.  `key` lists those
+// arguments; a replay is valid while they are unchanged.
+struct PushGraph {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> ev;   // profiled captures: event record nodes around each kernel
+    uint64_t used = 0;
+    PushGraph() = default;
+    PushGraph(const PushGraph &) = delete;
+    PushGraph &operator=(const PushGraph &) = delete;
+    ~PushGraph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+}  // namespace tsvd
+
 struct tsvd_ppr {
     tsvd_graph *g = nullptr;
     int32_t n = 0, nsrc = 0, s_pad = 0, ngroups = 0, ndir = 1, directed = 0;
@@ -48,10 +78,26 @@ struct tsvd_ppr {
     std::vector<int32_t> h_sources;
     std::vector<int32_t> out_row;            // tsvd_ppr_read row of lane s (empty: row s)
     tsvd::DBuf<int32_t> first[2];            // edge-order update: first new edge of a row (made on first use)
+    // Items whose reserves changed since the block norms last read them: the
+    // insertion update and the push set touched[v * ngroups + g] (any
+    // direction); touched_all stands for every item (set when p is rewritten
+    // wholesale: static refresh, clear, rebind, restore, pull rounds, the
+    // edge-order update).
+    tsvd::DBuf<uint8_t> touched;             // n * ngroups
+    bool touched_all = true;
+    // Push rounds replayed from captured graphs (1, the default) or launched
+    // one by one (0); captured graphs kept per launch shape.
+    int use_graphs = 1;
+    std::vector<std::unique_ptr<tsvd::PushGraph>> graphs;
+    int64_t graph_launches = 0, graph_captures = 0, graph_failures = 0;
+    uint64_t graph_clock = 0;                // last-use order of the cached graphs
+    tsvd::DBuf<tsvd::PushCsr> csr_ref;       // per direction, rewritten at every push
+    tsvd::PushCsr *h_csr_ref = nullptr;      // pinned staging of csr_ref
     ~tsvd_ppr()
     {
         if (h_counters) (void)hipHostFree(h_counters);
         if (h_stats) (void)hipHostFree(h_stats);
+        if (h_csr_ref) (void)hipHostFree(h_csr_ref);
     }
     void *p(int d) { return P[d].p; }
     void *r(int d) { return R[d].p; }

@@ -22,6 +22,9 @@ struct ProfScope {
 // Algorithmic work (bytes, flops) credited to a kernel name; only counted
 // while profiling is enabled.
 void prof_add_work(const char *name, double bytes, double flops);
+// One launch of `name` timed elsewhere (kernels replayed from a captured
+// graph, timed by event nodes of that graph).
+void prof_add_time(const char *name, double ms);
 
 }  // namespace tsvd
 

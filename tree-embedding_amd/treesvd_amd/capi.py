@@ -26,7 +26,7 @@ EXPORTS = [
     "tsvd_read_lines", "tsvd_write_embedding_csv",
     "tsvd_ppr_create", "tsvd_ppr_destroy", "tsvd_ppr_refresh", "tsvd_ppr_push", "tsvd_ppr_read",
     "tsvd_ppr_last_push_stats", "tsvd_ppr_set_two_sided", "tsvd_ppr_refresh_edges",
-    "tsvd_ppr_get_push_setting", "tsvd_ppr_set_push_setting",
+    "tsvd_ppr_get_push_setting", "tsvd_ppr_set_push_setting", "tsvd_ppr_set_graphs", "tsvd_ppr_graph_stats",
     "frPCA", "tsvd_frpca_csr", "tsvd_frpca_csr_batch", "tsvd_frpca_csr_noalloc",
     "tsvd_pipeline_create", "tsvd_pipeline_destroy", "tsvd_pipeline_snapshot",
     "tsvd_pipeline_embedding", "tsvd_pipeline_root_sigma", "tsvd_pipeline_graph", "tsvd_pipeline_ppr",
@@ -73,7 +73,7 @@ class SnapshotStats(ctypes.Structure):
                 ("ms_refresh", ctypes.c_double), ("ms_push", ctypes.c_double), ("ms_blocks", ctypes.c_double),
                 ("ms_frpca", ctypes.c_double), ("ms_merge", ctypes.c_double),
                 ("state_entries", ctypes.c_int64), ("state_batches", ctypes.c_int32),
-                ("push_line_relax", ctypes.c_int64)]
+                ("push_line_relax", ctypes.c_int64), ("norm_items", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -143,6 +143,8 @@ def lib():
                                              ctypes.c_int32, ctypes.c_int32]
         L.tsvd_ppr_get_push_setting.argtypes = [vp, vp]
         L.tsvd_ppr_set_push_setting.argtypes = [vp, vp]
+        L.tsvd_ppr_set_graphs.argtypes = [vp, ctypes.c_int]
+        L.tsvd_ppr_graph_stats.argtypes = [vp, vp, vp, vp]
         if hasattr(L, "frPCA"):
             L.frPCA.restype = None
             L.frPCA.argtypes = [ctypes.POINTER(MatCSR), vp, vp, vp, ctypes.c_int, ctypes.c_int]
@@ -425,6 +427,17 @@ class PPR:
 
     def set_two_sided(self, on=True):
         check(lib().tsvd_ppr_set_two_sided(self.h, int(on)))
+
+    def set_graphs(self, on=True):
+        """Push rounds replayed from captured hipGraphs, or launched one by
+        one (tsvd_ppr_set_graphs)."""
+        check(lib().tsvd_ppr_set_graphs(self.h, int(on)))
+
+    def graph_stats(self):
+        """(replays, captures, failed captures) of this handle's push graphs."""
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib().tsvd_ppr_graph_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     def refresh(self, mode, d=0, s0=0, s1=None):
         check(lib().tsvd_ppr_refresh(self.h, mode, d, s0, self.nsrc if s1 is None else s1))
