@@ -333,6 +333,8 @@ def main():
                     help="also compare at saturation with the two-sided push threshold")
     ap.add_argument("--served-no-overlap", action="store_true",
                     help="run the served form's steps one at a time (no back/front overlap)")
+    ap.add_argument("--push-graphs", action="store_true",
+                    help="replay the push rounds from captured hipGraphs (off by default: measured slower)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="first run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -341,6 +343,8 @@ def main():
                           "(source sharding over several GPUs is designed, not launched; see --shard)"}))
         sys.exit(2)
 
+    if args.push_graphs:
+        os.environ["TSVD_PUSH_GRAPHS"] = "1"   # read when each PPR handle is created
     from treesvd_amd import capi, gen
     t = time.time()
     owned = None
@@ -432,14 +436,14 @@ def main():
         j = next_batch()
         if i == 0 and not args.no_tune:
             # the push autotuner: this snapshot's push timed under each setting from the same state
-            cands = [(h, r) for h in (2048, 4096, 8192) for r in (0.25, 0.5, 1.0)]
+            cands = [(h, r, k) for h in (4096, 8192) for r in (0.25, 0.5) for k in (1, 8, 16)]
             st, chosen, ms = pipe.snapshot_tuned(*batches[j], cands)
-            tuning = {"candidates": [{"hub_chunk": h, "ride_frac": r, "push_ms": round(float(t), 1)}
-                                     for (h, r), t in zip(cands, ms)],
-                      "chosen": {"hub_chunk": chosen[0], "ride_frac": chosen[1]}}
+            tuning = {"candidates": [{"hub_chunk": h, "ride_frac": r, "items_per_wave": k,
+                                      "push_ms": round(float(t), 1)} for (h, r, k), t in zip(cands, ms)],
+                      "chosen": {"hub_chunk": chosen[0], "ride_frac": chosen[1], "items_per_wave": chosen[2]}}
             cur["push_setting"] = tuple(chosen)
-            log(f"push autotuner: chose hub_chunk {chosen[0]}, ride_frac {chosen[1]} "
-                f"({', '.join(f'{h}/{r}: {t:.0f}' for (h, r), t in zip(cands, ms))} ms)")
+            log(f"push autotuner: chose hub_chunk {chosen[0]}, ride_frac {chosen[1]}, items/wave {chosen[2]} "
+                f"({', '.join(f'{h}/{r}/{k}: {t:.0f}' for (h, r, k), t in zip(cands, ms))} ms)")
         else:
             st = pipe.snapshot(*batches[j])
         warm.append(st)
@@ -524,8 +528,10 @@ def main():
         "dirty_blocks_per_step": ndirty, "merged_nodes_per_step": nmerged, "pair_list_state": state,
         "norm_items_per_step": float(np.mean([s["norm_items"] for s in stats])),
         "push_graphs": {"replays": graph_replays, "captures": graph_captures, "failed_captures": graph_failures,
-                        "note": "push rounds replayed from captured hipGraphs, six rounds per replay, counted over "
-                                "the last pipeline (rebuilt at each replay of the batches)"},
+                        "enabled": bool(args.push_graphs),
+                        "note": "with --push-graphs the push rounds are replayed from captured hipGraphs, six "
+                                "rounds per replay; counted over the last pipeline (rebuilt at each replay of the "
+                                "batches)"},
         "push_tuning": tuning,
         "gemm_tile_tuning": gemm_tuning,
         "push_atomic_roofline": {"achieved": round(achieved_rate / 1e9, 3), "ceiling": round(ceiling / 1e9, 3),
@@ -554,7 +560,8 @@ def main():
                                                                      for s, d in w.snapshots[1:]])
         # request sizes of the workload's stream: median 100k at the 1.47e9-edge shape, 10k otherwise
         med = 100_000 if args.workload == "atscale" else 10_000
-        ps = (tuning["chosen"]["hub_chunk"], tuning["chosen"]["ride_frac"]) if tuning else None
+        ps = ((tuning["chosen"]["hub_chunk"], tuning["chosen"]["ride_frac"], tuning["chosen"]["items_per_wave"])
+              if tuning else None)
 
         def compare(name, rate, count, two_sided, window_frac=None):
             """One stream replayed by the plain method and by the served form."""

@@ -148,9 +148,10 @@ int tsvd_ppr_read(tsvd_ppr *s, int dir, int what, double *out);
  * within the push error bound of the exact PPR after dynamic updates.  No
  * reference counterpart. */
 int tsvd_ppr_set_two_sided(tsvd_ppr *s, int on);
-/* Push rounds replayed from captured hipGraphs, six rounds per replay (on = 1,
- * the default; TSVD_PUSH_GRAPHS=0 at creation turns it off), or launched one
- * by one (0).  No counterpart in the reference, whose push is a host loop
+/* Push rounds replayed from captured hipGraphs, six rounds per replay
+ * (on = 1; TSVD_PUSH_GRAPHS=1 at creation sets it), or launched one by one
+ * (0, the default, measured 0.4-2.8% faster on the batched and precision
+ * workloads).  No counterpart in the reference, whose push is a host loop
  * (ppr:1426-1487); the results are the same either way within the push
  * bound.  tsvd_ppr_graph_stats: replays, captures and failed captures of the
  * handle so far (a failed capture makes the handle launch rounds one by one). */
@@ -298,6 +299,8 @@ tsvd_ppr *tsvd_pipeline_ppr(tsvd_pipeline *p);
 typedef struct {
     int32_t hub_chunk;
     double ride_frac;
+    int32_t items_per_wave;   /* items a wave takes per push round (>= 1; default 8):
+                               * the round grid shrinks by this factor             */
 } tsvd_push_setting;
 /* Read or set the push setting of a push state (e.g. tsvd_pipeline_ppr's). */
 int tsvd_ppr_get_push_setting(const tsvd_ppr *s, tsvd_push_setting *out);

@@ -1,8 +1,8 @@
 """Push rounds replayed from captured hipGraphs, through the C ABI.
 
-The push enqueues its rounds in blocks of six (round kernel + split-row
-kernel each) captured once per launch shape and replayed until the device's
-frontier count is zero.  The replay must compute what rounds launched one by
+With graphs on (tsvd_ppr_set_graphs; off by default) the push enqueues its
+rounds in blocks of six (round kernel + split-row kernel each) captured once
+per launch shape and replayed until the device's frontier count is zero.  The replay must compute what rounds launched one by
 one compute (ppr:1394-1498 restated as synchronous rounds), so both forms are
 checked against the oracle's FIFO push with the bound of test_gpu_push.py,
 |p_gpu(t) - p_oracle(t)| <= (max|r_gpu| + max|r_oracle|) * c_t, over an
@@ -95,6 +95,7 @@ def test_profiled_replays_time_every_kernel(capi):
     g = capi.Graph(n, False); g.append(u, v)
     deg, _ = g.degrees(0)
     ppr = capi.PPR(g, sources_of(deg, 64, 23), ALPHA, 1e-6)
+    ppr.set_graphs(True)
     capi.profile_reset()
     capi.profile_enable(True)
     try:

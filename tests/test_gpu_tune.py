@@ -10,7 +10,7 @@ from treesvd_amd import gen
 from helpers import column_mass
 
 pytestmark = pytest.mark.gpu
-CANDS = [(1024, 0.5), (4096, 0.25), (64, 1.0)]
+CANDS = [(1024, 0.5, 1), (4096, 0.25, 8), (64, 1.0, 4)]
 
 
 def workload():

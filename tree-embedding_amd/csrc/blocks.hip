@@ -95,6 +95,7 @@ __global__ void __launch_bounds__(TPB) k_block_norms(const T *__restrict__ p0, c
 // The transform and the stored-column lookup are those of k_block_norms, so
 // a block's sums over its items are that kernel's sums in another order.
 constexpr int DONE_SHARDS = 32;
+constexpr int IB = 4;   // flagged items a wave loads together
 
 template <class T>
 __global__ void __launch_bounds__(TPB) k_item_norms(const T *__restrict__ p0, const T *__restrict__ p1,
@@ -115,31 +116,55 @@ __global__ void __launch_bounds__(TPB) k_item_norms(const T *__restrict__ p0, co
         const bool want = mine < nitems && (all || changed[mine] != 0);
         if (mine < nitems && (all || want)) changed[mine] = 0;
         unsigned long long bal = __ballot(want);
+        // IB flagged items at a time, so that their loads are in flight together
         while (bal != 0ull) {
-            const int q = __builtin_ctzll(bal);
-            bal &= bal - 1ull;
-            const int64_t it = it0 + q;
-            const int64_t v = it / ngroups;
-            const int32_t s = (int32_t)(it - v * ngroups) * 64 + lane;
-            const int64_t j = min(v / common, (int64_t)nparts - 1);
-            const int32_t sl = slot[v];
-            double a2 = 0.0, e2 = 0.0;
-            if (s < nsrc) {
-                T x = p0[v * s_pad + s];
-                if (p1) x += p1[v * s_pad + s];
-                const T a = xform<T>(x, rm, transform);
-                const T ao = sl >= 0 ? A[(j * common + sl) * s_pad + s] : T(0);
-                a2 = (double)a * (double)a;
-                const double d = (double)a - (double)ao;
-                e2 = d * d;
+            int64_t its[IB];
+#pragma unroll
+            for (int u = 0; u < IB; u++) {
+                its[u] = -1;
+                if (bal != 0ull) {
+                    its[u] = it0 + __builtin_ctzll(bal);
+                    bal &= bal - 1ull;
+                }
             }
-            a2 = wave_sum(a2);
-            e2 = wave_sum(e2);
-            if (lane == 0) {
-                sums[2 * it] = a2;
-                sums[2 * it + 1] = e2;
+            T x[IB], ao[IB];
+#pragma unroll
+            for (int u = 0; u < IB; u++) {
+                x[u] = T(0);
+                ao[u] = T(0);
+                if (its[u] < 0) continue;
+                const int64_t v = its[u] / ngroups;
+                const int32_t s = (int32_t)(its[u] - v * ngroups) * 64 + lane;
+                const int64_t j = min(v / common, (int64_t)nparts - 1);
+                const int32_t sl = slot[v];
+                if (s < nsrc) {
+                    x[u] = p0[v * s_pad + s];
+                    if (p1) x[u] += p1[v * s_pad + s];
+                    if (sl >= 0) ao[u] = A[(j * common + sl) * s_pad + s];
+                }
             }
-            mine_done++;
+            double a2[IB], e2[IB];
+#pragma unroll
+            for (int u = 0; u < IB; u++) {
+                const T a = xform<T>(x[u], rm, transform);
+                a2[u] = (double)a * (double)a;
+                const double d = (double)a - (double)ao[u];
+                e2[u] = d * d;
+            }
+#pragma unroll
+            for (int u = 0; u < IB; u++) {
+                a2[u] = wave_sum(a2[u]);
+                e2[u] = wave_sum(e2[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < IB; u++) {
+                if (its[u] < 0) continue;
+                if (lane == 0) {
+                    sums[2 * its[u]] = a2[u];
+                    sums[2 * its[u] + 1] = e2[u];
+                }
+                mine_done++;
+            }
         }
     }
     if (lane == 0 && mine_done) atomicAdd(&sdone, mine_done);

@@ -303,7 +303,8 @@ static void tune_push(tsvd_pipeline *p, const PushTune &t)
     tsvd_ppr *q = p->ppr;
     for (int32_t i = 0; i < t.ncand; i++) {
         if (t.cand_ms) t.cand_ms[i] = -1.0;
-        TSVD_REQUIRE(t.cand[i].hub_chunk >= 64 && t.cand[i].ride_frac >= 0.0, "bad push setting %d", i);
+        TSVD_REQUIRE(t.cand[i].hub_chunk >= 64 && t.cand[i].ride_frac >= 0.0 && t.cand[i].items_per_wave >= 1 &&
+                         t.cand[i].items_per_wave <= 1024, "bad push setting %d", i);
     }
     size_t fr = 0, tot = 0;
     TSVD_HIP(hipMemGetInfo(&fr, &tot));
@@ -311,11 +312,12 @@ static void tune_push(tsvd_pipeline *p, const PushTune &t)
     PushSave sv;
     ppr_save(q, sv);
     double best = 0.0;
-    tsvd_push_setting pick{q->hub_chunk, q->ride_frac};
+    tsvd_push_setting pick{q->hub_chunk, q->ride_frac, q->items_per_wave};
     for (int32_t i = 0; i < t.ncand; i++) {
         ppr_restore(q, sv);
         q->hub_chunk = t.cand[i].hub_chunk;
         q->ride_frac = t.cand[i].ride_frac;
+        q->items_per_wave = t.cand[i].items_per_wave;
         const double t0 = now_ms();
         for (int d = 0; d < q->ndir; d++) ppr_push(q, d, 0, p->cfg.nsrc);
         const double ms = now_ms() - t0;
@@ -327,6 +329,7 @@ static void tune_push(tsvd_pipeline *p, const PushTune &t)
     }
     q->hub_chunk = pick.hub_chunk;
     q->ride_frac = pick.ride_frac;
+    q->items_per_wave = pick.items_per_wave;
     ppr_restore(q, sv);
 }
 
@@ -534,6 +537,7 @@ extern "C" int tsvd_pipeline_snapshot_tuned(tsvd_pipeline *p, const int32_t *src
             tsvd_ppr *q = p->ppr ? p->ppr : p->sp.gp;
             chosen->hub_chunk = q->hub_chunk;
             chosen->ride_frac = q->ride_frac;
+            chosen->items_per_wave = q->items_per_wave;
         }
         if (stats) *stats = plan.S;
     });

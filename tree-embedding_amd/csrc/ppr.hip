@@ -77,9 +77,82 @@ __device__ inline void mark(unsigned long long bits, uint32_t item, unsigned lon
 // second for the whole device, so one global append per item would serialise
 // a round of millions of items on the list's counter.  A workgroup instead
 // collects its new items in LDS (LDS atomics) and reserves their place in
-// the list with a single global atomic when it finishes; past STAGE items it
-// appends directly.
+// the list with a single global atomic when it finishes; past its stage
+// capacity it appends directly.
 constexpr uint32_t STAGE = 4096;
+
+// The push kernels stage per wavefront: each wave of a workgroup has its own
+// LDS stage that only its lane 0 appends to, and the wave drains and flushes
+// it by itself, with no workgroup barrier.  A wave can then take several
+// items per round without waiting on the slowest wave of its workgroup, and
+// a round's list atomics are one per wave per drain.
+constexpr int PTPB = 256;
+constexpr int PWPB = PTPB / 64;
+constexpr uint32_t PSTAGE = 1024;   // items per wave
+
+struct WaveStage {
+    uint32_t *items;    // LDS [PSTAGE] of this wave
+    uint32_t *n;        // LDS: items staged (written by lane 0 only)
+    uint32_t *list;     // the global item list, its counter and capacity
+    uint32_t *count;
+    uint32_t cap;
+    uint32_t *overflow;
+};
+
+// Lane 0 only: sets `bits` in mask[item]; the first setter stages the item.
+__device__ inline void wave_mark(const WaveStage &ws, unsigned long long bits, uint32_t item,
+                                 unsigned long long *mask)
+{
+    const unsigned long long old = atomicOr(mask + item, bits);
+    if (old != 0ull) return;
+    const uint32_t pos = *ws.n;
+    *ws.n = pos + 1;
+    if (pos < PSTAGE) ws.items[pos] = item;
+    else append_item(item, ws.list, ws.count, ws.cap, ws.overflow);
+}
+
+// Every lane of the wave, at a wave-uniform point: moves the staged items to
+// the list once more than `thresh` wait (one list atomic for all of them).
+// Lane 0 wrote the stage; a wave's LDS operations complete in order, so the
+// other lanes read what it wrote.
+__device__ inline void wave_drain(const WaveStage &ws, uint32_t thresh, int lane)
+{
+    uint32_t n = lane == 0 ? min(*ws.n, PSTAGE) : 0u;
+    n = __shfl(n, 0);
+    if (n == 0 || n <= thresh) return;
+    uint32_t b = lane == 0 ? atomicAdd(ws.count, n) : 0u;
+    b = __shfl(b, 0);
+    for (uint32_t t = lane; t < n; t += 64) {
+        if (b + t < ws.cap) ws.list[b + t] = ws.items[t];
+        else atomicOr(ws.overflow, 1u);
+    }
+    if (lane == 0) *ws.n = 0u;
+}
+
+// Every thread of the workgroup, once, after its wave's last mark: the waves'
+// remaining staged items go to the list under one list atomic for the whole
+// workgroup.  `spos` is LDS for PWPB + 1 counts.
+__device__ inline void wave_flush_all(const WaveStage &ws, uint32_t *spos, int lane)
+{
+    const int wv = threadIdx.x >> 6;
+    if (lane == 0) spos[wv] = min(*ws.n, PSTAGE);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < PWPB; w++) {
+            const uint32_t c = spos[w];
+            spos[w] = tot;
+            tot += c;
+        }
+        spos[PWPB] = tot ? atomicAdd(ws.count, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t n = min(*ws.n, PSTAGE), b = spos[PWPB] + spos[wv];
+    for (uint32_t t = lane; t < n; t += 64) {
+        if (b + t < ws.cap) ws.list[b + t] = ws.items[t];
+        else atomicOr(ws.overflow, 1u);
+    }
+}
 
 // Push counters are spread over STAT_SHARDS copies (by workgroup), for the
 // same reason: one address per counter would serialise a workgroup's end.
@@ -91,13 +164,14 @@ __device__ inline unsigned long long *stat_slot(unsigned long long *stats)
 }
 
 struct Stage {
-    uint32_t *items;    // LDS [STAGE]
+    uint32_t *items;    // LDS [lcap]
     uint32_t *n;        // LDS: items staged
     uint32_t *base;     // LDS: reserved list position
     uint32_t *list;     // the global item list, its counter and capacity
     uint32_t *count;
     uint32_t cap;
     uint32_t *overflow;
+    uint32_t lcap;      // staged items the LDS holds
 };
 
 __device__ inline void stage_mark(const Stage &sg, unsigned long long bits, uint32_t item, unsigned long long *mask)
@@ -105,7 +179,7 @@ __device__ inline void stage_mark(const Stage &sg, unsigned long long bits, uint
     const unsigned long long old = atomicOr(mask + item, bits);
     if (old != 0ull) return;
     const uint32_t pos = atomicAdd(sg.n, 1u);
-    if (pos < STAGE) sg.items[pos] = item;
+    if (pos < sg.lcap) sg.items[pos] = item;
     else append_item(item, sg.list, sg.count, sg.cap, sg.overflow);
 }
 
@@ -113,7 +187,7 @@ __device__ inline void stage_mark(const Stage &sg, unsigned long long bits, uint
 __device__ inline void stage_flush(const Stage &sg)
 {
     __syncthreads();
-    const uint32_t n = min(*sg.n, STAGE);
+    const uint32_t n = min(*sg.n, sg.lcap);
     if (threadIdx.x == 0) *sg.base = n ? atomicAdd(sg.count, n) : 0u;
     __syncthreads();
     const uint32_t b = *sg.base;
@@ -131,7 +205,7 @@ __device__ inline void stage_drain(const Stage &sg, uint32_t thresh)
     __syncthreads();
     const uint32_t waiting = *sg.n;
     if (waiting <= thresh) return;
-    const uint32_t n = min(waiting, STAGE);
+    const uint32_t n = min(waiting, sg.lcap);
     if (threadIdx.x == 0) *sg.base = atomicAdd(sg.count, n);
     __syncthreads();
     const uint32_t b = *sg.base;
@@ -144,12 +218,22 @@ __device__ inline void stage_drain(const Stage &sg, uint32_t thresh)
     __syncthreads();
 }
 
-#define TSVD_STAGE(name, list_, count_, cap_, overflow_)                                     \
-    __shared__ uint32_t name##_items[STAGE];                                                 \
+#define TSVD_STAGE_N(name, lcap_, list_, count_, cap_, overflow_)                             \
+    __shared__ uint32_t name##_items[lcap_];                                                 \
     __shared__ uint32_t name##_n, name##_base;                                               \
     if (threadIdx.x == 0) name##_n = 0u;                                                     \
     __syncthreads();                                                                         \
-    const Stage name{name##_items, &name##_n, &name##_base, (list_), (count_), (cap_), (overflow_)}
+    const Stage name{name##_items, &name##_n, &name##_base, (list_), (count_), (cap_), (overflow_), (lcap_)}
+#define TSVD_STAGE(name, list_, count_, cap_, overflow_) TSVD_STAGE_N(name, STAGE, list_, count_, cap_, overflow_)
+
+// Per-wave stages of a PTPB-thread workgroup (the barrier here is the only one).
+#define TSVD_WAVE_STAGE(name, list_, count_, cap_, overflow_)                                \
+    __shared__ uint32_t name##_items[PWPB][PSTAGE];                                          \
+    __shared__ uint32_t name##_n[PWPB];                                                      \
+    if ((threadIdx.x & 63) == 0) name##_n[threadIdx.x >> 6] = 0u;                            \
+    __syncthreads();                                                                         \
+    const WaveStage name{name##_items[threadIdx.x >> 6], &name##_n[threadIdx.x >> 6], (list_), (count_), \
+                         (cap_), (overflow_)}
 
 template <class T>
 __global__ void k_zero_range(T *__restrict__ P, T *__restrict__ R, int32_t n, int32_t s_pad, int32_t s0, int32_t s1)
@@ -381,7 +465,7 @@ template <class T>
 __device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, int32_t j1, bool pushing, T delta,
                                    T *__restrict__ Rs, int32_t s_pad, const int32_t *__restrict__ deg, uint32_t G,
                                    int32_t g, T rm, int two, int lane, unsigned long long *__restrict__ nm,
-                                   const Stage &sg)
+                                   const WaveStage &sg)
 {
     // The neighbour ids of 64 consecutive slots arrive in one coalesced load
     // (one id per lane) and are broadcast with readlane, so each relaxation
@@ -413,7 +497,7 @@ __device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, i
 #pragma unroll
             for (int q = 0; q < U; q++) {
                 const unsigned long long b = __ballot(c[q]);
-                if (b && lane == 0 && deg[u[q]] > 0) stage_mark(sg, b, (uint32_t)u[q] * G + g, nm);
+                if (b && lane == 0 && deg[u[q]] > 0) wave_mark(sg, b, (uint32_t)u[q] * G + g, nm);
             }
         }
         if (q0 < cnt) {
@@ -436,9 +520,11 @@ __device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, i
             for (int q = 0; q < U; q++) {
                 if (q >= r) break;
                 const unsigned long long b = __ballot(c[q]);
-                if (b && lane == 0 && deg[u[q]] > 0) stage_mark(sg, b, (uint32_t)u[q] * G + g, nm);
+                if (b && lane == 0 && deg[u[q]] > 0) wave_mark(sg, b, (uint32_t)u[q] * G + g, nm);
             }
         }
+        // a block of 64 neighbours stages at most 64 items: keep that much room
+        wave_drain(sg, PSTAGE - 64, lane);
     }
 }
 
@@ -462,7 +548,7 @@ constexpr int HUB_CHUNK = 4096;   // default; TSVD_HUB_CHUNK overrides it per PP
 // the other set of split-row counters, zero_hub[0..1]); *rounds counts the
 // rounds that had work.
 template <class T>
-__global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__restrict__ R,
+__global__ void __launch_bounds__(PTPB) k_push_round(T *__restrict__ P, T *__restrict__ R,
                                                     const uint32_t *__restrict__ items,
                                                     const uint32_t *__restrict__ count_in, uint32_t in_cap,
                                                     uint32_t *__restrict__ zero_next,
@@ -497,11 +583,12 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
         zero_hub[1] = 0u;
         if (nitems > 0) atomicAdd(rounds, 1u);
     }
-    TSVD_STAGE(sg, next, next_count, cap, overflow);
+    TSVD_WAVE_STAGE(sg, next, next_count, cap, overflow);
     const int lane = threadIdx.x & 63;
-    for (uint32_t wbase = blockIdx.x * WPB; wbase < nitems; wbase += gridDim.x * WPB) {
-    const uint32_t w = wbase + (threadIdx.x >> 6);
-    if (w < nitems) {
+    // wave-strided: each wave walks its own items, with no workgroup barrier
+    const uint32_t nwaves = gridDim.x * PWPB;
+    for (uint32_t w = blockIdx.x * PWPB + (threadIdx.x >> 6); w < nitems; w += nwaves) {
+    {
         const uint32_t item = items[w];
         const int32_t v = (int32_t)(item / (uint32_t)ngroups);
         const int32_t g = (int32_t)(item - (uint32_t)v * (uint32_t)ngroups);
@@ -531,7 +618,7 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
             }
         }
         const unsigned long long ca = __ballot(again);
-        if (ca && lane == 0) stage_mark(sg, ca, item, nm);
+        if (ca && lane == 0) wave_mark(sg, ca, item, nm);
         unsigned long long pm = __ballot(pushing);
         if (pm != 0ull && ride_min < rm && dv > 0) {
             // The row is walked anyway: lanes whose residue is positive but at or
@@ -551,7 +638,7 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
                 }
             }
             const unsigned long long cb = __ballot(back);
-            if (cb && lane == 0) stage_mark(sg, cb, item, nm);
+            if (cb && lane == 0) wave_mark(sg, cb, item, nm);
             pm = __ballot(pushing);
         }
         if (pm != 0ull) {
@@ -598,16 +685,17 @@ __global__ void __launch_bounds__(TPB) k_push_round(T *__restrict__ P, T *__rest
             relax_range(nb, 0, jend, pushing, delta, R + s, s_pad, deg, (uint32_t)ngroups, g, rm, two, lane, nm, sg);
         }
     }
-    stage_drain(sg, STAGE / 2);
+    wave_drain(sg, PSTAGE / 2, lane);
     }
-    stage_flush(sg);
+    __shared__ uint32_t spos[PWPB + 1];
+    wave_flush_all(sg, spos, lane);
     if (threadIdx.x < 6 && sstat[threadIdx.x]) atomicAdd(stat_slot(stats) + threadIdx.x, sstat[threadIdx.x]);
 }
 
 
 // Further chunks of split rows (see HUB_CHUNK): one wave per chunk.
 template <class T>
-__global__ void __launch_bounds__(TPB) k_push_hub(T *__restrict__ R, const uint32_t *__restrict__ hub_item,
+__global__ void __launch_bounds__(PTPB) k_push_hub(T *__restrict__ R, const uint32_t *__restrict__ hub_item,
                                                   const T *__restrict__ hub_delta,
                                                   const unsigned long long *__restrict__ hub_chunks,
                                                   const uint32_t *__restrict__ chunk_count, uint32_t chunk_cap,
@@ -620,12 +708,12 @@ __global__ void __launch_bounds__(TPB) k_push_hub(T *__restrict__ R, const uint3
     const int64_t *__restrict__ rowoff = csr->rowoff;
     const int32_t *__restrict__ adj = csr->adj;
     const int32_t *__restrict__ deg = csr->deg;
-    TSVD_STAGE(sg, next, next_count, cap, overflow);
+    TSVD_WAVE_STAGE(sg, next, next_count, cap, overflow);
     const uint32_t nchunks = min(*chunk_count, chunk_cap);
     const int lane = threadIdx.x & 63;
-    for (uint32_t wbase = blockIdx.x * WPB; wbase < nchunks; wbase += gridDim.x * WPB) {
-    const uint32_t w = wbase + (threadIdx.x >> 6);
-    if (w < nchunks) {
+    const uint32_t nwaves = gridDim.x * PWPB;
+    for (uint32_t w = blockIdx.x * PWPB + (threadIdx.x >> 6); w < nchunks; w += nwaves) {
+    {
         const unsigned long long e = hub_chunks[w];
         const uint32_t slot = (uint32_t)(e >> 32), c = (uint32_t)(e & 0xFFFFFFFFull);
         const uint32_t item = hub_item[slot];
@@ -637,9 +725,10 @@ __global__ void __launch_bounds__(TPB) k_push_hub(T *__restrict__ R, const uint3
         relax_range(adj + rowoff[v], j0, j1, delta != T(0), delta, R + (int64_t)g * 64 + lane, s_pad, deg,
                     (uint32_t)ngroups, g, rm, two, lane, nm, sg);
     }
-    stage_drain(sg, STAGE / 2);
+    wave_drain(sg, PSTAGE / 2, lane);
     }
-    stage_flush(sg);
+    __shared__ uint32_t spos[PWPB + 1];
+    wave_flush_all(sg, spos, lane);
 }
 
 // ---------------------------------------------------------------- pull rounds
@@ -869,9 +958,9 @@ static void build_worklist(tsvd_ppr *s, int d)
 
 // Workgroups of a device-driven round.  The host knows the item count of the
 // first round of each batch it enqueues, not of the later ones: the grid
-// gives that count 1.5x headroom at one item per wave (the shape of a round
-// launched for its exact count) and never less than 2 workgroups per compute
-// unit; a round that outgrows it strides over its items.
+// gives that count 1.5x headroom at `items_per_wave` items per wave and never
+// less than 8 waves per compute unit; a round that outgrows it strides over
+// its items.
 static uint32_t cu_count()
 {
     static int cus[64] = {};
@@ -886,14 +975,14 @@ static uint32_t cu_count()
     return (uint32_t)cus[dev];
 }
 
-// A launch's work-item count (workgroups x 256) must stay below 2^32; larger
+// A launch's work-item count (workgroups x PTPB) must stay below 2^32; larger
 // rounds stride over their items.
-constexpr uint64_t MAX_ROUND_GRID = 1u << 23;
+constexpr uint64_t MAX_ROUND_GRID = (1ull << 31) / PTPB;
 
-static uint32_t round_grid(uint32_t count_hint)
+static uint32_t round_grid(uint32_t count_hint, int32_t per_wave)
 {
-    const uint64_t want = (uint64_t)ceil_div((int64_t)count_hint + count_hint / 2, WPB);
-    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, (uint64_t)cu_count() * 2), MAX_ROUND_GRID);
+    const uint64_t want = (uint64_t)ceil_div((int64_t)count_hint + count_hint / 2, (int64_t)PWPB * per_wave);
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, (uint64_t)cu_count() * 8 / PWPB), MAX_ROUND_GRID);
 }
 
 // Rounds until no item is pending.  Push rounds are device-driven: each reads
@@ -914,9 +1003,9 @@ constexpr int ROUNDS_PER_SYNC = 8;
 constexpr int GRAPH_ROUNDS = 6;
 constexpr size_t GRAPH_CACHE = 16;
 
-static uint32_t graph_grid(uint32_t count_hint)
+static uint32_t graph_grid(uint32_t count_hint, int32_t per_wave)
 {
-    const uint32_t g = round_grid(count_hint);
+    const uint32_t g = round_grid(count_hint, per_wave);
     uint32_t p = 1;
     while (p < g) p <<= 1;
     return std::min<uint32_t>(p, MAX_ROUND_GRID);
@@ -964,11 +1053,11 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
     s->hub_delta.ensure((size_t)std::max<uint32_t>(hub_cap, 1) * 64 * sizeof(T));
     s->hub_chunks.ensure(std::max<uint32_t>(chunk_cap, 1));
     const T ride_min = (T)(std::min(s->ride_frac, 1.0) * s->rmax);
-    const uint32_t hub_grid = cu_count() * 8;
+    const uint32_t hub_grid = cu_count() * 32 / PWPB;   // 32 waves per compute unit
     auto launch_round = [&](uint64_t ri, uint32_t grid) {
         uint32_t *in = ring[ri % 3], *next = ring[(ri + 1) % 3], *ahead = ring[(ri + 2) % 3];
         uint32_t *hub = hubset[ri % 2], *hub_ahead = hubset[(ri + 1) % 2];
-        hipLaunchKernelGGL(k_push_round<T>, dim3(grid), dim3(TPB), 0, st, P, R, s->F[d].p, in, cap, ahead,
+        hipLaunchKernelGGL(k_push_round<T>, dim3(grid), dim3(PTPB), 0, st, P, R, s->F[d].p, in, cap, ahead,
                            hub_ahead, cnt + 10, s->cm[d].p, s->nm.p, s->fnext.p, next, cap, cnt + 3, csr,
                            s->s_pad, s->ngroups, s->alpha, (T)s->rmax, s->two_sided, s->directed,
                            ride_min, s0, s1, s->stats.p, s->hub_item.p, (T *)s->hub_delta.p, s->hub_chunks.p, hub,
@@ -976,7 +1065,7 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
     };
     auto launch_hub = [&](uint64_t ri) {
         uint32_t *next = ring[(ri + 1) % 3], *hub = hubset[ri % 2];
-        hipLaunchKernelGGL(k_push_hub<T>, dim3(hub_grid), dim3(TPB), 0, st, R, s->hub_item.p,
+        hipLaunchKernelGGL(k_push_hub<T>, dim3(hub_grid), dim3(PTPB), 0, st, R, s->hub_item.p,
                            (const T *)s->hub_delta.p, s->hub_chunks.p, hub + 1, chunk_cap, s->nm.p, s->fnext.p,
                            next, cap, cnt + 3, csr, s->s_pad, s->ngroups, (T)s->rmax, s->two_sided,
                            s->hub_chunk);
@@ -1056,7 +1145,7 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
     while (count > 0) {
         PushGraph *pg = nullptr;
         if (graphs) {
-            pg = block_graph(i, graph_grid(count));
+            pg = block_graph(i, graph_grid(count, s->items_per_wave));
             if (pg) {
                 TSVD_HIP(hipGraphLaunch(pg->exec, st));
                 s->graph_launches++;
@@ -1070,7 +1159,7 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
             }
         }
         if (!pg) {
-            const uint32_t grid = round_grid(count);
+            const uint32_t grid = round_grid(count, s->items_per_wave);
             const int per_sync = pull_enabled ? 1 : ROUNDS_PER_SYNC;
             for (int kk = 0; kk < per_sync; kk++, i++) {
                 // pull rounds implement the reference's positive-side threshold only
@@ -1385,8 +1474,10 @@ extern "C" int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t ns
             // rows longer than this many neighbours are split across waves
             s->hub_chunk = HUB_CHUNK;
             if (const char *e = getenv("TSVD_HUB_CHUNK")) s->hub_chunk = std::max(64, atoi(e));
-            // 0: push rounds launched one by one instead of replayed from captured graphs
+            // 1: push rounds replayed from captured graphs instead of launched one by one
             if (const char *e = getenv("TSVD_PUSH_GRAPHS")) s->use_graphs = atoi(e) != 0;
+            // items a wave takes per round (the round grid shrinks by this factor)
+            if (const char *e = getenv("TSVD_ITEMS_PER_WAVE")) s->items_per_wave = std::max(1, atoi(e));
             s->h_sources.assign(sources, sources + nsrc);
             s->sources.alloc((size_t)nsrc);
             hipStream_t st = g->stream;
@@ -1464,6 +1555,7 @@ extern "C" int tsvd_ppr_get_push_setting(const tsvd_ppr *s, tsvd_push_setting *o
         TSVD_REQUIRE(s && out, "null argument");
         out->hub_chunk = s->hub_chunk;
         out->ride_frac = s->ride_frac;
+        out->items_per_wave = s->items_per_wave;
     });
 }
 
@@ -1471,9 +1563,11 @@ extern "C" int tsvd_ppr_set_push_setting(tsvd_ppr *s, const tsvd_push_setting *i
 {
     return guard([&] {
         TSVD_REQUIRE(s && in, "null argument");
-        TSVD_REQUIRE(in->hub_chunk >= 64 && in->ride_frac >= 0.0, "bad push setting");
+        TSVD_REQUIRE(in->hub_chunk >= 64 && in->ride_frac >= 0.0 && in->items_per_wave >= 1 &&
+                         in->items_per_wave <= 1024, "bad push setting");
         s->hub_chunk = in->hub_chunk;
         s->ride_frac = in->ride_frac;
+        s->items_per_wave = in->items_per_wave;
     });
 }
 

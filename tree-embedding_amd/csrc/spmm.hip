@@ -31,6 +31,8 @@ namespace {
 constexpr int TPB = 256;
 constexpr int WPB = TPB / 64;
 constexpr int RU = 16;   // column loads in flight per thread in the row walks
+constexpr int GU = 8;    // entries whose operand rows are in flight per wave in the gathers
+constexpr int CU = 4;    // 64-row segments in flight per wave in the column walks
 
 template <class T>
 __global__ void __launch_bounds__(TPB) k_col_counts(const SpItem *__restrict__ items, int64_t *__restrict__ cnt,
@@ -42,10 +44,15 @@ __global__ void __launch_bounds__(TPB) k_col_counts(const SpItem *__restrict__ i
     const int lane = threadIdx.x & 63;
     const T *col = (const T *)it.A + c * it.lda;
     uint32_t n = 0;
-    for (int32_t r0 = 0; r0 < it.m; r0 += 64) {
-        const int32_t r = r0 + lane;
-        const bool nz = r < it.m && col[r] != T(0);
-        n += (uint32_t)__popcll(__ballot(nz));
+    for (int32_t r0 = 0; r0 < it.m; r0 += 64 * CU) {
+        T x[CU];
+#pragma unroll
+        for (int q = 0; q < CU; q++) {
+            const int32_t r = r0 + q * 64 + lane;
+            x[q] = r < it.m ? col[r] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < CU; q++) n += (uint32_t)__popcll(__ballot(x[q] != T(0)));
     }
     if (lane == 0) {
         if (with_cnt) cnt[it.base + it.m + c] = (int64_t)n;
@@ -109,17 +116,26 @@ __global__ void __launch_bounds__(TPB) k_col_fill(const SpItem *__restrict__ ite
     const int lane = threadIdx.x & 63;
     const T *col = (const T *)it.A + c * it.lda;
     int64_t p = off[it.base + it.m + c];
-    for (int32_t r0 = 0; r0 < it.m; r0 += 64) {
-        const int32_t r = r0 + lane;
-        const T x = r < it.m ? col[r] : T(0);
-        const bool nz = x != T(0);
-        const unsigned long long b = __ballot(nz);
-        const int before = __popcll(b & ((1ull << lane) - 1ull));
-        if (nz) {
-            idx[p + before] = r;
-            val[p + before] = x;
+    for (int32_t r0 = 0; r0 < it.m; r0 += 64 * CU) {
+        T xs[CU];
+#pragma unroll
+        for (int q = 0; q < CU; q++) {
+            const int32_t r = r0 + q * 64 + lane;
+            xs[q] = r < it.m ? col[r] : T(0);
         }
-        p += __popcll(b);
+#pragma unroll
+        for (int q = 0; q < CU; q++) {
+            const int32_t r = r0 + q * 64 + lane;
+            const T x = xs[q];
+            const bool nz = x != T(0);
+            const unsigned long long b = __ballot(nz);
+            const int before = __popcll(b & ((1ull << lane) - 1ull));
+            if (nz) {
+                idx[p + before] = r;
+                val[p + before] = x;
+            }
+            p += __popcll(b);
+        }
     }
 }
 
@@ -151,7 +167,32 @@ __global__ void __launch_bounds__(TPB) k_spmm_gather(const SpItem *__restrict__ 
             v = val[t];
         }
         const int cnt = (int)min((int64_t)64, e - t0);
-        for (int u = 0; u < cnt; u++) {
+        int u = 0;
+        // GU entries at a time: their operand rows are loaded before any is
+        // used (the walk is latency-bound otherwise); the sums keep entry order
+        for (; u + GU <= cnt; u += GU) {
+            int32_t jj[GU];
+            T a[GU], x0[GU], x1[GU], x2[GU];
+#pragma unroll
+            for (int q = 0; q < GU; q++) {
+                jj[q] = __shfl(j, u + q);
+                a[q] = __shfl(v, u + q);
+            }
+#pragma unroll
+            for (int q = 0; q < GU; q++) {
+                const T *__restrict__ x = in + (int64_t)jj[q] * l;
+                x0[q] = lane < l ? x[lane] : T(0);
+                x1[q] = lane + 64 < l ? x[lane + 64] : T(0);
+                x2[q] = lane + 128 < l ? x[lane + 128] : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < GU; q++) {
+                if (lane < l) a0 += a[q] * x0[q];
+                if (lane + 64 < l) a1 += a[q] * x1[q];
+                if (lane + 128 < l) a2 += a[q] * x2[q];
+            }
+        }
+        for (; u < cnt; u++) {
             const int32_t jj = __shfl(j, u);
             const T a = __shfl(v, u);
             const T *__restrict__ x = in + (int64_t)jj * l;

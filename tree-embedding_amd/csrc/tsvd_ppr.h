@@ -75,6 +75,7 @@ struct tsvd_ppr {
     int32_t hub_chunk = 4096;                // rows longer than this are split across waves
     double pull_frac = 0.0;                  // pull when items > pull_frac * n (0: never)
     double ride_frac = 0.5;                  // sub-threshold lanes above ride_frac*rmax ride along (>= 1: off)
+    int32_t items_per_wave = 8;              // items per wave of a round's grid (grid-stride beyond)
     std::vector<int32_t> h_sources;
     std::vector<int32_t> out_row;            // tsvd_ppr_read row of lane s (empty: row s)
     tsvd::DBuf<int32_t> first[2];            // edge-order update: first new edge of a row (made on first use)
@@ -85,9 +86,10 @@ struct tsvd_ppr {
     // edge-order update).
     tsvd::DBuf<uint8_t> touched;             // n * ngroups
     bool touched_all = true;
-    // Push rounds replayed from captured graphs (1, the default) or launched
-    // one by one (0); captured graphs kept per launch shape.
-    int use_graphs = 1;
+    // Push rounds replayed from captured graphs (1) or launched one by one
+    // (0, the default: measured faster, since the rounds are already enqueued
+    // several at a time); captured graphs kept per launch shape.
+    int use_graphs = 0;
     std::vector<std::unique_ptr<tsvd::PushGraph>> graphs;
     int64_t graph_launches = 0, graph_captures = 0, graph_failures = 0;
     uint64_t graph_clock = 0;                // last-use order of the cached graphs

@@ -80,7 +80,7 @@ class SnapshotStats(ctypes.Structure):
 
 
 class PushSetting(ctypes.Structure):
-    _fields_ = [("hub_chunk", ctypes.c_int32), ("ride_frac", ctypes.c_double)]
+    _fields_ = [("hub_chunk", ctypes.c_int32), ("ride_frac", ctypes.c_double), ("items_per_wave", ctypes.c_int32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -407,14 +407,17 @@ class PPR:
 
     @property
     def push_setting(self):
-        """(hub_chunk, ride_frac) of the push (tsvd_ppr_get_push_setting)."""
+        """(hub_chunk, ride_frac, items_per_wave) of the push
+        (tsvd_ppr_get_push_setting)."""
         x = PushSetting()
         check(lib().tsvd_ppr_get_push_setting(self.h, ctypes.byref(x)))
-        return x.hub_chunk, x.ride_frac
+        return x.hub_chunk, x.ride_frac, x.items_per_wave
 
     @push_setting.setter
     def push_setting(self, hr):
-        x = PushSetting(int(hr[0]), float(hr[1]))
+        """(hub_chunk, ride_frac[, items_per_wave]); items_per_wave kept when omitted."""
+        ipw = int(hr[2]) if len(hr) > 2 else self.push_setting[2]
+        x = PushSetting(int(hr[0]), float(hr[1]), ipw)
         check(lib().tsvd_ppr_set_push_setting(self.h, ctypes.byref(x)))
 
     def refresh_edges(self, src, dst, d=0, s0=0, s1=None):
@@ -609,20 +612,23 @@ class Pipeline:
 
     def snapshot_tuned(self, src, dst, candidates):
         """One snapshot whose push is first timed under each (hub_chunk,
-        ride_frac) candidate (tsvd_pipeline_snapshot_tuned).  Returns (stats,
-        chosen setting, per-candidate push ms; -1 where not timed)."""
+        ride_frac[, items_per_wave]) candidate (tsvd_pipeline_snapshot_tuned;
+        items_per_wave 8 when omitted).  Returns (stats, chosen setting as
+        (hub_chunk, ride_frac, items_per_wave), per-candidate push ms; -1
+        where not timed)."""
         if isinstance(src, np.ndarray):
             src = np.ascontiguousarray(src, dtype=np.int32)
             dst = np.ascontiguousarray(dst, dtype=np.int32)
         ps, dev = _dptr(src)
         pd, _ = _dptr(dst)
-        cand = (PushSetting * max(len(candidates), 1))(*[PushSetting(int(h), float(r)) for h, r in candidates])
+        cand = (PushSetting * max(len(candidates), 1))(*[PushSetting(int(c[0]), float(c[1]), int(c[2]) if len(c) > 2 else 8)
+                                                          for c in candidates])
         ms = np.zeros(max(len(candidates), 1))
         chosen = PushSetting()
         st = SnapshotStats()
         check(lib().tsvd_pipeline_snapshot_tuned(self.h, ps, pd, int(src.shape[0]), dev, cand, len(candidates),
                                                  ctypes.byref(chosen), _p(ms), ctypes.byref(st)))
-        return st.as_dict(), (chosen.hub_chunk, chosen.ride_frac), ms[:len(candidates)]
+        return st.as_dict(), (chosen.hub_chunk, chosen.ride_frac, chosen.items_per_wave), ms[:len(candidates)]
 
     def embedding(self):
         E = np.zeros((self.nsrc, self.rank))
