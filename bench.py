@@ -325,7 +325,9 @@ def main():
                     help="append errors against a dense FP64 SVD and the reference algorithm's own error")
     ap.add_argument("--served-requests", type=int, default=48,
                     help="requests in the served-vs-plain stream (0: skip that comparison)")
-    ap.add_argument("--served-rate", type=float, default=12.0, help="Poisson arrival rate of that stream (1/s)")
+    ap.add_argument("--served-rate", type=float, default=24.0,
+                    help="Poisson arrival rate of the saturation streams (1/s): above what either method sustains, "
+                         "so that each runs flat out")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the push autotuner on the first warm-up snapshot and the MFMA tile-width "
                          "autotuner over the initial and warm-up snapshots")
