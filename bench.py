@@ -438,7 +438,7 @@ def main():
         j = next_batch()
         if i == 0 and not args.no_tune:
             # the push autotuner: this snapshot's push timed under each setting from the same state
-            cands = [(h, r, k) for h in (4096, 8192) for r in (0.25, 0.5) for k in (1, 8, 16)]
+            cands = [(h, 0.5, k) for h in (1024, 4096, 8192) for k in (1, 8, 16)]
             st, chosen, ms = pipe.snapshot_tuned(*batches[j], cands)
             tuning = {"candidates": [{"hub_chunk": h, "ride_frac": r, "items_per_wave": k,
                                       "push_ms": round(float(t), 1)} for (h, r, k), t in zip(cands, ms)],

@@ -263,7 +263,7 @@ __global__ void k_seed(T *__restrict__ R, const int32_t *__restrict__ src, int32
 
 // Dynamic branch: one wave per (grown row i, source group g), lanes = sources.
 template <class T>
-__global__ void __launch_bounds__(TPB) k_update(T *__restrict__ P, T *__restrict__ R, const int32_t *__restrict__ touched,
+__global__ void __launch_bounds__(TPB, 8) k_update(T *__restrict__ P, T *__restrict__ R, const int32_t *__restrict__ touched,
                                                 int32_t ntouched, const int64_t *__restrict__ rowoff,
                                                 const int32_t *__restrict__ adj, const int32_t *__restrict__ deg,
                                                 const int32_t *__restrict__ former, int32_t s_pad, int32_t ngroups,
@@ -548,7 +548,7 @@ constexpr int HUB_CHUNK = 4096;   // default; TSVD_HUB_CHUNK overrides it per PP
 // the other set of split-row counters, zero_hub[0..1]); *rounds counts the
 // rounds that had work.
 template <class T>
-__global__ void __launch_bounds__(PTPB) k_push_round(T *__restrict__ P, T *__restrict__ R,
+__global__ void __launch_bounds__(PTPB, 8) k_push_round(T *__restrict__ P, T *__restrict__ R,
                                                     const uint32_t *__restrict__ items,
                                                     const uint32_t *__restrict__ count_in, uint32_t in_cap,
                                                     uint32_t *__restrict__ zero_next,
@@ -958,9 +958,9 @@ static void build_worklist(tsvd_ppr *s, int d)
 
 // Workgroups of a device-driven round.  The host knows the item count of the
 // first round of each batch it enqueues, not of the later ones: the grid
-// gives that count 1.5x headroom at `items_per_wave` items per wave and never
-// less than 8 waves per compute unit; a round that outgrows it strides over
-// its items.
+// gives that count 1.5x headroom at up to `items_per_wave` items per wave and
+// never less than 8 waves per compute unit; a round that outgrows it strides
+// over its items.
 static uint32_t cu_count()
 {
     static int cus[64] = {};
@@ -981,7 +981,12 @@ constexpr uint64_t MAX_ROUND_GRID = (1ull << 31) / PTPB;
 
 static uint32_t round_grid(uint32_t count_hint, int32_t per_wave)
 {
-    const uint64_t want = (uint64_t)ceil_div((int64_t)count_hint + count_hint / 2, (int64_t)PWPB * per_wave);
+    // a small round keeps one item per wave up to two full occupancies of
+    // the device (64 waves per compute unit): fewer waves would leave it idle
+    const int64_t headroom = (int64_t)count_hint + count_hint / 2;
+    const int64_t fill = (int64_t)cu_count() * 64;
+    per_wave = (int32_t)std::max<int64_t>(1, std::min<int64_t>(per_wave, headroom / fill));
+    const uint64_t want = (uint64_t)ceil_div(headroom, (int64_t)PWPB * per_wave);
     return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, (uint64_t)cu_count() * 8 / PWPB), MAX_ROUND_GRID);
 }
 

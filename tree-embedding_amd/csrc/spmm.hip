@@ -216,14 +216,12 @@ void build_t(SparseLeaves &s, hipStream_t st)
         maxm = std::max<int64_t>(maxm, it.m);
         maxnc = std::max<int64_t>(maxnc, it.nc);
     }
-    const int64_t len = s.counts;   // rows then columns of every item
-    s.cnt.ensure((size_t)len + 1);
+    // rows then columns of every candidate of the density pass, whose column
+    // counts are in place (the runs of candidates that stay dense are zero)
+    const int64_t len = s.counts;
     s.off.ensure((size_t)len + 1);
-    TSVD_HIP(hipMemsetAsync(s.cnt.p, 0, sizeof(int64_t) * (len + 1), st));
     hipLaunchKernelGGL(k_row_counts<T>, dim3((unsigned)ceil_div(maxm, TPB), (unsigned)cnt), dim3(TPB), 0, st,
                        s.ditems.p, s.cnt.p);
-    hipLaunchKernelGGL(k_col_counts<T>, dim3((unsigned)ceil_div(maxnc, WPB), (unsigned)cnt), dim3(TPB), 0, st,
-                       s.ditems.p, s.cnt.p, s.nnz.p, 1);
     TSVD_HIP(hipGetLastError());
     size_t need = 0;
     TSVD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, s.cnt.p, s.off.p, len + 1, st));
@@ -245,11 +243,24 @@ void build_t(SparseLeaves &s, hipStream_t st)
 
 }  // namespace
 
-std::vector<double> leaf_densities(SparseLeaves &s, const std::vector<SpItem> &cand, int precision, hipStream_t st)
+std::vector<double> leaf_densities(SparseLeaves &s, std::vector<SpItem> &cand, int precision, hipStream_t st)
 {
     std::vector<double> dens(cand.size(), 1.0);
+    s.cand.clear();
+    s.counts = 0;
     if (cand.empty()) return dens;
     const int cnt = (int)cand.size();
+    // every candidate's run of row counts and column counts in one array: the
+    // column counts made here are the ones the CSC build of a sparse block uses
+    int64_t base = 0;
+    for (auto &it : cand) {
+        it.base = base;
+        base += (int64_t)it.m + it.nc;
+    }
+    s.counts = base;
+    s.cand = cand;
+    s.cnt.ensure((size_t)base + 1);
+    TSVD_HIP(hipMemsetAsync(s.cnt.p, 0, sizeof(int64_t) * (base + 1), st));
     s.ditems.ensure(cand.size());
     s.nnz.ensure(cand.size());
     TSVD_HIP(hipMemcpyAsync(s.ditems.p, cand.data(), sizeof(SpItem) * cand.size(), hipMemcpyHostToDevice, st));
@@ -260,10 +271,10 @@ std::vector<double> leaf_densities(SparseLeaves &s, const std::vector<SpItem> &c
         PROF("spmm_count", st);
         if (precision == TSVD_FP64)
             hipLaunchKernelGGL(k_col_counts<double>, dim3((unsigned)ceil_div(maxnc, WPB), (unsigned)cnt), dim3(TPB),
-                               0, st, s.ditems.p, nullptr, s.nnz.p, 0);
+                               0, st, s.ditems.p, s.cnt.p, s.nnz.p, 1);
         else
             hipLaunchKernelGGL(k_col_counts<float>, dim3((unsigned)ceil_div(maxnc, WPB), (unsigned)cnt), dim3(TPB),
-                               0, st, s.ditems.p, nullptr, s.nnz.p, 0);
+                               0, st, s.ditems.p, s.cnt.p, s.nnz.p, 1);
         TSVD_HIP(hipGetLastError());
     }
     std::vector<unsigned long long> h(cand.size());
@@ -281,16 +292,24 @@ void leaf_sparse_build(SparseLeaves &s, std::vector<SpItem> items, int precision
     s.items = std::move(items);
     s.entries = 0;
     if (s.items.empty()) return;
-    int64_t base = 0;
+    // the items are candidates of the last density pass (same runs); the
+    // candidates that stay dense get their column counts cleared so that the
+    // scan gives them empty runs
+    std::vector<char> chosen(s.cand.size(), 0);
     for (auto &it : s.items) {
-        it.base = base;
-        base += (int64_t)it.m + it.nc;
+        bool found = false;
+        for (size_t c = 0; c < s.cand.size() && !found; c++)
+            if (s.cand[c].base == it.base && s.cand[c].A == it.A && s.cand[c].nc == it.nc) {
+                chosen[c] = 1;
+                found = true;
+            }
+        TSVD_REQUIRE(found, "sparse leaf item without a density pass");
     }
-    s.counts = base;
+    for (size_t c = 0; c < s.cand.size(); c++)
+        if (!chosen[c] && s.cand[c].nc > 0)
+            TSVD_HIP(hipMemsetAsync(s.cnt.p + s.cand[c].base + s.cand[c].m, 0, sizeof(int64_t) * s.cand[c].nc, st));
     s.ditems.ensure(s.items.size());
-    s.nnz.ensure(s.items.size());
     TSVD_HIP(hipMemcpyAsync(s.ditems.p, s.items.data(), sizeof(SpItem) * s.items.size(), hipMemcpyHostToDevice, st));
-    TSVD_HIP(hipMemsetAsync(s.nnz.p, 0, sizeof(unsigned long long) * s.items.size(), st));
     PROF("spmm_build", st);
     if (precision == TSVD_FP64) build_t<double>(s, st);
     else build_t<float>(s, st);

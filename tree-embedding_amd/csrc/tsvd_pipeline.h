@@ -79,6 +79,7 @@ struct SpItem {
 };
 struct SparseLeaves {
     std::vector<SpItem> items;
+    std::vector<SpItem> cand;       // candidates of the last density pass, with their count runs
     DBuf<SpItem> ditems;
     DBuf<unsigned long long> nnz;   // per item
     DBuf<int64_t> cnt, off;         // counts and their exclusive scan (one run per row, then per column)
@@ -87,7 +88,9 @@ struct SparseLeaves {
     int64_t counts = 0, entries = 0;
 };
 // Non-zero share of each candidate block (items need A, lda, m, nc).
-std::vector<double> leaf_densities(SparseLeaves &s, const std::vector<SpItem> &cand, int precision, hipStream_t st);
+// Sets each candidate's `base` (its run of row then column counts) and keeps
+// the column counts for leaf_sparse_build.
+std::vector<double> leaf_densities(SparseLeaves &s, std::vector<SpItem> &cand, int precision, hipStream_t st);
 // CSR and CSC of the listed blocks (their Q / Qt pointers are used by leaf_spmm).
 void leaf_sparse_build(SparseLeaves &s, std::vector<SpItem> items, int precision, hipStream_t st);
 // Q = A Qt (transposed = false) or Qt = A^T Q for every built item.
