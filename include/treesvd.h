@@ -288,8 +288,9 @@ tsvd_graph *tsvd_pipeline_graph(tsvd_pipeline *p);
 tsvd_ppr *tsvd_pipeline_ppr(tsvd_pipeline *p);
 /* The project's autotuner for the push's launch shape (no reference
  * counterpart).  Setting: rows longer than hub_chunk neighbours are split
- * across wavefronts; lanes whose residue is above ride_frac * rmax (but not
- * above rmax) push along with a popped item (>= 1: off).  A tuned snapshot
+ * across wavefronts (default 256); lanes whose residue is above ride_frac *
+ * rmax (but not above rmax) push along with a popped item (default 0.5; >= 1:
+ * off).  A tuned snapshot
  * is tsvd_pipeline_snapshot, except that its push is first timed under each
  * of the ncand candidates from the same saved state (p, r and the pending
  * lists), then run for real with the fastest, which stays for later
@@ -299,7 +300,8 @@ tsvd_ppr *tsvd_pipeline_ppr(tsvd_pipeline *p);
 typedef struct {
     int32_t hub_chunk;
     double ride_frac;
-    int32_t items_per_wave;   /* items a wave takes per push round (>= 1; default 8):
+    int32_t items_per_wave;   /* items a wave takes per push round (>= 1; default 8;
+                               * fewer in rounds too small to fill the device):
                                * the round grid shrinks by this factor             */
 } tsvd_push_setting;
 /* Read or set the push setting of a push state (e.g. tsvd_pipeline_ppr's). */

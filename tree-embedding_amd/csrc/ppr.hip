@@ -532,7 +532,12 @@ __device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, i
 // the first chunk and hands the rest, with its 64 pushed amounts, to
 // k_push_hub (one wave per further chunk), so a hub does not serialise a
 // round behind one wavefront.
-constexpr int HUB_CHUNK = 4096;   // default; TSVD_HUB_CHUNK overrides it per PPR handle
+// Default 256: on graphs whose rounds are short (few items per wave) a round
+// lasts as long as its longest chunk walk (measured on the 2.7M-node
+// directed Chung-Lu graph: 256 -> 183 ms of push over 6 snapshots, 1024 ->
+// 262 ms, 4096 -> 421 ms); graphs with long rounds prefer 4096-8192, which
+// the autotuner finds (2.1M-node R-MAT: 4096 is 7% faster than 1024).
+constexpr int HUB_CHUNK = 256;    // default; TSVD_HUB_CHUNK overrides it per PPR handle
 
 // One synchronous push round: one wave per pending item pops its lanes and
 // relaxes v's neighbours.  The atomic add returns the previous residue, so a
