@@ -272,6 +272,7 @@ def served_run(capi, w, reqs, arrivals, cap, label, seed=7, overlap=True, push_s
     log(f"{label}: initial snapshot {time.perf_counter() - t:.1f}s")
     tickets = [None] * len(reqs)
     ready = [threading.Event() for _ in reqs]
+    pr0 = capi.profile_read()
     start = time.perf_counter()
 
     def feeder():
@@ -290,9 +291,19 @@ def served_run(capi, w, reqs, arrivals, cap, label, seed=7, overlap=True, push_s
         _, r = srv.wait(tickets[i], embedding=False)
         res.append(r)
         if r["last_ticket"] == r["ticket"]:
+            ph = r["step"]
             log(f"{label}: step {r['snapshot']} done: {r['step_requests']} requests, {r['step_edges']} edges, "
-                f"{r['done_ms'] - r['start_ms']:.0f} ms, latency of request {i}: {r['latency_ms']:.0f} ms")
+                f"{r['done_ms'] - r['start_ms']:.0f} ms, latency of request {i}: {r['latency_ms']:.0f} ms "
+                f"(ingest {ph['ms_ingest']:.0f}, push {ph['ms_push']:.0f}, blocks {ph['ms_blocks']:.0f}, "
+                f"frpca {ph['ms_frpca']:.0f}, merge {ph['ms_merge']:.0f} ms)")
     th.join()
+    pr1 = capi.profile_read()
+
+    def delta(name):
+        a, b = pr0.get(name, (0, 0.0)), pr1.get(name, (0, 0.0))
+        return f"{b[0] - a[0]} ({b[1] - a[1]:.0f} ms)"
+    log(f"{label}: CSR rebuilds {delta('graph_rebuild')}, hipMalloc {delta('hip_malloc')}, "
+        f"hipFree {delta('hip_free')} over the stream")
     sigma = srv.pipeline.root_sigma()
     srv.close()
     inside = [i for i in range(len(res)) if window is None or window[0] <= arrivals[i] <= window[1]]
@@ -336,6 +347,9 @@ def main():
                     help="also compare at saturation with the two-sided push threshold")
     ap.add_argument("--served-no-overlap", action="store_true",
                     help="run the served form's steps one at a time (no back/front overlap)")
+    ap.add_argument("--push-setting", default=None, metavar="HUB,RIDE,K",
+                    help="fixed push setting (hub chunk, rider threshold, items per wave) for every pipeline; "
+                         "no push autotuning")
     ap.add_argument("--push-graphs", action="store_true",
                     help="replay the push rounds from captured hipGraphs (off by default: measured slower)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -400,6 +414,11 @@ def main():
         # every merge recomputed) and the warm-up steps, kept for the timed steps
         capi.gemm_tuning(True)
     pipe = pipeline_for(capi, w)
+    fixed = None
+    if args.push_setting:
+        h, r, k = args.push_setting.split(",")
+        fixed = (int(h), float(r), int(k))
+        pipe.ppr.push_setting = fixed
     t = time.perf_counter()
     st0 = pipe.snapshot(*batches[0])
     init_ms = (time.perf_counter() - t) * 1e3
@@ -415,7 +434,7 @@ def main():
     # the batches run again from the first one, so every timed step is one of
     # the configuration's own insertion snapshots applied to its own
     # predecessor state.
-    cur = {"pos": 1, "replays": [], "push_setting": None}
+    cur = {"pos": 1, "replays": [], "push_setting": fixed}
 
     def next_batch():
         nonlocal pipe
@@ -437,7 +456,7 @@ def main():
     tuning = None
     for i in range(args.warmup):
         j = next_batch()
-        if i == 0 and not args.no_tune:
+        if i == 0 and not args.no_tune and fixed is None:
             # the push autotuner: this snapshot's push timed under each setting from the same state
             cands = [(h, 0.5, k) for h in (256, 1024, 4096, 8192) for k in (1, 8)]
             st, chosen, ms = pipe.snapshot_tuned(*batches[j], cands)
@@ -624,8 +643,13 @@ def main():
         roof["traffic"] = profiled_traffic("k_push_round")
         roof["traffic_source"] = f"profiles/push_round_{WORKLOAD_TAG}_hbm_traffic.md"
     out["roofline"] = roof
+    host = ("hip_malloc", "hip_free", "graph_rebuild")   # host wall times, not kernels
     out["kernels"] = {k: {"launches": v[0], "ms": round(v[1], 3), "GB": round(v[2] / 1e9, 3),
-                          "GFLOP": round(v[3] / 1e9, 3)} for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+                          "GFLOP": round(v[3] / 1e9, 3)} for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])
+                      if k not in host}
+    out["host_wall"] = {"note": "calls and host wall-clock ms over the whole run: device allocations and frees "
+                                "of the library, CSR rebuilds (allocation, row move and free)",
+                        **{k: {"calls": prof[k][0], "ms": round(prof[k][1], 1)} for k in host if k in prof}}
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(w, ndirty, nmerged, args.steps)

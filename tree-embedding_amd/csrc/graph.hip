@@ -14,9 +14,11 @@
 // batch are listed (ascending) for the dynamic PPR update.
 #include "tsvd_internal.h"
 #include "tsvd_graph.h"
+#include "tsvd_profile.h"
 
 #include <hipcub/hipcub.hpp>
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 
 namespace tsvd {
@@ -182,12 +184,12 @@ static void append_dir(tsvd_graph *g, int d, const int32_t *dsrc, const int32_t 
     const int64_t ne = g->directed ? m : 2 * m;
     TSVD_REQUIRE(ne < (int64_t)INT32_MAX, "batch of %lld entries exceeds the int32 sort range", (long long)ne);
 
-    g->k_in.ensure((size_t)ne);
-    g->v_in.ensure((size_t)ne);
-    g->k_out.ensure((size_t)ne);
-    g->v_out.ensure((size_t)ne);
-    g->scan_flag.ensure((size_t)ne);
-    g->scan_pos.ensure((size_t)ne);
+    g->k_in.grow((size_t)ne);
+    g->v_in.grow((size_t)ne);
+    g->k_out.grow((size_t)ne);
+    g->v_out.grow((size_t)ne);
+    g->scan_flag.grow((size_t)ne);
+    g->scan_pos.grow((size_t)ne);
 
     D.appended = ne;
     // former <- degree (Graph_dynamic.h:545-547 / :135-138)
@@ -205,7 +207,7 @@ static void append_dir(tsvd_graph *g, int d, const int32_t *dsrc, const int32_t 
     const int bits = bits_for(g->n);
     TSVD_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, g->k_in.p, g->k_out.p, g->v_in.p, g->v_out.p, (int)ne, 0,
                                                 bits, st));
-    g->cub_tmp.ensure(tmp + 16);
+    g->cub_tmp.grow(tmp + 16);
     TSVD_HIP(hipcub::DeviceRadixSort::SortPairs(g->cub_tmp.p, tmp, g->k_in.p, g->k_out.p, g->v_in.p, g->v_out.p,
                                                 (int)ne, 0, bits, st));
 
@@ -216,7 +218,7 @@ static void append_dir(tsvd_graph *g, int d, const int32_t *dsrc, const int32_t 
     hipLaunchKernelGGL(k_boundary_flags, dim3(ge), dim3(TPB), 0, st, g->k_out.p, ne, g->scan_flag.p);
     tmp = 0;
     TSVD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, g->scan_flag.p, g->scan_pos.p, (int)ne, st));
-    g->cub_tmp.ensure(tmp + 16);
+    g->cub_tmp.grow(tmp + 16);
     TSVD_HIP(hipcub::DeviceScan::ExclusiveSum(g->cub_tmp.p, tmp, g->scan_flag.p, g->scan_pos.p, (int)ne, st));
     hipLaunchKernelGGL(k_boundary_list, dim3(ge), dim3(TPB), 0, st, g->k_out.p, ne, g->scan_pos.p, D.touched.p);
 
@@ -237,14 +239,17 @@ static void append_dir(tsvd_graph *g, int d, const int32_t *dsrc, const int32_t 
     D.ntouched = hpos_last + hflag_last;
 
     if (overflow) {
-        // rebuild with fresh slack (exclusive scan of the new capacities)
+        // rebuild with fresh slack (exclusive scan of the new capacities);
+        // its wall time, allocations and frees included, is profiled as
+        // "graph_rebuild"
+        const auto t_rb = std::chrono::steady_clock::now();
         DBuf<int64_t> cap((size_t)g->n + 1);
         DBuf<int64_t> newoff((size_t)g->n + 1);
         hipLaunchKernelGGL(k_new_cap, dim3(gn), dim3(TPB), 0, st, D.deg.p, D.cnt.p, g->n, g->slack, cap.p);
         TSVD_HIP(hipMemsetAsync(cap.p + g->n, 0, sizeof(int64_t), st));
         tmp = 0;
         TSVD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cap.p, newoff.p, g->n + 1, st));
-        g->cub_tmp.ensure(tmp + 16);
+        g->cub_tmp.grow(tmp + 16);
         TSVD_HIP(hipcub::DeviceScan::ExclusiveSum(g->cub_tmp.p, tmp, cap.p, newoff.p, g->n + 1, st));
         int64_t total = 0;
         TSVD_HIP(hipMemcpyAsync(&total, newoff.p + g->n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -260,6 +265,8 @@ static void append_dir(tsvd_graph *g, int d, const int32_t *dsrc, const int32_t 
         D.rowoff = std::move(newoff);
         D.cap_total = total;
         g->rebuilds++;
+        prof_add_time("graph_rebuild",
+                      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_rb).count());
     }
     hipLaunchKernelGGL(k_scatter, dim3(ge), dim3(TPB), 0, st, g->k_out.p, g->v_out.p, ne, g->segstart.p, D.rowoff.p,
                        D.deg.p, D.adj.p);
@@ -273,8 +280,8 @@ void graph_append(tsvd_graph *g, const int32_t *src, const int32_t *dst, int64_t
     hipStream_t st = g->stream;
     const int32_t *dsrc = src, *ddst = dst;
     if (m > 0 && !on_device) {
-        g->e_src.ensure((size_t)m);
-        g->e_dst.ensure((size_t)m);
+        g->e_src.grow((size_t)m);
+        g->e_dst.grow((size_t)m);
         TSVD_HIP(hipMemcpyAsync(g->e_src.p, src, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
         TSVD_HIP(hipMemcpyAsync(g->e_dst.p, dst, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
         dsrc = g->e_src.p;

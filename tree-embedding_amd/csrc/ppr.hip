@@ -951,8 +951,8 @@ static void build_worklist(tsvd_ppr *s, int d)
     int32_t total = 0;
     TSVD_HIP(hipMemcpyAsync(&total, off.p + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     TSVD_HIP(hipStreamSynchronize(st));
-    s->wl_u[d].ensure((size_t)std::max(total, 1));
-    s->wl_c[d].ensure((size_t)std::max(total, 1));
+    s->wl_u[d].grow((size_t)std::max(total, 1));
+    s->wl_c[d].grow((size_t)std::max(total, 1));
     hipLaunchKernelGGL(k_chunk_fill, dim3((unsigned)ceil_div(n, TPB)), dim3(TPB), 0, st, s->wl_cnt.p, off.p, n,
                        s->wl_u[d].p, s->wl_c[d].p);
     TSVD_HIP(hipGetLastError());
@@ -1059,9 +1059,9 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
         hub_cap = (uint32_t)std::min<uint64_t>(p2, cap);
     }
     const uint32_t chunk_cap = hub_cap;
-    s->hub_item.ensure(std::max<uint32_t>(hub_cap, 1));
-    s->hub_delta.ensure((size_t)std::max<uint32_t>(hub_cap, 1) * 64 * sizeof(T));
-    s->hub_chunks.ensure(std::max<uint32_t>(chunk_cap, 1));
+    s->hub_item.grow(std::max<uint32_t>(hub_cap, 1));
+    s->hub_delta.grow((size_t)std::max<uint32_t>(hub_cap, 1) * 64 * sizeof(T));
+    s->hub_chunks.grow(std::max<uint32_t>(chunk_cap, 1));
     const T ride_min = (T)(std::min(s->ride_frac, 1.0) * s->rmax);
     const uint32_t hub_grid = cu_count() * 32 / PWPB;   // 32 waves per compute unit
     auto launch_round = [&](uint64_t ri, uint32_t grid) {

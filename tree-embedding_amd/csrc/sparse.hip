@@ -98,8 +98,8 @@ void store_t(SparseWork &w, PairList &L, const void *Xv, int32_t n, int32_t s_pa
     const int32_t nchunk = (int32_t)ceil_div(n, CHUNK);
     const int32_t gcount = (int32_t)ceil_div(count, 64);
     const int64_t cells = (int64_t)count * nchunk;
-    w.cnt.ensure((size_t)cells + 1);
-    w.pos.ensure((size_t)cells + 1);
+    w.cnt.grow((size_t)cells + 1);
+    w.pos.grow((size_t)cells + 1);
     TSVD_HIP(hipMemsetAsync(w.cnt.p + cells, 0, sizeof(int64_t), st));
     const int64_t waves = (int64_t)nchunk * gcount;
     if (g_profile) prof_add_work("sparse_store", 2.0 * (double)n * gcount * 64 * sizeof(T), 0.0);
@@ -108,7 +108,7 @@ void store_t(SparseWork &w, PairList &L, const void *Xv, int32_t n, int32_t s_pa
                        gcount, nchunk, w.cnt.p);
     size_t tmp = 0;
     TSVD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.cnt.p, w.pos.p, cells + 1, st));
-    w.tmp.ensure(tmp + 16);
+    w.tmp.grow(tmp + 16);
     TSVD_HIP(hipcub::DeviceScan::ExclusiveSum(w.tmp.p, tmp, w.cnt.p, w.pos.p, cells + 1, st));
     int64_t total = 0;
     TSVD_HIP(hipMemcpyAsync(&total, w.pos.p + cells, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -118,7 +118,7 @@ void store_t(SparseWork &w, PairList &L, const void *Xv, int32_t n, int32_t s_pa
         L.idx.alloc(want);
         L.val.alloc(want * sizeof(T));
     }
-    L.off.ensure((size_t)count + 1);
+    L.off.grow((size_t)count + 1);
     hipLaunchKernelGGL(k_list_write<T>, dim3((unsigned)ceil_div(waves, WPB)), dim3(TPB), 0, st, X, n, s_pad, count,
                        gcount, nchunk, w.pos.p, L.idx.p, (T *)L.val.p);
     hipLaunchKernelGGL(k_list_offsets, dim3((unsigned)ceil_div(count + 1, TPB)), dim3(TPB), 0, st, w.pos.p, count,

@@ -219,20 +219,20 @@ void build_t(SparseLeaves &s, hipStream_t st)
     // rows then columns of every candidate of the density pass, whose column
     // counts are in place (the runs of candidates that stay dense are zero)
     const int64_t len = s.counts;
-    s.off.ensure((size_t)len + 1);
+    s.off.grow((size_t)len + 1);
     hipLaunchKernelGGL(k_row_counts<T>, dim3((unsigned)ceil_div(maxm, TPB), (unsigned)cnt), dim3(TPB), 0, st,
                        s.ditems.p, s.cnt.p);
     TSVD_HIP(hipGetLastError());
     size_t need = 0;
     TSVD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, s.cnt.p, s.off.p, len + 1, st));
-    s.tmp.ensure(need + 256);
+    s.tmp.grow(need + 256);
     need = s.tmp.n;
     TSVD_HIP(hipcub::DeviceScan::ExclusiveSum(s.tmp.p, need, s.cnt.p, s.off.p, len + 1, st));
     int64_t total = 0;
     TSVD_HIP(hipMemcpyAsync(&total, s.off.p + len, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     TSVD_HIP(hipStreamSynchronize(st));
-    s.idx.ensure((size_t)std::max<int64_t>(total, 1));
-    s.val.ensure((size_t)std::max<int64_t>(total, 1) * sizeof(T));
+    s.idx.grow((size_t)std::max<int64_t>(total, 1));
+    s.val.grow((size_t)std::max<int64_t>(total, 1) * sizeof(T));
     hipLaunchKernelGGL(k_row_fill<T>, dim3((unsigned)ceil_div(maxm, TPB), (unsigned)cnt), dim3(TPB), 0, st,
                        s.ditems.p, s.off.p, s.idx.p, (T *)s.val.p);
     hipLaunchKernelGGL(k_col_fill<T>, dim3((unsigned)ceil_div(maxnc, WPB), (unsigned)cnt), dim3(TPB), 0, st,
@@ -259,10 +259,10 @@ std::vector<double> leaf_densities(SparseLeaves &s, std::vector<SpItem> &cand, i
     }
     s.counts = base;
     s.cand = cand;
-    s.cnt.ensure((size_t)base + 1);
+    s.cnt.grow((size_t)base + 1);
     TSVD_HIP(hipMemsetAsync(s.cnt.p, 0, sizeof(int64_t) * (base + 1), st));
-    s.ditems.ensure(cand.size());
-    s.nnz.ensure(cand.size());
+    s.ditems.grow(cand.size());
+    s.nnz.grow(cand.size());
     TSVD_HIP(hipMemcpyAsync(s.ditems.p, cand.data(), sizeof(SpItem) * cand.size(), hipMemcpyHostToDevice, st));
     TSVD_HIP(hipMemsetAsync(s.nnz.p, 0, sizeof(unsigned long long) * cand.size(), st));
     int64_t maxnc = 0;
@@ -308,7 +308,7 @@ void leaf_sparse_build(SparseLeaves &s, std::vector<SpItem> items, int precision
     for (size_t c = 0; c < s.cand.size(); c++)
         if (!chosen[c] && s.cand[c].nc > 0)
             TSVD_HIP(hipMemsetAsync(s.cnt.p + s.cand[c].base + s.cand[c].m, 0, sizeof(int64_t) * s.cand[c].nc, st));
-    s.ditems.ensure(s.items.size());
+    s.ditems.grow(s.items.size());
     TSVD_HIP(hipMemcpyAsync(s.ditems.p, s.items.data(), sizeof(SpItem) * s.items.size(), hipMemcpyHostToDevice, st));
     PROF("spmm_build", st);
     if (precision == TSVD_FP64) build_t<double>(s, st);

@@ -63,9 +63,9 @@ int32_t tree_update(Tree &t, const std::vector<int32_t> &dirty_leaves, int trans
     for (size_t lvl = 1; lvl <= top && !touched.empty(); lvl++) {
         std::vector<int32_t> parents(touched.begin(), touched.end());
         const int cnt = (int)parents.size();
-        t.G.ensure((size_t)cnt * nc * nc);
-        t.lam.ensure((size_t)cnt * nc);
-        t.Wm.ensure((size_t)cnt * nc * k);
+        t.G.grow((size_t)cnt * nc * nc);
+        t.lam.grow((size_t)cnt * nc);
+        t.Wm.grow((size_t)cnt * nc * k);
         TSVD_HIP(hipMemsetAsync(t.G.p, 0, sizeof(double) * (size_t)cnt * nc * nc, st));
         std::vector<GemmProb> gp;
         for (int i = 0; i < cnt; i++) {

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdarg>
@@ -53,6 +54,11 @@ int guard(F &&body)
 }
 
 // ------------------------------------------------------------ device memory
+inline double wall_ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 template <class T>
 struct DBuf {
     T *p = nullptr;
@@ -68,10 +74,14 @@ struct DBuf {
         return *this;
     }
     ~DBuf() { release(); }
+    // hipMalloc and hipFree wall times are profiled as "hip_malloc" and
+    // "hip_free" (a large allocation can wait for the driver to clear freed
+    // memory)
     void alloc(size_t count)
     {
         release();
         if (count == 0) return;
+        const auto t0 = std::chrono::steady_clock::now();
         hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
         if (e != hipSuccess) {
             p = nullptr;
@@ -80,15 +90,28 @@ struct DBuf {
                                   hipGetErrorString(e))};
         }
         n = count;
+        if (g_profile) prof_add_time("hip_malloc", wall_ms_since(t0));
     }
     // grow (contents discarded) when the request exceeds the current size
     void ensure(size_t count)
     {
         if (count > n) alloc(count);
     }
+    // grow (contents discarded): the first allocation is exact, a regrowth
+    // takes 1/8 headroom -- for workspaces whose need creeps up from step to
+    // step (with the graph's edge count, the batch size, the dirty set), so
+    // that steps rarely reallocate them
+    void grow(size_t count)
+    {
+        if (count > n) alloc(n == 0 ? count : count + count / 8);
+    }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            const auto t0 = std::chrono::steady_clock::now();
+            (void)hipFree(p);
+            if (g_profile) prof_add_time("hip_free", wall_ms_since(t0));
+        }
         p = nullptr;
         n = 0;
     }
