@@ -9,7 +9,7 @@
 // A workgroup of 4 wavefronts owns a 64 x (16*NT) tile of C; wave w owns rows
 // [16w, 16w+16) and all NT 16-column MFMA tiles, so one A fragment feeds NT
 // matrix instructions (NT = 9 covers rank + oversampling = 133 in one tile).
-// K advances in 16-deep slabs staged through LDS (double-buffered through
+// K advances in 32-deep slabs staged through LDS (double-buffered through
 // registers: the next slab is fetched from HBM while the current one is fed
 // to the MFMAs).  MFMA 16x16x4 operand layout: lane l holds A(l%16, l/16) and
 // B(l/16, l%16); the FP32 accumulator holds C(4*(l/16)+r, l%16) and the FP64
@@ -24,7 +24,8 @@ namespace tsvd {
 namespace {
 
 constexpr int BM = 64;
-constexpr int BK = 16;
+constexpr int BK = 16;    // K slab of the bf16x3 products
+constexpr int KB = 32;    // K slab of the FP32 / FP64 products
 constexpr int TPB = 256;
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -52,10 +53,14 @@ template <class TA, class TB, class TC, class TACC, int NT>
 __global__ void __launch_bounds__(TPB) k_gemm(const GemmProb *__restrict__ probs, int nprob)
 {
     constexpr int BN = 16 * NT;
-    constexpr int A_PER = BM * BK / TPB;             // 4
-    constexpr int B_PER = (BN * BK + TPB - 1) / TPB; // 9 or 5
-    __shared__ TACC As[BK][BM];
-    __shared__ TACC Bs[BK][BN];
+    constexpr int A_PER = BM * KB / TPB;             // 8
+    constexpr int B_PER = (BN * KB + TPB - 1) / TPB; // 18 or 10
+    // operands stay in their own type in registers and LDS (converted to
+    // TACC as they feed the MFMAs); one padding element per row: the stash
+    // writes 32 consecutive k of a column into 32 rows, which land in
+    // different banks
+    __shared__ TA As[KB][BM + 1];
+    __shared__ TB Bs[KB][BN + 1];
 
     // locate the problem (binary search over tile_begin)
     const int64_t tile = blockIdx.x;
@@ -88,17 +93,18 @@ __global__ void __launch_bounds__(TPB) k_gemm(const GemmProb *__restrict__ probs
 #pragma unroll
     for (int n = 0; n < NT; n++) acc[n] = typename Acc<TACC>::V{0, 0, 0, 0};
 
-    TACC ra[A_PER], rb[B_PER];
+    TA ra[A_PER];
+    TB rb[B_PER];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < A_PER; q++) {
             const int e = tid + q * TPB;
             int ii, kk;
-            if (P.transA) { ii = e / BK; kk = e % BK; }
+            if (P.transA) { ii = e / KB; kk = e % KB; }
             else { kk = e / BM; ii = e % BM; }
             const int gi = i0 + ii, gk = k0 + kk;
-            TACC v = TACC(0);
-            if (gi < M && gk < kend) v = (TACC)(P.transA ? A[(int64_t)gi * lda + gk] : A[(int64_t)gk * lda + gi]);
+            TA v = TA(0);
+            if (gi < M && gk < kend) v = P.transA ? A[(int64_t)gi * lda + gk] : A[(int64_t)gk * lda + gi];
             ra[q] = v;
         }
 #pragma unroll
@@ -106,11 +112,11 @@ __global__ void __launch_bounds__(TPB) k_gemm(const GemmProb *__restrict__ probs
             const int e = tid + q * TPB;
             int jj, kk;
             if (P.transB) { kk = e / BN; jj = e % BN; }
-            else { jj = e / BK; kk = e % BK; }
+            else { jj = e / KB; kk = e % KB; }
             const int gj = j0 + jj, gk = k0 + kk;
-            TACC v = TACC(0);
-            if (e < BN * BK && gj < N && gk < kend)
-                v = (TACC)(P.transB ? B[(int64_t)gk * ldb + gj] : B[(int64_t)gj * ldb + gk]);
+            TB v = TB(0);
+            if (e < BN * KB && gj < N && gk < kend)
+                v = P.transB ? B[(int64_t)gk * ldb + gj] : B[(int64_t)gj * ldb + gk];
             rb[q] = v;
         }
     };
@@ -119,32 +125,34 @@ __global__ void __launch_bounds__(TPB) k_gemm(const GemmProb *__restrict__ probs
         for (int q = 0; q < A_PER; q++) {
             const int e = tid + q * TPB;
             int ii, kk;
-            if (P.transA) { ii = e / BK; kk = e % BK; }
+            if (P.transA) { ii = e / KB; kk = e % KB; }
             else { kk = e / BM; ii = e % BM; }
             As[kk][ii] = ra[q];
         }
 #pragma unroll
         for (int q = 0; q < B_PER; q++) {
             const int e = tid + q * TPB;
-            if (e < BN * BK) {
+            if (e < BN * KB) {
                 if (P.transB) Bs[e / BN][e % BN] = rb[q];
-                else Bs[e % BK][e / BK] = rb[q];
+                else Bs[e % KB][e / KB] = rb[q];
             }
         }
     };
 
     if (kbeg < kend) fetch(kbeg);
-    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    for (int k0 = kbeg; k0 < kend; k0 += KB) {
         __syncthreads();
         stash();
         __syncthreads();
-        if (k0 + BK < kend) fetch(k0 + BK);
-#pragma unroll
-        for (int ks = 0; ks < BK; ks += 4) {
-            const TACC a = As[ks + (lane >> 4)][wave * 16 + (lane & 15)];
+        if (k0 + KB < kend) fetch(k0 + KB);
+        // two k-steps of operands at a time (a full unroll hoists all eight
+        // steps' LDS reads and halves the occupancy)
+#pragma unroll 2
+        for (int ks = 0; ks < KB; ks += 4) {
+            const TACC a = (TACC)As[ks + (lane >> 4)][wave * 16 + (lane & 15)];
 #pragma unroll
             for (int n = 0; n < NT; n++) {
-                const TACC b = Bs[ks + (lane >> 4)][n * 16 + (lane & 15)];
+                const TACC b = (TACC)Bs[ks + (lane >> 4)][n * 16 + (lane & 15)];
                 acc[n] = Acc<TACC>::mfma(a, b, acc[n]);
             }
         }
@@ -434,7 +442,7 @@ void GemmRunner::launch_once(hipStream_t st, GemmProb *probs, int nprobs, GemmTy
         p.tiles_n = (int32_t)ceil_div(p.N, BN);
         p.tiles_k = 1;
         if (p.kchunk > 0 && p.K > p.kchunk) {
-            TSVD_REQUIRE(p.kchunk % BK == 0, "kchunk must be a multiple of 16");
+            TSVD_REQUIRE(p.kchunk % KB == 0, "kchunk must be a multiple of 32");
             p.tiles_k = (int32_t)ceil_div(p.K, p.kchunk);
         }
         p.tile_begin = tiles;
