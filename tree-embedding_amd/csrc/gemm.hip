@@ -83,8 +83,8 @@ __global__ void __launch_bounds__(TPB) k_gemm(const GemmProb *__restrict__ probs
         kbeg = tk * P.kchunk;
         kend = min(P.K, kbeg + P.kchunk);
     }
-    const TA *__restrict__ A = (const TA *)P.A;
-    const TB *__restrict__ B = (const TB *)P.B;
+    const gptr<const TA> __restrict__ A = gbl((const TA *)P.A);
+    const gptr<const TB> __restrict__ B = gbl((const TB *)P.B);
     const int64_t lda = P.lda, ldb = P.ldb;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -216,8 +216,8 @@ __global__ void __launch_bounds__(TPB) k_gemm_bf3(const GemmProb *__restrict__ p
     const int tn = (int)(t / P.tiles_m);
     const int M = P.M, N = P.N, K = P.K;
     const int i0 = tm * BM, j0 = tn * BN;
-    const float *__restrict__ A = (const float *)P.A;
-    const TB *__restrict__ B = (const TB *)P.B;
+    const gptr<const float> __restrict__ A = gbl((const float *)P.A);
+    const gptr<const TB> __restrict__ B = gbl((const TB *)P.B);
     const int64_t lda = P.lda, ldb = P.ldb;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     f32x4 acc[NT];
@@ -229,13 +229,16 @@ __global__ void __launch_bounds__(TPB) k_gemm_bf3(const GemmProb *__restrict__ p
         for (int q = 0; q < A_PER; q++) {
             const int e = tid + q * TPB, kk = e / BM, ii = e % BM;
             const int gi = i0 + ii, gk = k0 + kk;
-            ra[q] = (gi < M && gk < K) ? A[(int64_t)gk * lda + gi] : 0.f;
+            // clamped (not conditional) loads, so that they are all in flight together
+            const float v = A[(int64_t)min(gk, K - 1) * lda + min(gi, M - 1)];
+            ra[q] = (gi < M && gk < K) ? v : 0.f;
         }
 #pragma unroll
         for (int q = 0; q < B_PER; q++) {
             const int e = tid + q * TPB, jj = e / BK, kk = e % BK;
             const int gj = j0 + jj, gk = k0 + kk;
-            rb[q] = (e < BN * BK && gj < N && gk < K) ? (float)B[(int64_t)gj * ldb + gk] : 0.f;
+            const float v = (float)B[(int64_t)min(gj, N - 1) * ldb + min(gk, K - 1)];
+            rb[q] = (e < BN * BK && gj < N && gk < K) ? v : 0.f;
         }
     };
     auto stash = [&]() {

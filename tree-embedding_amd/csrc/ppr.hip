@@ -462,8 +462,9 @@ __global__ void __launch_bounds__(TPB) k_edge_frontier(const T *__restrict__ R, 
 // atomic add per neighbour; a lane whose residue crosses rmax upward is
 // flagged, and the wave's ballot is OR-ed into the next round's mask of (u, g).
 template <class T>
-__device__ inline void relax_range(const int32_t *__restrict__ nb, int32_t j0, int32_t j1, bool pushing, T delta,
-                                   T *__restrict__ Rs, int32_t s_pad, const int32_t *__restrict__ deg, uint32_t G,
+__device__ inline void relax_range(const gptr<const int32_t> __restrict__ nb, int32_t j0, int32_t j1, bool pushing,
+                                   T delta, T *__restrict__ Rs, int32_t s_pad, const gptr<const int32_t> __restrict__ deg,
+                                   uint32_t G,
                                    int32_t g, T rm, int two, int lane, unsigned long long *__restrict__ nm,
                                    const WaveStage &sg)
 {
@@ -576,9 +577,9 @@ __global__ void __launch_bounds__(PTPB, 8) k_push_round(T *__restrict__ P, T *__
     // [4] relaxations done here (split rows hand the rest to k_push_hub),
     // [5] 128-byte lines the row's atomics touch (row visits x lines holding
     //     a pushing lane: the L2 atomic units work per line, not per lane)
-    const int64_t *__restrict__ rowoff = csr->rowoff;
-    const int32_t *__restrict__ adj = csr->adj;
-    const int32_t *__restrict__ deg = csr->deg;
+    const gptr<const int64_t> __restrict__ rowoff = gbl(csr->rowoff);
+    const gptr<const int32_t> __restrict__ adj = gbl(csr->adj);
+    const gptr<const int32_t> __restrict__ deg = gbl(csr->deg);
     __shared__ unsigned long long sstat[6];
     if (threadIdx.x < 6) sstat[threadIdx.x] = 0ull;
     const uint32_t nitems = min(*count_in, in_cap);
@@ -659,7 +660,7 @@ __global__ void __launch_bounds__(PTPB, 8) k_push_round(T *__restrict__ P, T *__
                 atomicAdd(&sstat[5], (unsigned long long)lines * (unsigned long long)dv);
             }
             const T delta = pushing ? (T)((1.0 - alpha) * (double)x / (double)dv) : T(0);
-            const int32_t *nb = adj + rowoff[v];
+            const gptr<const int32_t> nb = adj + rowoff[v];
             int32_t jend = dv;
             if (dv > hub_chunk) {
                 const uint32_t more = (uint32_t)((dv - 1) / hub_chunk);   // chunks after the first
@@ -710,9 +711,9 @@ __global__ void __launch_bounds__(PTPB) k_push_hub(T *__restrict__ R, const uint
                                                   const PushCsr *__restrict__ csr,
                                                   int32_t s_pad, int32_t ngroups, T rm, int two, int32_t hub_chunk)
 {
-    const int64_t *__restrict__ rowoff = csr->rowoff;
-    const int32_t *__restrict__ adj = csr->adj;
-    const int32_t *__restrict__ deg = csr->deg;
+    const gptr<const int64_t> __restrict__ rowoff = gbl(csr->rowoff);
+    const gptr<const int32_t> __restrict__ adj = gbl(csr->adj);
+    const gptr<const int32_t> __restrict__ deg = gbl(csr->deg);
     TSVD_WAVE_STAGE(sg, next, next_count, cap, overflow);
     const uint32_t nchunks = min(*chunk_count, chunk_cap);
     const int lane = threadIdx.x & 63;

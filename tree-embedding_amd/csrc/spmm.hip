@@ -14,12 +14,14 @@
 //
 // Building: (1) one wave per stored column counts its non-zero rows (ballot)
 // and adds the count to its block's total; the host picks the sparse blocks
-// by density; (2) for those, one wave per 64-row chunk counts each row's
-// non-zeros (lanes = rows, walking the block's columns, which are contiguous
-// per column); (3) one exclusive scan over every sparse block's row counts
-// and column counts, laid out block by block (rows, then columns), gives each
-// row and column its run in one entry array; (4) the same two walks write
-// the entries (CSR: column ids ascending; CSC: row ids ascending).
+// by density; (2) for those, each of 8 column ranges of a block has one
+// thread per row count that row's non-zeros in the range (lanes = rows,
+// walking the range's columns, which are contiguous per column), and the
+// ranges' partial counts are summed per row; (3) one exclusive scan over
+// every sparse block's row counts and column counts, laid out block by block
+// (rows, then columns), gives each row and column its run in one entry
+// array; (4) the same walks write the entries, a range's row entries after
+// the earlier ranges' (CSR: column ids ascending; CSC: row ids ascending).
 #include "tsvd_pipeline.h"
 
 #include <hipcub/hipcub.hpp>
@@ -31,73 +33,122 @@ namespace {
 constexpr int TPB = 256;
 constexpr int WPB = TPB / 64;
 constexpr int RU = 16;   // column loads in flight per thread in the row walks
+constexpr int RS = 8;    // column ranges of a block, each walked by its own threads in the row walks
 constexpr int GU = 8;    // entries whose operand rows are in flight per wave in the gathers
-constexpr int CU = 4;    // 64-row segments in flight per wave in the column walks
+constexpr int CU = 16;   // 64-row segments in flight per wave in the column walks
+constexpr int CPW = 8;   // columns per wave in the column walks
 
-template <class T>
-__global__ void __launch_bounds__(TPB) k_col_counts(const SpItem *__restrict__ items, int64_t *__restrict__ cnt,
-                                                    unsigned long long *__restrict__ item_nnz, int with_cnt)
+// Column range s of nc columns, [c0, c1).
+__device__ inline void col_range(int64_t nc, int s, int64_t &c0, int64_t &c1)
 {
-    const SpItem &it = items[blockIdx.y];
-    const int64_t c = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (c >= it.nc) return;
-    const int lane = threadIdx.x & 63;
-    const T *col = (const T *)it.A + c * it.lda;
-    uint32_t n = 0;
-    for (int32_t r0 = 0; r0 < it.m; r0 += 64 * CU) {
-        T x[CU];
-#pragma unroll
-        for (int q = 0; q < CU; q++) {
-            const int32_t r = r0 + q * 64 + lane;
-            x[q] = r < it.m ? col[r] : T(0);
-        }
-#pragma unroll
-        for (int q = 0; q < CU; q++) n += (uint32_t)__popcll(__ballot(x[q] != T(0)));
-    }
-    if (lane == 0) {
-        if (with_cnt) cnt[it.base + it.m + c] = (int64_t)n;
-        atomicAdd(item_nnz + blockIdx.y, (unsigned long long)n);
-    }
+    c0 = nc * s / RS;
+    c1 = nc * (s + 1) / RS;
 }
 
 template <class T>
-__global__ void __launch_bounds__(TPB) k_row_counts(const SpItem *__restrict__ items, int64_t *__restrict__ cnt)
+__global__ void __launch_bounds__(TPB) k_col_counts(const SpItem *__restrict__ items, int64_t *__restrict__ cnt)
+{
+    const SpItem &it = items[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    // a wave walks every (gridDim.x * WPB)-th column of its item
+    for (int64_t c = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6); c < it.nc; c += (int64_t)gridDim.x * WPB) {
+        const gptr<const T> col = gbl((const T *)it.A) + c * it.lda;
+        uint32_t n = 0;
+        for (int32_t r0 = 0; r0 < it.m; r0 += 64 * CU) {
+            T x[CU];
+            // clamped (not conditional) loads, so that all CU are in flight together
+#pragma unroll
+            for (int q = 0; q < CU; q++) x[q] = col[min(r0 + q * 64 + lane, it.m - 1)];
+#pragma unroll
+            for (int q = 0; q < CU; q++)
+                n += (uint32_t)__popcll(__ballot(r0 + q * 64 + lane < it.m && x[q] != T(0)));
+        }
+        if (lane == 0) cnt[it.base + it.m + c] = (int64_t)n;
+    }
+}
+
+// Non-zeros of each item: the sum of its column counts, one workgroup per
+// item (a fixed summation order, no atomics)
+__global__ void __launch_bounds__(TPB) k_item_nnz(const SpItem *__restrict__ items, const int64_t *__restrict__ cnt,
+                                                  unsigned long long *__restrict__ item_nnz)
+{
+    const SpItem &it = items[blockIdx.x];
+    __shared__ unsigned long long part[WPB];
+    unsigned long long n = 0;
+    for (int64_t c = threadIdx.x; c < it.nc; c += TPB) n += (unsigned long long)cnt[it.base + it.m + c];
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < WPB; w++) t += part[w];
+        item_nnz[blockIdx.x] = t;
+    }
+}
+
+// Row counts of one column range (blockIdx.z) per thread: lane = row,
+// walking the range's columns (each column contiguous, so a wave's loads are
+// coalesced); partial counts go to part[item][range][row]
+template <class T>
+__global__ void __launch_bounds__(TPB) k_row_counts(const SpItem *__restrict__ items, int32_t *__restrict__ part,
+                                                    int64_t maxm)
 {
     const SpItem &it = items[blockIdx.y];
     const int32_t r = (int32_t)((int64_t)blockIdx.x * TPB + threadIdx.x);
-    if ((int64_t)blockIdx.x * TPB >= it.m) return;
     if (r >= it.m) return;
-    const T *__restrict__ A = (const T *)it.A + r;
-    int64_t n = 0;
+    int64_t c0, c1;
+    col_range(it.nc, blockIdx.z, c0, c1);
+    const gptr<const T> __restrict__ A = gbl((const T *)it.A) + r;
+    int32_t n = 0;
     // RU column loads in flight per thread (the loop is latency-bound otherwise)
-    for (int64_t c0 = 0; c0 < it.nc; c0 += RU) {
+    for (int64_t cb = c0; cb < c1; cb += RU) {
         T x[RU];
+        // clamped (not conditional) loads, so that all RU are in flight together
 #pragma unroll
-        for (int u = 0; u < RU; u++) x[u] = c0 + u < it.nc ? A[(c0 + u) * it.lda] : T(0);
+        for (int u = 0; u < RU; u++) x[u] = A[min(cb + u, c1 - 1) * it.lda];
 #pragma unroll
-        for (int u = 0; u < RU; u++) n += x[u] != T(0);
+        for (int u = 0; u < RU; u++) n += (cb + u < c1 && x[u] != T(0));
     }
+    part[((int64_t)blockIdx.y * RS + blockIdx.z) * maxm + r] = n;
+}
+
+// Each row's count: the sum of its ranges' partial counts
+__global__ void __launch_bounds__(TPB) k_row_sum(const SpItem *__restrict__ items, const int32_t *__restrict__ part,
+                                                 int64_t maxm, int64_t *__restrict__ cnt)
+{
+    const SpItem &it = items[blockIdx.y];
+    const int32_t r = (int32_t)((int64_t)blockIdx.x * TPB + threadIdx.x);
+    if (r >= it.m) return;
+    int64_t n = 0;
+#pragma unroll
+    for (int q = 0; q < RS; q++) n += part[((int64_t)blockIdx.y * RS + q) * maxm + r];
     cnt[it.base + r] = n;
 }
 
-// CSR entries: lane = row, walking the columns in order
+// CSR entries of one column range (blockIdx.z): lane = row, walking the
+// range's columns in order from the row's offset plus the earlier ranges'
+// counts, so every row's column ids come out ascending
 template <class T>
 __global__ void __launch_bounds__(TPB) k_row_fill(const SpItem *__restrict__ items, const int64_t *__restrict__ off,
+                                                  const int32_t *__restrict__ part, int64_t maxm,
                                                   int32_t *__restrict__ idx, T *__restrict__ val)
 {
     const SpItem &it = items[blockIdx.y];
     const int32_t r = (int32_t)((int64_t)blockIdx.x * TPB + threadIdx.x);
     if (r >= it.m) return;
     int64_t p = off[it.base + r];
-    const T *__restrict__ A = (const T *)it.A + r;
-    for (int64_t c0 = 0; c0 < it.nc; c0 += RU) {
+    for (int q = 0; q < (int)blockIdx.z; q++) p += part[((int64_t)blockIdx.y * RS + q) * maxm + r];
+    int64_t c0, c1;
+    col_range(it.nc, blockIdx.z, c0, c1);
+    const gptr<const T> __restrict__ A = gbl((const T *)it.A) + r;
+    for (int64_t cb = c0; cb < c1; cb += RU) {
         T x[RU];
 #pragma unroll
-        for (int u = 0; u < RU; u++) x[u] = c0 + u < it.nc ? A[(c0 + u) * it.lda] : T(0);
+        for (int u = 0; u < RU; u++) x[u] = cb + u < c1 ? A[(cb + u) * it.lda] : T(0);
 #pragma unroll
         for (int u = 0; u < RU; u++) {
             if (x[u] != T(0)) {
-                idx[p] = (int32_t)(c0 + u);
+                idx[p] = (int32_t)(cb + u);
                 val[p] = x[u];
                 p++;
             }
@@ -105,36 +156,35 @@ __global__ void __launch_bounds__(TPB) k_row_fill(const SpItem *__restrict__ ite
     }
 }
 
-// CSC entries: one wave per column, rows in order (ballot + prefix popcount)
+// CSC entries: a wave per column at a time, rows in order (ballot + prefix popcount)
 template <class T>
 __global__ void __launch_bounds__(TPB) k_col_fill(const SpItem *__restrict__ items, const int64_t *__restrict__ off,
                                                   int32_t *__restrict__ idx, T *__restrict__ val)
 {
     const SpItem &it = items[blockIdx.y];
-    const int64_t c = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (c >= it.nc) return;
     const int lane = threadIdx.x & 63;
-    const T *col = (const T *)it.A + c * it.lda;
-    int64_t p = off[it.base + it.m + c];
-    for (int32_t r0 = 0; r0 < it.m; r0 += 64 * CU) {
-        T xs[CU];
+    // a wave walks every (gridDim.x * WPB)-th column of its item
+    for (int64_t c = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6); c < it.nc; c += (int64_t)gridDim.x * WPB) {
+        const gptr<const T> col = gbl((const T *)it.A) + c * it.lda;
+        int64_t p = off[it.base + it.m + c];
+        for (int32_t r0 = 0; r0 < it.m; r0 += 64 * CU) {
+            T xs[CU];
+            // clamped (not conditional) loads, so that all CU are in flight together
 #pragma unroll
-        for (int q = 0; q < CU; q++) {
-            const int32_t r = r0 + q * 64 + lane;
-            xs[q] = r < it.m ? col[r] : T(0);
-        }
+            for (int q = 0; q < CU; q++) xs[q] = col[min(r0 + q * 64 + lane, it.m - 1)];
 #pragma unroll
-        for (int q = 0; q < CU; q++) {
-            const int32_t r = r0 + q * 64 + lane;
-            const T x = xs[q];
-            const bool nz = x != T(0);
-            const unsigned long long b = __ballot(nz);
-            const int before = __popcll(b & ((1ull << lane) - 1ull));
-            if (nz) {
-                idx[p + before] = r;
-                val[p + before] = x;
+            for (int q = 0; q < CU; q++) {
+                const int32_t r = r0 + q * 64 + lane;
+                const T x = xs[q];
+                const bool nz = r < it.m && x != T(0);
+                const unsigned long long b = __ballot(nz);
+                const int before = __popcll(b & ((1ull << lane) - 1ull));
+                if (nz) {
+                    idx[p + before] = r;
+                    val[p + before] = x;
+                }
+                p += __popcll(b);
             }
-            p += __popcll(b);
         }
     }
 }
@@ -153,8 +203,8 @@ __global__ void __launch_bounds__(TPB) k_spmm_gather(const SpItem *__restrict__ 
     const int64_t nout = ROWS ? it.m : it.nc;
     if (r >= nout) return;
     const int lane = threadIdx.x & 63;
-    const T *__restrict__ in = (const T *)(ROWS ? it.Qt : it.Q);
-    T *__restrict__ out = (T *)(ROWS ? it.Q : it.Qt);
+    const gptr<const T> __restrict__ in = gbl((const T *)(ROWS ? it.Qt : it.Q));
+    const gptr<T> __restrict__ out = gbl((T *)(ROWS ? it.Q : it.Qt));
     const int64_t at = ROWS ? it.base + r : it.base + it.m + r;
     const int64_t b = off[at], e = off[at + 1];
     T a0 = T(0), a1 = T(0), a2 = T(0);
@@ -180,7 +230,7 @@ __global__ void __launch_bounds__(TPB) k_spmm_gather(const SpItem *__restrict__ 
             }
 #pragma unroll
             for (int q = 0; q < GU; q++) {
-                const T *__restrict__ x = in + (int64_t)jj[q] * l;
+                const gptr<const T> __restrict__ x = in + (int64_t)jj[q] * l;
                 x0[q] = lane < l ? x[lane] : T(0);
                 x1[q] = lane + 64 < l ? x[lane + 64] : T(0);
                 x2[q] = lane + 128 < l ? x[lane + 128] : T(0);
@@ -195,13 +245,13 @@ __global__ void __launch_bounds__(TPB) k_spmm_gather(const SpItem *__restrict__ 
         for (; u < cnt; u++) {
             const int32_t jj = __shfl(j, u);
             const T a = __shfl(v, u);
-            const T *__restrict__ x = in + (int64_t)jj * l;
+            const gptr<const T> __restrict__ x = in + (int64_t)jj * l;
             if (lane < l) a0 += a * x[lane];
             if (lane + 64 < l) a1 += a * x[lane + 64];
             if (lane + 128 < l) a2 += a * x[lane + 128];
         }
     }
-    T *__restrict__ y = out + r * l;
+    const gptr<T> __restrict__ y = out + r * l;
     if (lane < l) y[lane] = a0;
     if (lane + 64 < l) y[lane + 64] = a1;
     if (lane + 128 < l) y[lane + 128] = a2;
@@ -220,8 +270,11 @@ void build_t(SparseLeaves &s, hipStream_t st)
     // counts are in place (the runs of candidates that stay dense are zero)
     const int64_t len = s.counts;
     s.off.grow((size_t)len + 1);
-    hipLaunchKernelGGL(k_row_counts<T>, dim3((unsigned)ceil_div(maxm, TPB), (unsigned)cnt), dim3(TPB), 0, st,
-                       s.ditems.p, s.cnt.p);
+    s.part.grow((size_t)cnt * RS * maxm);
+    const dim3 rgrid((unsigned)ceil_div(maxm, TPB), (unsigned)cnt, RS);
+    hipLaunchKernelGGL(k_row_counts<T>, rgrid, dim3(TPB), 0, st, s.ditems.p, s.part.p, maxm);
+    hipLaunchKernelGGL(k_row_sum, dim3((unsigned)ceil_div(maxm, TPB), (unsigned)cnt), dim3(TPB), 0, st, s.ditems.p,
+                       s.part.p, maxm, s.cnt.p);
     TSVD_HIP(hipGetLastError());
     size_t need = 0;
     TSVD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, s.cnt.p, s.off.p, len + 1, st));
@@ -233,9 +286,9 @@ void build_t(SparseLeaves &s, hipStream_t st)
     TSVD_HIP(hipStreamSynchronize(st));
     s.idx.grow((size_t)std::max<int64_t>(total, 1));
     s.val.grow((size_t)std::max<int64_t>(total, 1) * sizeof(T));
-    hipLaunchKernelGGL(k_row_fill<T>, dim3((unsigned)ceil_div(maxm, TPB), (unsigned)cnt), dim3(TPB), 0, st,
-                       s.ditems.p, s.off.p, s.idx.p, (T *)s.val.p);
-    hipLaunchKernelGGL(k_col_fill<T>, dim3((unsigned)ceil_div(maxnc, WPB), (unsigned)cnt), dim3(TPB), 0, st,
+    hipLaunchKernelGGL(k_row_fill<T>, rgrid, dim3(TPB), 0, st, s.ditems.p, s.off.p, s.part.p, maxm, s.idx.p,
+                       (T *)s.val.p);
+    hipLaunchKernelGGL(k_col_fill<T>, dim3((unsigned)ceil_div(maxnc, WPB * CPW), (unsigned)cnt), dim3(TPB), 0, st,
                        s.ditems.p, s.off.p, s.idx.p, (T *)s.val.p);
     TSVD_HIP(hipGetLastError());
     s.entries = total;
@@ -270,11 +323,12 @@ std::vector<double> leaf_densities(SparseLeaves &s, std::vector<SpItem> &cand, i
     if (maxnc > 0) {
         PROF("spmm_count", st);
         if (precision == TSVD_FP64)
-            hipLaunchKernelGGL(k_col_counts<double>, dim3((unsigned)ceil_div(maxnc, WPB), (unsigned)cnt), dim3(TPB),
-                               0, st, s.ditems.p, s.cnt.p, s.nnz.p, 1);
+            hipLaunchKernelGGL(k_col_counts<double>, dim3((unsigned)ceil_div(maxnc, WPB * CPW), (unsigned)cnt), dim3(TPB),
+                               0, st, s.ditems.p, s.cnt.p);
         else
-            hipLaunchKernelGGL(k_col_counts<float>, dim3((unsigned)ceil_div(maxnc, WPB), (unsigned)cnt), dim3(TPB),
-                               0, st, s.ditems.p, s.cnt.p, s.nnz.p, 1);
+            hipLaunchKernelGGL(k_col_counts<float>, dim3((unsigned)ceil_div(maxnc, WPB * CPW), (unsigned)cnt), dim3(TPB),
+                               0, st, s.ditems.p, s.cnt.p);
+        hipLaunchKernelGGL(k_item_nnz, dim3((unsigned)cnt), dim3(TPB), 0, st, s.ditems.p, s.cnt.p, s.nnz.p);
         TSVD_HIP(hipGetLastError());
     }
     std::vector<unsigned long long> h(cand.size());

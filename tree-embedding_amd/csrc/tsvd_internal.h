@@ -119,6 +119,23 @@ struct DBuf {
 };
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// A pointer read from a descriptor in memory is generic to the compiler, so
+// its accesses become flat instructions, which also count against the LDS
+// wait counter: a flat load in flight then stalls every LDS wait after it.
+// Device buffers are global memory; these casts make the accesses global.
+template <class T>
+using gptr = T __attribute__((address_space(1))) *;
+template <class T>
+__device__ inline gptr<const T> gbl(const T *p)
+{
+    return (gptr<const T>)p;
+}
+template <class T>
+__device__ inline gptr<T> gbl(T *p)
+{
+    return (gptr<T>)p;
+}
 inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
 
 // Event-bracketed timing on one stream.

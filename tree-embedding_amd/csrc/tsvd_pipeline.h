@@ -84,6 +84,7 @@ struct SparseLeaves {
     DBuf<unsigned long long> nnz;   // per item
     DBuf<int64_t> cnt, off;         // counts and their exclusive scan (one run per row, then per column)
     DBuf<int32_t> idx;              // CSR column ids / CSC row ids
+    DBuf<int32_t> part;             // row counts per column range (item, range, row)
     DBuf<uint8_t> val, tmp;
     int64_t counts = 0, entries = 0;
 };
