@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(TPB) k_block_norms(const T *__restrict__ p0, c
 // The transform and the stored-column lookup are those of k_block_norms, so
 // a block's sums over its items are that kernel's sums in another order.
 constexpr int DONE_SHARDS = 32;
-constexpr int IB = 4;   // flagged items a wave loads together
+constexpr int IB = 8;   // flagged items a wave loads together
 
 template <class T>
 __global__ void __launch_bounds__(TPB) k_item_norms(const T *__restrict__ p0, const T *__restrict__ p1,
@@ -127,21 +127,32 @@ __global__ void __launch_bounds__(TPB) k_item_norms(const T *__restrict__ p0, co
                     bal &= bal - 1ull;
                 }
             }
-            T x[IB], ao[IB];
+            // every load is issued unconditionally at an in-bounds address
+            // (an absent item reads item it0, a lane past nsrc reads the row's
+            // padding, an unstored vertex reads its block's first stored
+            // column) and masked afterwards, so all are in flight together
+            int64_t vv[IB];
+            int32_t ss[IB], sl[IB];
 #pragma unroll
             for (int u = 0; u < IB; u++) {
-                x[u] = T(0);
-                ao[u] = T(0);
-                if (its[u] < 0) continue;
-                const int64_t v = its[u] / ngroups;
-                const int32_t s = (int32_t)(its[u] - v * ngroups) * 64 + lane;
-                const int64_t j = min(v / common, (int64_t)nparts - 1);
-                const int32_t sl = slot[v];
-                if (s < nsrc) {
-                    x[u] = p0[v * s_pad + s];
-                    if (p1) x[u] += p1[v * s_pad + s];
-                    if (sl >= 0) ao[u] = A[(j * common + sl) * s_pad + s];
-                }
+                const int64_t item = its[u] < 0 ? it0 : its[u];
+                vv[u] = item / ngroups;
+                ss[u] = (int32_t)(item - vv[u] * ngroups) * 64 + lane;
+                sl[u] = slot[vv[u]];
+            }
+            T x[IB], x1[IB], ao[IB];
+#pragma unroll
+            for (int u = 0; u < IB; u++) {
+                const int64_t j = min(vv[u] / common, (int64_t)nparts - 1);
+                x[u] = p0[vv[u] * s_pad + ss[u]];
+                x1[u] = p1 ? p1[vv[u] * s_pad + ss[u]] : T(0);
+                ao[u] = A[(j * common + max(sl[u], 0)) * s_pad + ss[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < IB; u++) {
+                const bool live = its[u] >= 0 && ss[u] < nsrc;
+                x[u] = live ? x[u] + x1[u] : T(0);
+                ao[u] = (live && sl[u] >= 0) ? ao[u] : T(0);
             }
             double a2[IB], e2[IB];
 #pragma unroll
