@@ -194,9 +194,7 @@ static void sparse_front(tsvd_pipeline *p, int32_t it, const int32_t *dsrc, cons
             if (fresh) {
                 ppr_refresh(gp, TSVD_REFRESH_INIT, d, 0, cnt);
             } else {
-                if (ss.resident) {
-                    ppr_clamp_negative(gp, d);   // what storing the positive entries drops
-                } else {
+                if (!ss.resident) {
                     ppr_clear(gp, d);
                     list_load(L[(size_t)(2 * d)], gp->p(d), gp->s_pad, st);
                     list_load(L[(size_t)(2 * d + 1)], gp->r(d), gp->s_pad, st);
@@ -205,8 +203,12 @@ static void sparse_front(tsvd_pipeline *p, int32_t it, const int32_t *dsrc, cons
             }
             const double t1 = now_ms();
             ppr_push(gp, d, 0, cnt);
-            S.ms_refresh += t1 - t0;
-            S.ms_push += now_ms() - t1;
+            const double t2p = now_ms();
+            // what storing the positive entries drops: a resident state keeps
+            // its negative residues, which sit on the rows the update lowered
+            if (!fresh && ss.resident) ppr_clamp_rows(gp, d);
+            S.ms_refresh += (t1 - t0) + (now_ms() - t2p);
+            S.ms_push += t2p - t1;
             S.push_rounds += gp->last_rounds[d];
             S.push_pops += gp->last_pops[d];
             S.push_relax += gp->last_relax[d];

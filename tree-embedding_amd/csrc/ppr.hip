@@ -370,13 +370,6 @@ __global__ void k_lane_column(const T *__restrict__ X, int32_t n, int32_t s_pad,
     if (v < n) out[v] = (double)X[v * s_pad + lane];
 }
 
-template <class T>
-__global__ void k_clamp_negative(T *__restrict__ R, int64_t cells)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < cells && R[i] < T(0)) R[i] = T(0);
-}
-
 __global__ void k_first_edge(const int32_t *__restrict__ efrom, int64_t m, const int32_t *__restrict__ former,
                              int32_t *__restrict__ first, int reset)
 {
@@ -387,6 +380,14 @@ __global__ void k_first_edge(const int32_t *__restrict__ efrom, int64_t m, const
     else if (former[i] == 0) atomicMin(first + i, (int32_t)e);
 }
 
+// The edge-order update kernels take EB edges (or rows) per wave and source
+// group: lanes 0..EB-1 fetch the block's ids and what decides each one's use
+// (clamped in-bounds loads, all in flight together), readlane hands them to
+// the wave, and the EB residue loads and atomics are issued together.
+constexpr int EB = 8;
+
+__device__ inline int32_t rdl(int32_t v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
 template <class T>
 __global__ void __launch_bounds__(TPB) k_edge_rows(T *__restrict__ P, T *__restrict__ R,
                                                    const int32_t *__restrict__ touched, int32_t ntouched,
@@ -395,23 +396,43 @@ __global__ void __launch_bounds__(TPB) k_edge_rows(T *__restrict__ P, T *__restr
                                                    double alpha, int phase)
 {
     const int64_t wave = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (wave >= (int64_t)ntouched * gcount) return;
-    const int32_t t = (int32_t)(wave / gcount);
-    const int32_t s = (int32_t)(wave - (int64_t)t * gcount) * 64 + (threadIdx.x & 63);
-    if (s < s0 || s >= s1) return;
-    const int32_t i = touched[t];
-    const int32_t d = former[i];
-    const int64_t at = (int64_t)i * s_pad + s;
+    const int64_t nblk = ((int64_t)ntouched + EB - 1) / EB;
+    if (wave >= nblk * gcount) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t tb = wave / gcount;
+    const int32_t s = (int32_t)(wave - tb * gcount) * 64 + lane;
+    const bool on = s >= s0 && s < s1;
+    const int32_t tl = (int32_t)min(tb * EB + min(lane, EB - 1), (int64_t)ntouched - 1);
+    const int32_t il = touched[tl];
+    const int32_t fl = former[il], dl = deg[il];
+    int32_t iq[EB], dq[EB], mq[EB];
+    bool use[EB];
+#pragma unroll
+    for (int q = 0; q < EB; q++) {
+        iq[q] = rdl(il, q);
+        dq[q] = rdl(fl, q);
+        mq[q] = rdl(dl, q) - dq[q];
+        use[q] = on && tb * EB + q < ntouched;
+    }
     if (phase == 0) {
-        if (d == 0) R[at] = T(0);
+#pragma unroll
+        for (int q = 0; q < EB; q++)
+            if (use[q] && dq[q] == 0) R[(int64_t)iq[q] * s_pad + s] = T(0);
         return;
     }
-    if (d == 0) return;
-    const int32_t m = deg[i] - d;
-    const double p0 = (double)P[at];
-    const double c = p0 / ((double)d * alpha);
-    R[at] = (T)((double)R[at] - (double)m * c);
-    P[at] = (T)(p0 * ((double)(d + m) / (double)d));
+    T p0[EB], r0[EB];
+#pragma unroll
+    for (int q = 0; q < EB; q++) {
+        p0[q] = P[(int64_t)iq[q] * s_pad + s];
+        r0[q] = R[(int64_t)iq[q] * s_pad + s];
+    }
+#pragma unroll
+    for (int q = 0; q < EB; q++) {
+        if (!use[q] || dq[q] == 0) continue;
+        const double c = (double)p0[q] / ((double)dq[q] * alpha);
+        R[(int64_t)iq[q] * s_pad + s] = (T)((double)r0[q] - (double)mq[q] * c);
+        P[(int64_t)iq[q] * s_pad + s] = (T)((double)p0[q] * ((double)(dq[q] + mq[q]) / (double)dq[q]));
+    }
 }
 
 template <class T>
@@ -422,16 +443,37 @@ __global__ void __launch_bounds__(TPB) k_edge_adds(const T *__restrict__ P, T *_
                                                    int32_t s0, int32_t s1, double alpha)
 {
     const int64_t wave = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (wave >= m * gcount) return;
-    const int64_t e = wave / gcount;
-    const int32_t s = (int32_t)(wave - e * gcount) * 64 + (threadIdx.x & 63);
-    if (s < s0 || s >= s1) return;
-    const int32_t i = efrom[e], to = eto[e];
-    const int32_t d = former[i];
-    if (d == 0) return;
-    if (former[to] == 0 && first[to] != NO_EDGE && e < (int64_t)first[to]) return;
-    const double c = (double)P[(int64_t)i * s_pad + s] / ((double)d * alpha);
-    atomicAdd(R + (int64_t)to * s_pad + s, (T)((1.0 - alpha) * c));
+    const int64_t nblk = (m + EB - 1) / EB;
+    if (wave >= nblk * gcount) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t eb = wave / gcount;
+    const int32_t s = (int32_t)(wave - eb * gcount) * 64 + lane;
+    const bool on = s >= s0 && s < s1;
+    const int64_t el = min(eb * EB + min(lane, EB - 1), m - 1);
+    const int32_t fi = efrom[el], ti = eto[el];
+    const int32_t df = former[fi], dt = former[ti], ft = first[ti];
+    int32_t iq[EB], tq[EB], dq[EB];
+    bool use[EB];
+#pragma unroll
+    for (int q = 0; q < EB; q++) {
+        const int64_t e = eb * EB + q;
+        iq[q] = rdl(fi, q);
+        tq[q] = rdl(ti, q);
+        dq[q] = rdl(df, q);
+        const int32_t dto = rdl(dt, q), f = rdl(ft, q);
+        // the edge moves nothing from a row of former degree 0, and nothing
+        // into a former-degree-0 row before that row's first edge
+        use[q] = on && e < m && dq[q] > 0 && !(dto == 0 && f != NO_EDGE && e < (int64_t)f);
+    }
+    T pv[EB];
+#pragma unroll
+    for (int q = 0; q < EB; q++) pv[q] = P[(int64_t)iq[q] * s_pad + s];
+#pragma unroll
+    for (int q = 0; q < EB; q++) {
+        if (!use[q]) continue;
+        const double c = (double)pv[q] / ((double)dq[q] * alpha);
+        atomicAdd(R + (int64_t)tq[q] * s_pad + s, (T)((1.0 - alpha) * c));
+    }
 }
 
 template <class T>
@@ -445,17 +487,46 @@ __global__ void __launch_bounds__(TPB) k_edge_frontier(const T *__restrict__ R, 
 {
     TSVD_STAGE(sg, list, count, cap, overflow);
     const int64_t wave = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (wave < 2 * m * gcount) {
-        const int64_t e2 = wave / gcount;
-        const int32_t g = (int32_t)(wave - e2 * gcount);
-        const int64_t e = e2 >> 1;
-        const int32_t v = (e2 & 1) ? eto[e] : efrom[e];
-        const int32_t s = g * 64 + (threadIdx.x & 63);
-        const bool out = s >= s0 && s < s1 && deg[v] > 0 && R[(int64_t)v * s_pad + s] > rm;
-        const unsigned long long c = __ballot(out);
-        if (c && (threadIdx.x & 63) == 0) stage_mark(sg, c, (uint32_t)v * (uint32_t)ngroups + (uint32_t)g, cm);
+    const int64_t ends = 2 * m, nblk = (ends + EB - 1) / EB;
+    if (wave < nblk * gcount) {
+        const int lane = threadIdx.x & 63;
+        const int64_t eb = wave / gcount;
+        const int32_t g = (int32_t)(wave - eb * gcount);
+        const int32_t s = g * 64 + lane;
+        const bool on = s >= s0 && s < s1;
+        // end 2e is edge e's source, 2e + 1 its target
+        const int64_t xl = min(eb * EB + min(lane, EB - 1), ends - 1);
+        const int32_t vl = (xl & 1) ? eto[xl >> 1] : efrom[xl >> 1];
+        const int32_t dl = deg[vl];
+        int32_t vq[EB];
+        T rq[EB];
+#pragma unroll
+        for (int q = 0; q < EB; q++) {
+            vq[q] = rdl(vl, q);
+            rq[q] = R[(int64_t)vq[q] * s_pad + s];
+        }
+#pragma unroll
+        for (int q = 0; q < EB; q++) {
+            const bool out = on && eb * EB + q < ends && rdl(dl, q) > 0 && rq[q] > rm;
+            const unsigned long long c = __ballot(out);
+            if (c && lane == 0) stage_mark(sg, c, (uint32_t)vq[q] * (uint32_t)ngroups + (uint32_t)g, cm);
+        }
     }
     stage_flush(sg);
+}
+
+// Negative residues, which storing the positive entries of the state drops,
+// can only sit on the rows an edge-order update lowered: clamping those rows
+// after the push leaves the state the full clamp would.
+template <class T>
+__global__ void __launch_bounds__(TPB) k_clamp_rows(T *__restrict__ R, const int32_t *__restrict__ touched,
+                                                    int32_t ntouched, int32_t s_pad)
+{
+    const int64_t x = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (x >= (int64_t)ntouched * s_pad) return;
+    const int64_t t = x / s_pad;
+    T *r = R + (int64_t)touched[t] * s_pad + (x - t * s_pad);
+    if (*r < T(0)) *r = T(0);
 }
 
 // Relax neighbours nb[j0, j1) of a popped (v, g) item: one coalesced 64-lane
@@ -1309,18 +1380,19 @@ static void update_edges_t(tsvd_ppr *s, int d, const int32_t *src, const int32_t
     if (m > 0)
         hipLaunchKernelGGL(k_first_edge, dim3((unsigned)ceil_div(m, TPB)), dim3(TPB), 0, st, efrom, m, D.former.p,
                            first, 0);
-    const int64_t rw = (int64_t)D.ntouched * gcount;
+    const int64_t rw = ceil_div(D.ntouched, EB) * gcount;   // waves of EB rows, EB edges, EB edge ends
+    const int64_t ew = ceil_div(m, EB) * gcount, xw = ceil_div(2 * m, EB) * gcount;
     if (rw > 0)
         hipLaunchKernelGGL(k_edge_rows<T>, dim3((unsigned)ceil_div(rw, WPB)), dim3(TPB), 0, st, P, R, D.touched.p,
                            D.ntouched, D.deg.p, D.former.p, s->s_pad, gcount, s0, s1, s->alpha, 0);
     if (m > 0)
-        hipLaunchKernelGGL(k_edge_adds<T>, dim3((unsigned)ceil_div(m * gcount, WPB)), dim3(TPB), 0, st, P, R, efrom,
-                           eto, m, D.former.p, first, s->s_pad, gcount, s0, s1, s->alpha);
+        hipLaunchKernelGGL(k_edge_adds<T>, dim3((unsigned)ceil_div(ew, WPB)), dim3(TPB), 0, st, P, R, efrom, eto, m,
+                           D.former.p, first, s->s_pad, gcount, s0, s1, s->alpha);
     if (rw > 0)
         hipLaunchKernelGGL(k_edge_rows<T>, dim3((unsigned)ceil_div(rw, WPB)), dim3(TPB), 0, st, P, R, D.touched.p,
                            D.ntouched, D.deg.p, D.former.p, s->s_pad, gcount, s0, s1, s->alpha, 1);
     if (m > 0) {
-        hipLaunchKernelGGL(k_edge_frontier<T>, dim3((unsigned)ceil_div(2 * m * gcount, WPB)), dim3(TPB), 0, st, R,
+        hipLaunchKernelGGL(k_edge_frontier<T>, dim3((unsigned)ceil_div(xw, WPB)), dim3(TPB), 0, st, R,
                            efrom, eto, m, D.deg.p, s->s_pad, s->ngroups, gcount, s0, s1, (T)s->rmax, s->cm[d].p,
                            s->F[d].p, cnt + d, cap, cnt + 3);
         hipLaunchKernelGGL(k_first_edge, dim3((unsigned)ceil_div(m, TPB)), dim3(TPB), 0, st, efrom, m, D.former.p,
@@ -1415,16 +1487,18 @@ void ppr_read_lane(tsvd_ppr *s, int dir, int what, int32_t lane, double *out)
     TSVD_HIP(hipStreamSynchronize(st));
 }
 
-void ppr_clamp_negative(tsvd_ppr *s, int dir)
+void ppr_clamp_rows(tsvd_ppr *s, int dir)
 {
     hipStream_t st = s->g->stream;
-    const int64_t cells = (int64_t)s->n * s->s_pad;
+    const GraphDir &D = s->g->dir[dir];
+    const int64_t cells = (int64_t)D.ntouched * s->s_pad;
+    if (cells == 0) return;
     if (s->precision == TSVD_FP64)
-        hipLaunchKernelGGL(k_clamp_negative<double>, dim3((unsigned)ceil_div(cells, TPB)), dim3(TPB), 0, st,
-                           (double *)s->R[dir].p, cells);
+        hipLaunchKernelGGL(k_clamp_rows<double>, dim3((unsigned)ceil_div(cells, TPB)), dim3(TPB), 0, st,
+                           (double *)s->R[dir].p, D.touched.p, D.ntouched, s->s_pad);
     else
-        hipLaunchKernelGGL(k_clamp_negative<float>, dim3((unsigned)ceil_div(cells, TPB)), dim3(TPB), 0, st,
-                           (float *)s->R[dir].p, cells);
+        hipLaunchKernelGGL(k_clamp_rows<float>, dim3((unsigned)ceil_div(cells, TPB)), dim3(TPB), 0, st,
+                           (float *)s->R[dir].p, D.touched.p, D.ntouched, s->s_pad);
     TSVD_HIP(hipGetLastError());
 }
 

@@ -119,8 +119,9 @@ void ppr_bind_sources(tsvd_ppr *s, const int32_t *sources, int32_t count);
 void ppr_clear(tsvd_ppr *s, int dir);
 // Host FP64 copy of one lane's reserves (what 0) or residues (1), n values.
 void ppr_read_lane(tsvd_ppr *s, int dir, int what, int32_t lane, double *out);
-// r = max(r, 0) over a direction's residues.
-void ppr_clamp_negative(tsvd_ppr *s, int dir);
+// r = max(r, 0) over the rows the last graph append grew (the only rows an
+// edge-order update can leave negative), every lane.
+void ppr_clamp_rows(tsvd_ppr *s, int dir);
 // The push's input (p, r, pending masks and lists of every direction), saved
 // and put back, so that one push can be timed under several settings from the
 // same start.  A push leaves the next-round masks zero, so these suffice.
