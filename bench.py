@@ -338,8 +338,9 @@ def main():
                     help="requests in the served-vs-plain stream (0: skip that comparison)")
     ap.add_argument("--served-rate", type=float, default=24.0,
                     help="lowest Poisson arrival rate of the saturation streams (1/s); the rate used is at least "
-                         "4x the plain method's measured snapshots/s, above what either method sustains, so that "
-                         "each runs flat out")
+                         "4x the plain method's measured snapshots/s times the ratio of the workload's median "
+                         "batch to the median request, above what either method sustains, so that each runs "
+                         "flat out")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the push autotuner on the first warm-up snapshot and the MFMA tile-width "
                          "autotuner over the initial and warm-up snapshots")
@@ -614,9 +615,11 @@ def main():
         # runs flat out and its requests/s is its capacity (the p99 of an
         # overloaded queue grows with the stream's length, so it is not a
         # steady-state figure)
-        # above both capacities: the plain method's measured snapshot rate bounds
-        # its request rate from above on requests no larger than its batches
-        sat_rate = max(args.served_rate, 4.0 * value)
+        # above both capacities: 4x the plain method's snapshot rate, scaled by
+        # how much smaller the requests are than the batches that rate was
+        # measured on (a step's cost grows with its edges)
+        batch_med = float(np.median([x.size for x, _ in w.snapshots[1:]])) if len(w.snapshots) > 1 else med
+        sat_rate = max(args.served_rate, 4.0 * value * max(1.0, batch_med / (med / (1 << args.scale_down))))
         rows["saturation"] = compare("saturation", sat_rate, args.served_requests, False)
         plain_cap = rows["saturation"]["plain"]["requests_per_s"]
         # steady state: half the plain method's capacity, latencies of the
