@@ -459,7 +459,7 @@ def main():
         j = next_batch()
         if i == 0 and not args.no_tune and fixed is None:
             # the push autotuner: this snapshot's push timed under each setting from the same state
-            cands = [(h, 0.5, k) for h in (256, 1024, 4096, 8192) for k in (1, 8)]
+            cands = [(h, 0.5, k) for h in (128, 256, 4096, 8192) for k in (1, 8)]
             st, chosen, ms = pipe.snapshot_tuned(*batches[j], cands)
             tuning = {"candidates": [{"hub_chunk": h, "ride_frac": r, "items_per_wave": k,
                                       "push_ms": round(float(t), 1)} for (h, r, k), t in zip(cands, ms)],

@@ -288,7 +288,9 @@ tsvd_graph *tsvd_pipeline_graph(tsvd_pipeline *p);
 tsvd_ppr *tsvd_pipeline_ppr(tsvd_pipeline *p);
 /* The project's autotuner for the push's launch shape (no reference
  * counterpart).  Setting: rows longer than hub_chunk neighbours are split
- * across wavefronts (default 256); lanes whose residue is above ride_frac *
+ * across wavefronts (without a setting from the caller or TSVD_HUB_CHUNK,
+ * the first push picks 64 or 128 when the worst-case chunk lists fit in
+ * 4 GiB, else keeps 256); lanes whose residue is above ride_frac *
  * rmax (but not above rmax) push along with a popped item (default 0.5; >= 1:
  * off).  A tuned snapshot
  * is tsvd_pipeline_snapshot, except that its push is first timed under each
@@ -296,7 +298,7 @@ tsvd_ppr *tsvd_pipeline_ppr(tsvd_pipeline *p);
  * lists), then run for real with the fastest, which stays for later
  * snapshots.  cand_ms (may be NULL) gets each candidate's push time.  A
  * snapshot that cannot be timed so (static, sparse_state, or too little free
- * HBM to save the state) runs untuned: settings unchanged, cand_ms -1. */
+ * HBM to save the state) runs untuned: the handle's own setting, cand_ms -1. */
 typedef struct {
     int32_t hub_chunk;
     double ride_frac;

@@ -51,6 +51,10 @@ def test_tuned_snapshot_keeps_a_candidate_and_matches_the_plain_push(capi):
 def test_static_snapshot_is_not_tuned(capi):
     n, snaps, sub = workload()
     p = make(capi, n, sub)
-    before = p.ppr.push_setting
+    before = p.ppr.push_setting             # before any push: the fallback chunk
     st, chosen, ms = p.snapshot_tuned(*snaps[0], CANDS)
-    assert (ms == -1).all() and chosen == before
+    # untimed; the handle keeps its own setting, whose chunk the first push
+    # chose without a caller setting (64 neighbours: this graph's hub buffers
+    # are small)
+    assert (ms == -1).all() and chosen == p.ppr.push_setting
+    assert chosen[1:] == before[1:] and before[0] == 256 and chosen[0] == 64

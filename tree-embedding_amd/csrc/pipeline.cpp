@@ -330,6 +330,7 @@ static void tune_push(tsvd_pipeline *p, const PushTune &t)
         }
     }
     q->hub_chunk = pick.hub_chunk;
+    q->hub_auto = false;
     q->ride_frac = pick.ride_frac;
     q->items_per_wave = pick.items_per_wave;
     ppr_restore(q, sv);

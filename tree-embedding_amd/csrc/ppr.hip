@@ -604,12 +604,16 @@ __device__ inline void relax_range(const gptr<const int32_t> __restrict__ nb, in
 // the first chunk and hands the rest, with its 64 pushed amounts, to
 // k_push_hub (one wave per further chunk), so a hub does not serialise a
 // round behind one wavefront.
-// Default 256: on graphs whose rounds are short (few items per wave) a round
-// lasts as long as its longest chunk walk (measured on the 2.7M-node
-// directed Chung-Lu graph: 256 -> 183 ms of push over 6 snapshots, 1024 ->
-// 262 ms, 4096 -> 421 ms); graphs with long rounds prefer 4096-8192, which
-// the autotuner finds (2.1M-node R-MAT: 4096 is 7% faster than 1024).
-constexpr int HUB_CHUNK = 256;    // default; TSVD_HUB_CHUNK overrides it per PPR handle
+// Chunk length: on graphs whose rounds are short (few items per wave) a round
+// lasts as long as its longest chunk walk, so short chunks win (the 2.7M-node
+// directed Chung-Lu graph: 64 -> 29 ms of push per snapshot, 256 -> 34 ms,
+// 512 -> 40 ms); graphs with long rounds prefer 4096-8192, which the
+// autotuner finds (2.1M-node R-MAT: 8192 is 19% faster than 256).  The hub
+// buffers grow as the chunk shrinks, so without a caller setting the first
+// push takes 64 or 128 when their worst case fits HUB_AUTO_BYTES, else
+// HUB_CHUNK.
+constexpr int HUB_CHUNK = 256;    // fallback; TSVD_HUB_CHUNK overrides it per PPR handle
+constexpr double HUB_AUTO_BYTES = 4.0 * (1ull << 30);   // budget of the automatic chunk's hub buffers
 
 // One synchronous push round: one wave per pending item pops its lanes and
 // relaxes v's neighbours.  The atomic add returns the previous residue, so a
@@ -1120,6 +1124,21 @@ void push_t(tsvd_ppr *s, int d, int32_t s0, int32_t s1)
     uint32_t *hubset[2] = {cnt + 4, cnt + 6};
     const bool pull_enabled = !s->two_sided && s->ngroups <= 64 && s->pull_frac > 0.0;
     bool graphs = s->use_graphs != 0 && !pull_enabled;
+    if (s->hub_auto) {
+        // no setting from the caller: the smallest of 64 and 128 neighbours
+        // whose worst-case hub buffers fit in HUB_AUTO_BYTES, else HUB_CHUNK
+        // (short chunks shorten the longest walk of a round; measured on the
+        // 2.7M-node directed Chung-Lu graph: 64 -> 29 ms of push per snapshot,
+        // 128 -> 31 ms, 256 -> 34 ms, 512 -> 40 ms)
+        s->hub_auto = false;
+        for (const int32_t c : {64, 128}) {
+            const int64_t hc = std::min<int64_t>((int64_t)cap, (int64_t)s->ngroups * (ceil_div(D.cap_total, c) + 1));
+            if ((double)hc * (4.0 + 8.0 + 64.0 * sizeof(T)) <= HUB_AUTO_BYTES) {
+                s->hub_chunk = c;
+                break;
+            }
+        }
+    }
     // split rows: at most one chunk list entry per (hub row, group) per hub_chunk
     // neighbours (rounded up to a power of two in graph mode, so that the
     // capacities, which are launch arguments, change rarely)
@@ -1558,7 +1577,10 @@ extern "C" int tsvd_ppr_create(tsvd_graph *g, const int32_t *sources, int32_t ns
             if (const char *e = getenv("TSVD_RIDE_FRAC")) s->ride_frac = atof(e);
             // rows longer than this many neighbours are split across waves
             s->hub_chunk = HUB_CHUNK;
-            if (const char *e = getenv("TSVD_HUB_CHUNK")) s->hub_chunk = std::max(64, atoi(e));
+            if (const char *e = getenv("TSVD_HUB_CHUNK")) {
+                s->hub_chunk = std::max(64, atoi(e));
+                s->hub_auto = false;
+            }
             // 1: push rounds replayed from captured graphs instead of launched one by one
             if (const char *e = getenv("TSVD_PUSH_GRAPHS")) s->use_graphs = atoi(e) != 0;
             // items a wave takes per round (the round grid shrinks by this factor)
@@ -1651,6 +1673,7 @@ extern "C" int tsvd_ppr_set_push_setting(tsvd_ppr *s, const tsvd_push_setting *i
         TSVD_REQUIRE(in->hub_chunk >= 64 && in->ride_frac >= 0.0 && in->items_per_wave >= 1 &&
                          in->items_per_wave <= 1024, "bad push setting");
         s->hub_chunk = in->hub_chunk;
+        s->hub_auto = false;
         s->ride_frac = in->ride_frac;
         s->items_per_wave = in->items_per_wave;
     });

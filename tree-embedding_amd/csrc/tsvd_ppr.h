@@ -73,6 +73,7 @@ struct tsvd_ppr {
     int64_t wl_n[2] = {0, 0}, wl_version[2] = {-1, -1};
     int two_sided = 0;                       // 1: push |r| > rmax, 0: r > rmax (the reference)
     int32_t hub_chunk = 256;                 // rows longer than this are split across waves
+    bool hub_auto = true;                    // chunk chosen at the first push (no caller setting)
     double pull_frac = 0.0;                  // pull when items > pull_frac * n (0: never)
     double ride_frac = 0.5;                  // sub-threshold lanes above ride_frac*rmax ride along (>= 1: off)
     int32_t items_per_wave = 8;              // items per wave of a round's grid (grid-stride beyond)
