@@ -49,8 +49,8 @@ constexpr int SEG = 48;          // leaf segment rows: one wave holds a segment'
 constexpr int LDS_N = 136;       // tridiagonalization in LDS up to this n
 constexpr int TRI_T = 1024;
 constexpr int TRG_T = 1024;
-constexpr int PANEL_MAX = 8;
-constexpr int VW_CAP = 4096;     // doubles of each panel matrix (V, W) in LDS
+constexpr int PANEL_MAX = 16;
+constexpr int VW_CAP = 8192;     // doubles of each panel matrix (V, W) in LDS (16 columns of n = 512)
 constexpr int LEAF_T = 128;      // wave 0 drives the chase, wave 1 rotates the rows of Z
 constexpr int MRG_T = 512;
 constexpr int BACK_T = 1024;
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(TRG_T) k_tri_panel(double *__restrict__ Hall, 
 }
 
 // trailing block [b0, n)^2 -= V W^T + W V^T (b0 = j0 + np) as 16 x 16 tiles,
-// D = H - [V W] [W V]^T on FP64 MFMA (K = 2 np <= 16): one tile per wave;
+// D = H - [V W] [W V]^T on FP64 MFMA (K = 2 np <= 32): one tile per wave;
 // lane l holds A(l & 15, l >> 4), B(l >> 4, l & 15) and D rows (l >> 4) + 4r
 // of column l & 15
 __global__ void __launch_bounds__(256) k_tri_trail(double *__restrict__ Hall, const double *__restrict__ VWall, int n,
