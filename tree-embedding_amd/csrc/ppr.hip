@@ -315,7 +315,10 @@ __global__ void __launch_bounds__(TPB, 8) k_update(T *__restrict__ P, T *__restr
 #pragma unroll
             for (int q = 0; q < U; q++) {
                 old[q] = T(0);
-                if (active && q0 + q < cnt) old[q] = atomicAdd(R + (int64_t)to[q] * s_pad + s, add[q]);
+                // a source with no reserve at row i adds exactly 0: no atomic (a
+                // half-wave with no non-zero add then touches no cache line)
+                if (active && q0 + q < cnt && add[q] != T(0))
+                    old[q] = atomicAdd(R + (int64_t)to[q] * s_pad + s, add[q]);
             }
 #pragma unroll
             for (int q = 0; q < U; q++) {
@@ -470,7 +473,7 @@ __global__ void __launch_bounds__(TPB) k_edge_adds(const T *__restrict__ P, T *_
     for (int q = 0; q < EB; q++) pv[q] = P[(int64_t)iq[q] * s_pad + s];
 #pragma unroll
     for (int q = 0; q < EB; q++) {
-        if (!use[q]) continue;
+        if (!use[q] || pv[q] == T(0)) continue;   // a zero reserve moves nothing
         const double c = (double)pv[q] / ((double)dq[q] * alpha);
         atomicAdd(R + (int64_t)tq[q] * s_pad + s, (T)((1.0 - alpha) * c));
     }
