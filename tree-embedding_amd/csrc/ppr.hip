@@ -337,8 +337,9 @@ __global__ void __launch_bounds__(TPB, 8) k_update(T *__restrict__ P, T *__restr
     }
     unsigned long long c2 = __ballot(self);
     if (c2 && lane == 0) stage_mark(sg, c2, (uint32_t)i * ngroups + g, cm);
-    // the row's reserves were rescaled: the block norms re-read this item
-    if (__ballot(active) != 0ull && lane == 0) changed[(int64_t)i * ngroups + g] = 1;
+    // the row's reserves were rescaled: the block norms re-read this item (a
+    // group with no reserve at the row is unchanged, so its sums still hold)
+    if (__ballot(active && pv != T(0)) != 0ull && lane == 0) changed[(int64_t)i * ngroups + g] = 1;
     }
     stage_flush(sg);
 }
