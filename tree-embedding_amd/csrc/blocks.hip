@@ -322,29 +322,24 @@ __device__ inline uint64_t mix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
-// Standard normal sketch entries from a counter-based hash (Box-Muller).
+// Standard normal sketch entries from a counter-based hash (Box-Muller), for
+// a batch of matrices in one launch (blockIdx.y = matrix).  Entry (i, c) of a
+// rows x cols matrix is number t = c * rows + i of its seed, stored at
+// X[c * ld + i] (column-major) or X[i * cols + c] (row-major).
 template <class T>
-__global__ void k_gaussian(T *__restrict__ X, int64_t rows, int64_t cols, int64_t ld, uint64_t seed)
+__global__ void k_gaussian(const GaussBatch b)
 {
+    const GaussItem &g = b.it[blockIdx.y];
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= rows * cols) return;
-    const int64_t c = t / rows, i = t - c * rows;
-    const uint64_t h = mix64(seed ^ mix64((uint64_t)t));
+    if (t >= g.rows * g.cols) return;
+    const int64_t c = t / g.rows, i = t - c * g.rows;
+    const uint64_t h = mix64(g.seed ^ mix64((uint64_t)t));
     const double u1 = ((double)(h >> 40) + 1.0) / 16777217.0;           // (0, 1]
     const double u2 = (double)(h & 0xFFFFFFull) / 16777216.0;           // [0, 1)
-    X[c * ld + i] = (T)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
-}
-
-template <class T>
-__global__ void k_gaussian_rm(T *__restrict__ X, int64_t rows, int64_t cols, uint64_t seed)
-{
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= rows * cols) return;
-    const int64_t c = t / rows, i = t - c * rows;
-    const uint64_t h = mix64(seed ^ mix64((uint64_t)t));
-    const double u1 = ((double)(h >> 40) + 1.0) / 16777217.0;
-    const double u2 = (double)(h & 0xFFFFFFull) / 16777216.0;
-    X[i * cols + c] = (T)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+    const T v = (T)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+    T *X = (T *)g.X;
+    if (g.rowmajor) X[i * g.cols + c] = v;
+    else X[c * g.ld + i] = v;
 }
 
 __global__ void k_eig_rotation(double *__restrict__ W, const double *__restrict__ V, const double *__restrict__ lam,
@@ -614,30 +609,23 @@ void lp_scores(const Blocks &b, const double *V, const double *const *ublk, cons
     TSVD_HIP(hipGetLastError());
 }
 
-void gaussian_fill(void *X, int precision, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, hipStream_t st)
+void gaussian_fill(const std::vector<GaussItem> &items, int precision, hipStream_t st)
 {
-    const int64_t tot = rows * cols;
-    if (tot <= 0) return;
-    if (precision == TSVD_FP64)
-        hipLaunchKernelGGL(k_gaussian<double>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st, (double *)X,
-                           rows, cols, ld, seed);
-    else
-        hipLaunchKernelGGL(k_gaussian<float>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st, (float *)X, rows,
-                           cols, ld, seed);
-    TSVD_HIP(hipGetLastError());
-}
-
-void gaussian_fill_rowmajor(void *X, int precision, int64_t rows, int64_t cols, uint64_t seed, hipStream_t st)
-{
-    const int64_t tot = rows * cols;
-    if (tot <= 0) return;
-    if (precision == TSVD_FP64)
-        hipLaunchKernelGGL(k_gaussian_rm<double>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st,
-                           (double *)X, rows, cols, seed);
-    else
-        hipLaunchKernelGGL(k_gaussian_rm<float>, dim3((unsigned)ceil_div(tot, TPB)), dim3(TPB), 0, st, (float *)X,
-                           rows, cols, seed);
-    TSVD_HIP(hipGetLastError());
+    // the batch travels as a kernel argument (GAUSS_BATCH items per launch)
+    for (size_t b0 = 0; b0 < items.size(); b0 += GAUSS_BATCH) {
+        GaussBatch gb{};
+        const int n = (int)std::min<size_t>(GAUSS_BATCH, items.size() - b0);
+        int64_t maxtot = 0;
+        for (int k = 0; k < n; k++) {
+            gb.it[k] = items[b0 + k];
+            maxtot = std::max<int64_t>(maxtot, gb.it[k].rows * gb.it[k].cols);
+        }
+        if (maxtot <= 0) continue;
+        const dim3 grid((unsigned)ceil_div(maxtot, TPB), (unsigned)n);
+        if (precision == TSVD_FP64) hipLaunchKernelGGL(k_gaussian<double>, grid, dim3(TPB), 0, st, gb);
+        else hipLaunchKernelGGL(k_gaussian<float>, grid, dim3(TPB), 0, st, gb);
+        TSVD_HIP(hipGetLastError());
+    }
 }
 
 void eig_rotation(double *W, const double *V, const double *lam, int l, int count, double eps_abs, double eps_rel,

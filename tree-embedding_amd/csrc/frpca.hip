@@ -134,18 +134,15 @@ void frpca_core(std::vector<FrpcaItem> &items, int precision, int32_t l, int32_t
 {
     TSVD_REQUIRE(q >= 1, "frPCA needs q >= 1");
     const DT T = precision == TSVD_FP64 ? DT::F64 : DT::F32;
+    std::vector<GaussItem> sk;
     if (q % 2 == 0) {
-        for (auto &it : items) {
-            if (it.sp < 0) gaussian_fill(it.Qt, precision, it.nc, l, it.ldqt, it.seed, st);
-            else gaussian_fill_rowmajor(it.Qt, precision, it.nc, l, it.seed, st);
-        }
+        for (auto &it : items) sk.push_back(GaussItem{it.Qt, it.nc, l, it.ldqt, it.seed, it.sp >= 0});
+        gaussian_fill(sk, precision, st);
         product(items, precision, l, false, gemm, st, sp);
         orth(items, precision, l, G, W, lam, gemm, eig, st);
     } else {
-        for (auto &it : items) {
-            if (it.sp < 0) gaussian_fill(it.Q, precision, it.m, l, it.ldq, it.seed, st);
-            else gaussian_fill_rowmajor(it.Q, precision, it.m, l, it.seed, st);
-        }
+        for (auto &it : items) sk.push_back(GaussItem{it.Q, it.m, l, it.ldq, it.seed, it.sp >= 0});
+        gaussian_fill(sk, precision, st);
     }
     const int niter = (q - 1) / 2;
     for (int i = 1; i <= niter; i++) {

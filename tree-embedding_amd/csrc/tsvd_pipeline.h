@@ -169,9 +169,20 @@ int32_t tree_update(Tree &t, const std::vector<int32_t> &dirty_leaves, int trans
                     hipStream_t st);
 
 // Elementwise helpers (blocks.hip)
-void gaussian_fill(void *X, int precision, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, hipStream_t st);
-// The same numbers stored row-major (entry (i, c) at X[i * cols + c]).
-void gaussian_fill_rowmajor(void *X, int precision, int64_t rows, int64_t cols, uint64_t seed, hipStream_t st);
+// Standard normal sketches: entry (i, c) of a rows x cols matrix is number
+// c * rows + i of its seed, stored column-major (leading dimension ld) or
+// row-major (entry (i, c) at X[i * cols + c]); one launch per GAUSS_BATCH.
+struct GaussItem {
+    void *X;
+    int64_t rows, cols, ld;
+    uint64_t seed;
+    int32_t rowmajor;
+};
+constexpr int GAUSS_BATCH = 64;
+struct GaussBatch {
+    GaussItem it[GAUSS_BATCH];
+};
+void gaussian_fill(const std::vector<GaussItem> &items, int precision, hipStream_t st);
 // W[:, c] = V[:, c] * scale(c) with scale = 1/sqrt(lam) (zeroed below the
 // guard) for the orthonormalising rotation; batch of `count` l x l matrices.
 void eig_rotation(double *W, const double *V, const double *lam, int l, int count, double eps_abs, double eps_rel,
