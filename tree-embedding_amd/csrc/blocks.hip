@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(TPB) k_block_norms(const T *__restrict__ p0, c
 // The transform and the stored-column lookup are those of k_block_norms, so
 // a block's sums over its items are that kernel's sums in another order.
 constexpr int DONE_SHARDS = 32;
-constexpr int IB = 8;   // flagged items a wave loads together
+constexpr int IB = 8;   // flagged items a wave loads together (the sums' reduce-scatter assumes 8)
 
 template <class T>
 __global__ void __launch_bounds__(TPB) k_item_norms(const T *__restrict__ p0, const T *__restrict__ p1,
@@ -113,6 +113,8 @@ __global__ void __launch_bounds__(TPB) k_item_norms(const T *__restrict__ p0, co
     unsigned long long mine_done = 0;
     for (int64_t w = (int64_t)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6); w * 64 < nitems; w += nw) {
         const int64_t it0 = w * 64, mine = it0 + lane;
+        const int64_t vw = it0 / ngroups;   // the window's first item as (vertex, group)
+        const int32_t gw = (int32_t)(it0 - vw * ngroups);
         const bool want = mine < nitems && (all || changed[mine] != 0);
         if (mine < nitems && (all || want)) changed[mine] = 0;
         unsigned long long bal = __ballot(want);
@@ -135,15 +137,17 @@ __global__ void __launch_bounds__(TPB) k_item_norms(const T *__restrict__ p0, co
             int32_t ss[IB], sl[IB];
 #pragma unroll
             for (int u = 0; u < IB; u++) {
-                const int64_t item = its[u] < 0 ? it0 : its[u];
-                vv[u] = item / ngroups;
-                ss[u] = (int32_t)(item - vv[u] * ngroups) * 64 + lane;
+                // item = (vw, gw) + k, k < 64: 32-bit arithmetic from the window's start
+                const uint32_t idx = (uint32_t)gw + (uint32_t)((its[u] < 0 ? it0 : its[u]) - it0);
+                const uint32_t dv = idx / (uint32_t)ngroups;
+                vv[u] = vw + dv;
+                ss[u] = (int32_t)(idx - dv * (uint32_t)ngroups) * 64 + lane;
                 sl[u] = slot[vv[u]];
             }
             T x[IB], x1[IB], ao[IB];
 #pragma unroll
             for (int u = 0; u < IB; u++) {
-                const int64_t j = min(vv[u] / common, (int64_t)nparts - 1);
+                const int64_t j = min((int64_t)((uint32_t)vv[u] / (uint32_t)common), (int64_t)nparts - 1);
                 x[u] = p0[vv[u] * s_pad + ss[u]];
                 x1[u] = p1 ? p1[vv[u] * s_pad + ss[u]] : T(0);
                 ao[u] = A[(j * common + max(sl[u], 0)) * s_pad + ss[u]];
@@ -154,28 +158,42 @@ __global__ void __launch_bounds__(TPB) k_item_norms(const T *__restrict__ p0, co
                 x[u] = live ? x[u] + x1[u] : T(0);
                 ao[u] = (live && sl[u] >= 0) ? ao[u] : T(0);
             }
-            double a2[IB], e2[IB];
+            // the 2 IB = 16 sums of the batch, value 2u + 0/1 = a2/e2 of item u
+            double sv[2 * IB];
 #pragma unroll
             for (int u = 0; u < IB; u++) {
                 const T a = xform<T>(x[u], rm, transform);
-                a2[u] = (double)a * (double)a;
+                sv[2 * u] = (double)a * (double)a;
                 const double d = (double)a - (double)ao[u];
-                e2[u] = d * d;
+                sv[2 * u + 1] = d * d;
             }
+            // reduce-scatter over the wave: at the step of mask m the lane keeps
+            // the lower or upper half of its values (by its bit m) and adds the
+            // partner's copy of that half; after masks 32..4 lane l holds, for
+            // value (l >> 2) & 15, the sum over the 16 lanes that share l's bits
+            // 0-1, and two plain steps complete it
 #pragma unroll
-            for (int u = 0; u < IB; u++) {
-                a2[u] = wave_sum(a2[u]);
-                e2[u] = wave_sum(e2[u]);
-            }
+            for (int m = 32, c = 2 * IB; m >= 4; m >>= 1, c >>= 1) {
+                const bool upper = (lane & m) != 0;
 #pragma unroll
-            for (int u = 0; u < IB; u++) {
-                if (its[u] < 0) continue;
-                if (lane == 0) {
-                    sums[2 * its[u]] = a2[u];
-                    sums[2 * its[u] + 1] = e2[u];
+                for (int q = 0; q < c / 2; q++) {
+                    const double lo = sv[q], hi = sv[q + c / 2];
+                    const double recv = __shfl_xor(upper ? lo : hi, m, 64);
+                    sv[q] = (upper ? hi : lo) + recv;
                 }
-                mine_done++;
             }
+            double tot = sv[0];
+            tot += __shfl_xor(tot, 2, 64);
+            tot += __shfl_xor(tot, 1, 64);
+            const int vix = (lane >> 2) & 15;   // value held by this lane
+            int64_t myit = its[0];
+#pragma unroll
+            for (int u = 1; u < IB; u++)
+                if ((vix >> 1) == u) myit = its[u];
+            if ((lane & 3) == 0 && myit >= 0) sums[2 * myit + (vix & 1)] = tot;
+#pragma unroll
+            for (int u = 0; u < IB; u++)
+                if (its[u] >= 0) mine_done++;
         }
     }
     if (lane == 0 && mine_done) atomicAdd(&sdone, mine_done);
