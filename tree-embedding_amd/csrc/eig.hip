@@ -70,16 +70,51 @@ struct MergeTable {
     int a[MAX_PIECES], n1[MAX_PIECES], n2[MAX_PIECES];
 };
 
+// v of the lane that DPP control CTRL selects, both 32-bit halves moved
+// (every lane active at the call: the wave reductions below run with the
+// full execution mask)
+template <int CTRL>
+__device__ inline double dpp_f64(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// lane l's value of v, for a constant l
+template <int L>
+__device__ inline double lane_f64(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, L);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), L);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Wave reductions, the same value in every lane: within each row of 16
+// lanes by DPP (quad_perm [1,0,3,2] and [2,3,0,1] pair lanes l ^ 1 and
+// l ^ 2; row_half_mirror pairs the two quads of an 8-lane half, row_mirror
+// the two halves), the operation being commutative so that every lane of a
+// row holds the same row total; then the four row totals in a fixed order.
+// Four VALU steps and four lane reads, where a shuffle butterfly takes six
+// dependent crossbar round trips.
 __device__ inline double wave_sum(double v)
 {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    v += dpp_f64<0x141>(v);
+    v += dpp_f64<0x140>(v);
+    return ((lane_f64<0>(v) + lane_f64<16>(v)) + lane_f64<32>(v)) + lane_f64<48>(v);
 }
 
 __device__ inline double wave_max(double v)
 {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
+    v = fmax(v, dpp_f64<0xB1>(v));
+    v = fmax(v, dpp_f64<0x4E>(v));
+    v = fmax(v, dpp_f64<0x141>(v));
+    v = fmax(v, dpp_f64<0x140>(v));
+    return fmax(fmax(fmax(lane_f64<0>(v), lane_f64<16>(v)), lane_f64<32>(v)), lane_f64<48>(v));
 }
 
 // every thread of the block gets the same sum (fixed order)
