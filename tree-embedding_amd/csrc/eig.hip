@@ -239,13 +239,12 @@ __global__ void __launch_bounds__(TRI_T) k_tridiag_lds(const double *__restrict_
         double pv = 0.0;
         for (int r = r0 + tid; r < n; r += TRI_T) pv += y[r] * v[r];
         const double K = 0.5 * t * t * block_sum_once<TRI_T>(pv, redB);
-        for (int r = r0 + tid; r < n; r += TRI_T) y[r] = t * y[r] - K * v[r];
-        __syncthreads();
-        // A22 -= v w^T + w v^T (both triangles, exactly symmetric)
+        // A22 -= v w^T + w v^T (both triangles), w = t y - K v formed where it is
+        // used, every element by the same explicit fma
         if (grp < ngrp) {
             const int r = r0 + ch * 64 + lane;
             if (r < n) {
-                const double vr = v[r], wr = y[r];
+                const double vr = v[r], wr = fma(t, y[r], -K * vr);
                 int c = r0 + grp;
                 for (; c + 3 * ngrp < n; c += 4 * ngrp) {
                     double h[4];
@@ -254,10 +253,14 @@ __global__ void __launch_bounds__(TRI_T) k_tridiag_lds(const double *__restrict_
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
                         const int cc = c + u * ngrp;
-                        A[cc * n + r] = h[u] - (vr * y[cc] + wr * v[cc]);
+                        const double vc = v[cc], wc = fma(t, y[cc], -K * vc);
+                        A[cc * n + r] = h[u] - (vr * wc + wr * vc);
                     }
                 }
-                for (; c < n; c += ngrp) A[c * n + r] -= vr * y[c] + wr * v[c];
+                for (; c < n; c += ngrp) {
+                    const double vc = v[c], wc = fma(t, y[c], -K * vc);
+                    A[c * n + r] -= vr * wc + wr * vc;
+                }
             }
         }
         __syncthreads();
@@ -318,16 +321,17 @@ __global__ void __launch_bounds__(TRG_T) k_tri_panel(double *__restrict__ Hall, 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = 0; i < np; i++) {
         const int j = j0 + i, r0 = j + 1;
-        // column j brought up to date with the panel's earlier reflectors (rows j..n-1)
+        // column j brought up to date with the panel's earlier reflectors (rows
+        // j..n-1), its squared norm below row r0 summed on the way; the sum's
+        // barrier also publishes xv and the cleared y
+        double sg = 0.0;
         for (int r = j + tid; r < n; r += TRG_T) {
             double x = H[(int64_t)j * ldh + r];
             for (int q = 0; q < i; q++) x -= V[q * n + r] * W[q * n + j] + W[q * n + r] * V[q * n + j];
             xv[r] = x;
             y[r] = 0.0;
+            if (r > r0) sg += x * x;
         }
-        __syncthreads();
-        double sg = 0.0;
-        for (int r = r0 + 1 + tid; r < n; r += TRG_T) sg += xv[r] * xv[r];
         const double sigma = block_sum_once<TRG_T>(sg, redA);
         double t, beta, scale;
         house(xv[r0], sigma, t, beta, scale);
