@@ -695,7 +695,9 @@ __device__ inline void secular_root(const double *dk, const double *z2, int K, d
     if (!last) {
         const double mid = 0.5 * (dk[j] + dk[j + 1]);
         double f = 1.0;
-        for (int i = 0; i < K; i++) f += rho * z2[i] / (dk[i] - mid);
+        // only the sign of f at the midpoint is used (mid lies strictly between
+        // two distinct poles, so no divisor is zero)
+        for (int i = 0; i < K; i++) f += rho * z2[i] * rcp_nr(dk[i] - mid);
         if (f >= 0.0) {
             o = j;
             tlo = 0.0;
